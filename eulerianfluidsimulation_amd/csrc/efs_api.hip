@@ -1,0 +1,544 @@
+// C ABI of the HIP backend (declared in include/efs.h): handle lifecycle, field transfer in the
+// reference layout, and the step orchestration of ISimulator::update (ISimulator.h:12-38) with
+// CPUSimulator3D::project's control flow (CPUSimulator3D.cpp:506-566).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "efs_internal.h"
+
+static thread_local std::string g_err;
+
+void efs_set_error(const std::string& msg) { g_err = msg; }
+int efs_hip_fail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return -(1000 + (int)e);
+}
+
+namespace {
+
+// AoS float[3] <-> three SoA arrays
+__global__ void k_deinterleave(int n, const float* __restrict__ aos, float* __restrict__ a, float* __restrict__ b,
+                               float* __restrict__ c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    a[i] = aos[3 * (size_t)i];
+    b[i] = aos[3 * (size_t)i + 1];
+    c[i] = aos[3 * (size_t)i + 2];
+}
+__global__ void k_interleave(int n, const float* __restrict__ a, const float* __restrict__ b, const float* __restrict__ c,
+                             float* __restrict__ aos) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    aos[3 * (size_t)i] = a[i];
+    aos[3 * (size_t)i + 1] = b[i];
+    aos[3 * (size_t)i + 2] = c[i];
+}
+__global__ void k_fill_f32(int n, float* a, float v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+inline int blocks1d(int n) { return (n + 255) / 256; }
+
+bool valid_handle(efs_sim* s) {
+    if (!s) {
+        efs_set_error("null handle");
+        return false;
+    }
+    return true;
+}
+
+size_t field_bytes(const efs_sim* s, int f) {
+    if (f == EFS_FIELD_VEL || f == EFS_FIELD_FORCES) return (size_t)s->N * 12;
+    return (size_t)s->N * 4;
+}
+
+float* scalar_ptr(efs_sim* s, int f) {
+    switch (f) {
+        case EFS_FIELD_RHO: return s->rho;
+        case EFS_FIELD_GAMMA: return s->gamma;
+        case EFS_FIELD_BETA: return s->beta;
+        case EFS_FIELD_DIVERGENCE: return s->div;
+        case EFS_FIELD_PRESSURE: return s->p;
+        case EFS_FIELD_BOUNDARY: return s->bndf;
+    }
+    return nullptr;
+}
+
+int ensure_forces(efs_sim* s) {
+    if (s->fu) return 0;
+    const size_t b = (size_t)s->N * 4;
+    EFS_HIP(hipMalloc(&s->fu, b));
+    EFS_HIP(hipMalloc(&s->fv, b));
+    EFS_HIP(hipMalloc(&s->fw, b));
+    EFS_HIP(hipMemsetAsync(s->fu, 0, b, s->stream));
+    EFS_HIP(hipMemsetAsync(s->fv, 0, b, s->stream));
+    EFS_HIP(hipMemsetAsync(s->fw, 0, b, s->stream));
+    return 0;
+}
+
+// Moves `bytes` of field f between the library and a caller buffer of the given memcpy kind.
+int transfer(efs_sim* s, int f, void* other, size_t bytes, bool upload, hipMemcpyKind kind) {
+    if (!valid_handle(s)) return -1;
+    if (f < 0 || f > EFS_FIELD_BOUNDARY) {
+        efs_set_error("unknown field id");
+        return -1;
+    }
+    if (!other) {
+        efs_set_error("null buffer");
+        return -1;
+    }
+    if (bytes != field_bytes(s, f)) {
+        efs_set_error("byte count does not match the field size");
+        return -1;
+    }
+    if (f == EFS_FIELD_VEL || f == EFS_FIELD_FORCES) {
+        if (f == EFS_FIELD_FORCES) {
+            int rc = ensure_forces(s);
+            if (rc) return rc;
+        }
+        float* a = f == EFS_FIELD_VEL ? s->u : s->fu;
+        float* b = f == EFS_FIELD_VEL ? s->v : s->fv;
+        float* c = f == EFS_FIELD_VEL ? s->w : s->fw;
+        if (upload) {
+            EFS_HIP(hipMemcpyAsync(s->stage_bytes, other, bytes, kind, s->stream));
+            hipLaunchKernelGGL(k_deinterleave, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->stage_bytes, a, b, c);
+            EFS_HIP(hipGetLastError());
+        } else {
+            hipLaunchKernelGGL(k_interleave, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, a, b, c, s->stage_bytes);
+            EFS_HIP(hipGetLastError());
+            EFS_HIP(hipMemcpyAsync(other, s->stage_bytes, bytes, kind, s->stream));
+        }
+    } else {
+        float* dst = scalar_ptr(s, f);
+        if (upload) EFS_HIP(hipMemcpyAsync(dst, other, bytes, kind, s->stream));
+        else EFS_HIP(hipMemcpyAsync(other, dst, bytes, kind, s->stream));
+        if (upload && f == EFS_FIELD_BOUNDARY) {
+            int rc = efs_launch_boundary_flags(s);
+            if (rc) return rc;
+        }
+    }
+    if (kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToHost) EFS_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+void free_all(efs_sim* s) {
+    float* fl[] = {s->u, s->v, s->w, s->fu, s->fv, s->fw, s->rho, s->gamma, s->beta, s->div, s->p, s->bndf,
+                   s->tu, s->tv, s->tw, s->s0, s->s1, s->bwd_w, s->fwd_w, s->stage_bytes};
+    for (float* p : fl)
+        if (p) hipFree(p);
+    if (s->bf) hipFree(s->bf);
+    if (s->valA) hipFree(s->valA);
+    if (s->valB) hipFree(s->valB);
+    if (s->bwd_p0) hipFree(s->bwd_p0);
+    if (s->fwd_p0) hipFree(s->fwd_p0);
+    if (s->acc0) hipFree(s->acc0);
+    if (s->acc1) hipFree(s->acc1);
+    if (s->acc2) hipFree(s->acc2);
+    efs_learned_free(s);
+    efs_solver_free(s);
+    for (auto& t : s->ktimers) {
+        for (auto e : t.start) hipEventDestroy(e);
+        for (auto e : t.stop) hipEventDestroy(e);
+    }
+    for (auto e : s->ev_stage)
+        if (e) hipEventDestroy(e);
+    if (s->own_stream && s->stream) hipStreamDestroy(s->stream);
+}
+
+int alloc_all(efs_sim* s) {
+    const size_t b = (size_t)s->N * 4;
+    float** fl[] = {&s->u, &s->v, &s->w, &s->rho, &s->gamma, &s->beta, &s->div, &s->p, &s->bndf,
+                    &s->tu, &s->tv, &s->tw, &s->s0, &s->s1};
+    for (float** p : fl) {
+        EFS_HIP(hipMalloc(p, b));
+        EFS_HIP(hipMemset(*p, 0, b));
+    }
+    EFS_HIP(hipMalloc(&s->stage_bytes, b * 3));
+    EFS_HIP(hipMalloc(&s->bf, (size_t)s->N));
+    EFS_HIP(hipMalloc(&s->valA, (size_t)s->N));
+    EFS_HIP(hipMalloc(&s->valB, (size_t)s->N));
+    EFS_HIP(hipMemset(s->valA, 0, (size_t)s->N));
+    EFS_HIP(hipMemset(s->valB, 0, (size_t)s->N));
+    EFS_HIP(hipMalloc(&s->bwd_p0, b));
+    EFS_HIP(hipMalloc(&s->fwd_p0, b));
+    EFS_HIP(hipMemset(s->bwd_p0, 0xFF, b));
+    EFS_HIP(hipMemset(s->fwd_p0, 0xFF, b));
+    EFS_HIP(hipMalloc(&s->bwd_w, b * 8));
+    EFS_HIP(hipMalloc(&s->fwd_w, b * 8));
+    EFS_HIP(hipMalloc(&s->acc0, (size_t)s->N * 8));
+    EFS_HIP(hipMalloc(&s->acc1, (size_t)s->N * 8));
+    EFS_HIP(hipMalloc(&s->acc2, (size_t)s->N * 8));
+    EFS_HIP(hipMemset(s->acc0, 0, (size_t)s->N * 8));
+    EFS_HIP(hipMemset(s->acc1, 0, (size_t)s->N * 8));
+    EFS_HIP(hipMemset(s->acc2, 0, (size_t)s->N * 8));
+    return efs_solver_alloc(s);
+}
+
+// Grid3D ctor (Grid3D.cpp:10-48): gamma = 1, border solid with rho 0.
+int init_grid(efs_sim* s) {
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->gamma, 1.f);
+    std::vector<float> bnd((size_t)s->N, 0.f);
+    for (int z = 0; z < s->D; z++)
+        for (int y = 0; y < s->H; y++)
+            for (int x = 0; x < s->W; x++)
+                if (x == 0 || y == 0 || z == 0 || x == s->W - 1 || y == s->H - 1 || z == s->D - 1)
+                    bnd[((size_t)z * s->H + y) * s->W + x] = 1.f;
+    EFS_HIP(hipMemcpyAsync(s->bndf, bnd.data(), bnd.size() * 4, hipMemcpyHostToDevice, s->stream));
+    int rc = efs_launch_boundary_flags(s);
+    if (rc) return rc;
+    EFS_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+void stage_event(efs_sim* s, int k) {
+    if (!s->timing) return;
+    if (!s->ev_stage[k]) hipEventCreate(&s->ev_stage[k]);
+    hipEventRecord(s->ev_stage[k], s->stream);
+}
+
+int run_project(efs_sim* s, efs_stats* st) {
+    int rc;
+    EFS_HIP(hipMemsetAsync(s->p, 0, (size_t)s->N * 4, s->stream));  // pressure.clear()
+    if ((rc = efs_launch_walls_divergence(s))) return rc;
+    efs_stats local{};
+    efs_stats* out = st ? st : &local;
+    int status;
+    if (s->P.solver_type == EFS_SOLVER_JACOBI) status = efs_jacobi_solve(s, out);
+    else if (s->P.precond == EFS_PRECOND_LEARNED) status = efs_learned_cg_solve(s, out);
+    else status = efs_cg_solve(s, out);
+    if (status < 0) return status;
+    if (status == 1) return 1;  // not converged: velocity keeps the pre-solve state
+    if ((rc = efs_launch_gradient_walls(s))) return rc;
+    return status;
+}
+
+int run_stage(efs_sim* s, int stage, efs_stats* st) {
+    switch (stage) {
+        case EFS_STAGE_VEL_DECAY: return efs_launch_decay(s);
+        case EFS_STAGE_VEL_EXTRAPOLATE: return efs_launch_extrapolate(s);
+        case EFS_STAGE_RHO_SHARPEN: return efs_launch_sharpen(s);
+        case EFS_STAGE_RHO_ADVECT: return efs_launch_rho_advect(s);
+        case EFS_STAGE_VEL_ADVECT: return efs_launch_vel_advect(s);
+        case EFS_STAGE_ADD_FORCE: return efs_launch_add_force(s);
+        case EFS_STAGE_PROJECT: return run_project(s, st);
+    }
+    efs_set_error("unknown stage id");
+    return -1;
+}
+
+// counter-based generator shared with the seeded scenes
+inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline double uniform01(uint64_t seed, uint64_t stream, uint64_t ctr) {
+    const uint64_t key = mix64(seed) ^ (stream * 0xD1B54A32D192ED03ull);
+    return (double)(mix64(key + ctr * 0x9E3779B97F4A7C15ull) >> 11) * 0x1.0p-53;
+}
+inline double normal01(uint64_t seed, uint64_t stream, uint64_t ctr) {
+    const double u1 = uniform01(seed, stream, 2 * ctr);
+    const double u2 = uniform01(seed, stream, 2 * ctr + 1);
+    return std::sqrt(-2.0 * std::log(1.0 - u1)) * std::cos(6.283185307179586 * u2);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* efs_last_error(void) { return g_err.c_str(); }
+const char* efs_version(void) { return "efs-hip gfx950 1.0"; }
+
+void efs_default_params(efs_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->dt = 0.0167f;
+    p->gravity_force = 10.f;
+    p->lambda_minus = 0.5f;
+    p->lambda_plus = 0.1f;
+    p->eta = 1.f;
+    p->liquid_threshold = 0.5f;
+    p->rho_d = 3.f;
+    p->vel_decay = 0.8f;
+    p->gravity_enabled = 1;
+    p->solver_type = EFS_SOLVER_SELF_CG;
+    p->max_steps = 2000;
+    p->solver_threshold = 1e-5f;
+    p->disable_mask = 0;
+    p->criterion = EFS_CRIT_REFERENCE;
+    p->precond = EFS_PRECOND_NONE;
+    p->true_residual = 0;
+}
+
+int efs_create_3d(int32_t width, int32_t height, int32_t depth, const efs_params* params, efs_handle* out) {
+    if (!out) {
+        efs_set_error("null output handle");
+        return -1;
+    }
+    *out = nullptr;
+    if (width < 3 || height < 3 || depth < 3) {
+        efs_set_error("grid must be at least 3 cells per axis (one interior cell plus the border)");
+        return -1;
+    }
+    const long long n = (long long)width * height * depth;
+    if (n >= (1ll << 31) / 8) {
+        efs_set_error("grid too large for 32-bit cell indexing of the 8-entry advection arrays");
+        return -1;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        efs_set_error("no HIP device available");
+        return -3;
+    }
+    efs_sim* s = new efs_sim();
+    s->W = width; s->H = height; s->D = depth; s->N = (int)n;
+    if (params) s->P = *params;
+    else efs_default_params(&s->P);
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        return efs_hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    s->own_stream = true;
+    int rc = alloc_all(s);
+    if (rc == 0) rc = init_grid(s);
+    if (rc) {
+        free_all(s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+int efs_destroy(efs_handle h) {
+    if (!h) return 0;
+    hipStreamSynchronize(h->stream);
+    free_all(h);
+    delete h;
+    return 0;
+}
+
+int efs_set_params(efs_handle h, const efs_params* p) {
+    if (!valid_handle(h) || !p) return -1;
+    if (p->solver_type != EFS_SOLVER_SELF_CG && p->solver_type != EFS_SOLVER_JACOBI) {
+        efs_set_error("solver_type must be SELF_CG (2) or JACOBI (3) for the 3D backend");
+        return -1;
+    }
+    h->P = *p;
+    return 0;
+}
+int efs_get_params(efs_handle h, efs_params* p) {
+    if (!valid_handle(h) || !p) return -1;
+    *p = h->P;
+    return 0;
+}
+int efs_dims(efs_handle h, int32_t dims[3]) {
+    if (!valid_handle(h)) return -1;
+    dims[0] = h->W; dims[1] = h->H; dims[2] = h->D;
+    return 0;
+}
+
+int efs_upload(efs_handle h, int32_t f, const void* host, size_t bytes) {
+    return transfer(h, f, const_cast<void*>(host), bytes, true, hipMemcpyHostToDevice);
+}
+int efs_download(efs_handle h, int32_t f, void* host, size_t bytes) {
+    return transfer(h, f, host, bytes, false, hipMemcpyDeviceToHost);
+}
+int efs_upload_device(efs_handle h, int32_t f, const void* dev, size_t bytes) {
+    return transfer(h, f, const_cast<void*>(dev), bytes, true, hipMemcpyDeviceToDevice);
+}
+int efs_download_device(efs_handle h, int32_t f, void* dev, size_t bytes) {
+    return transfer(h, f, dev, bytes, false, hipMemcpyDeviceToDevice);
+}
+
+int efs_stage(efs_handle h, int32_t stage, efs_stats* st) {
+    if (!valid_handle(h)) return -1;
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->true_rel_residual = -1;
+    }
+    const int rc = run_stage(h, stage, st);
+    if (rc < 0) return rc;
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    if (st) st->status = rc;
+    return rc;
+}
+
+int efs_step(efs_handle h, efs_stats* st) {
+    if (!valid_handle(h)) return -1;
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->true_rel_residual = -1;
+    }
+    int status = 0;
+    stage_event(h, 0);
+    for (int k = 0; k <= EFS_STAGE_PROJECT; k++) {
+        if (!(h->P.disable_mask & (1u << k))) {
+            const int rc = run_stage(h, k, st);
+            if (rc < 0) return rc;
+            if (k == EFS_STAGE_PROJECT) status = rc;
+        }
+        stage_event(h, k + 1);
+    }
+    if (h->timing && st) {
+        EFS_HIP(hipEventSynchronize(h->ev_stage[7]));
+        for (int k = 0; k < 7; k++) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, h->ev_stage[k], h->ev_stage[k + 1]);
+            st->stage_ms[k] = ms;
+        }
+    }
+    if (st) st->status = status;
+    return status;
+}
+
+int efs_solve_pressure(efs_handle h, efs_stats* st) {
+    if (!valid_handle(h)) return -1;
+    efs_stats local{};
+    efs_stats* out = st ? st : &local;
+    std::memset(out, 0, sizeof(*out));
+    out->true_rel_residual = -1;
+    EFS_HIP(hipMemsetAsync(h->p, 0, (size_t)h->N * 4, h->stream));
+    int rc;
+    if (h->P.solver_type == EFS_SOLVER_JACOBI) rc = efs_jacobi_solve(h, out);
+    else if (h->P.precond == EFS_PRECOND_LEARNED) rc = efs_learned_cg_solve(h, out);
+    else rc = efs_cg_solve(h, out);
+    if (rc < 0) return rc;
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    out->status = rc;
+    return rc;
+}
+
+int efs_apply_operator(efs_handle h, const float* x, float* y) {
+    if (!valid_handle(h) || !x || !y) return -1;
+    const int rc = efs_launch_spmv(h, x, y);
+    if (rc) return rc;
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int efs_set_stream(efs_handle h, void* stream) {
+    if (!valid_handle(h)) return -1;
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    if (h->own_stream) hipStreamDestroy(h->stream);
+    h->own_stream = false;
+    h->stream = (hipStream_t)stream;
+    return 0;
+}
+void* efs_get_stream(efs_handle h) { return h ? (void*)h->stream : nullptr; }
+int efs_synchronize(efs_handle h) {
+    if (!valid_handle(h)) return -1;
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+int efs_set_timing(efs_handle h, int32_t on) {
+    if (!valid_handle(h)) return -1;
+    h->timing = on != 0;
+    return 0;
+}
+int efs_set_kernel_timing(efs_handle h, int32_t on) {
+    if (!valid_handle(h)) return -1;
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    for (auto& t : h->ktimers) {
+        for (auto e : t.start) hipEventDestroy(e);
+        for (auto e : t.stop) hipEventDestroy(e);
+        t.start.clear();
+        t.stop.clear();
+        t.launches = 0;
+        t.total_ms = 0;
+    }
+    h->ktiming = on != 0;
+    return 0;
+}
+int efs_kernel_time(efs_handle h, int32_t k, int64_t* launches, double* total_ms) {
+    if (!valid_handle(h) || k < 0 || k > 3) return -1;
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    TimerSlot& t = h->ktimers[k];
+    for (size_t i = 0; i < t.start.size(); i++) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, t.start[i], t.stop[i]);
+        t.total_ms += ms;
+        hipEventDestroy(t.start[i]);
+        hipEventDestroy(t.stop[i]);
+    }
+    t.start.clear();
+    t.stop.clear();
+    if (launches) *launches = t.launches;
+    if (total_ms) *total_ms = t.total_ms;
+    return 0;
+}
+
+int efs_scene_rasterise(efs_handle h, const efs_prim* prims, int32_t nprims) {
+    if (!valid_handle(h) || (nprims > 0 && !prims)) return -1;
+    const int W = h->W, H = h->H, D = h->D;
+    std::vector<float> rho((size_t)h->N), bnd((size_t)h->N);
+    EFS_HIP(hipMemcpyAsync(rho.data(), h->rho, rho.size() * 4, hipMemcpyDeviceToHost, h->stream));
+    EFS_HIP(hipMemcpyAsync(bnd.data(), h->bndf, bnd.size() * 4, hipMemcpyDeviceToHost, h->stream));
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    const double hx = 1.0 / (W - 2), hy = 1.0 / (H - 2), hz = 1.0 / (D - 2);
+    for (int z = 0; z < D; z++)
+        for (int y = 0; y < H; y++)
+            for (int x = 0; x < W; x++) {
+                const size_t c = ((size_t)z * H + y) * W + x;
+                float r = rho[c], b = bnd[c];
+                if (x == 0 || y == 0 || z == 0 || x == W - 1 || y == H - 1 || z == D - 1) {
+                    b = 1.f;
+                    r = 0.f;
+                } else {
+                    const double cx = (x - 0.5) * hx, cy = (y - 0.5) * hy, cz = (z - 0.5) * hz;
+                    for (int k = 0; k < nprims; k++) {
+                        const efs_prim& P = prims[k];
+                        bool in;
+                        if (P.kind == 0) {
+                            in = cx >= P.a[0] && cx < P.b[0] && cy >= P.a[1] && cy < P.b[1] && cz >= P.a[2] && cz < P.b[2];
+                        } else {
+                            const double dx = cx - P.a[0], dy = cy - P.a[1], dz = cz - P.a[2];
+                            in = dx * dx + dy * dy + dz * dz <= P.b[0] * P.b[0];
+                        }
+                        if (!in) continue;
+                        if (P.solid) {
+                            b = 1.f;
+                            r = 0.f;
+                        } else if (b != 1.f) {
+                            r = 1.f;
+                        }
+                    }
+                }
+                rho[c] = r;
+                bnd[c] = b;
+            }
+    int rc = efs_upload(h, EFS_FIELD_RHO, rho.data(), rho.size() * 4);
+    if (rc) return rc;
+    return efs_upload(h, EFS_FIELD_BOUNDARY, bnd.data(), bnd.size() * 4);
+}
+
+int efs_scene_face_noise(efs_handle h, uint64_t seed, double sigma) {
+    if (!valid_handle(h)) return -1;
+    const int W = h->W, H = h->H, D = h->D;
+    std::vector<float> bnd((size_t)h->N), vel((size_t)h->N * 3);
+    EFS_HIP(hipMemcpyAsync(bnd.data(), h->bndf, bnd.size() * 4, hipMemcpyDeviceToHost, h->stream));
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    int rc = efs_download(h, EFS_FIELD_VEL, vel.data(), vel.size() * 4);
+    if (rc) return rc;
+    for (int z = 1; z < D - 1; z++)
+        for (int y = 1; y < H - 1; y++)
+            for (int x = 1; x < W - 1; x++) {
+                const size_t c = ((size_t)z * H + y) * W + x;
+                if (bnd[c] == 1.f) continue;
+                const size_t nb[3] = {c - 1, c - (size_t)W, c - (size_t)W * H};
+                for (int k = 0; k < 3; k++) {
+                    if (bnd[nb[k]] == 1.f) continue;
+                    vel[c * 3 + k] += (float)(sigma * normal01(seed, 1, c * 3 + k));
+                }
+            }
+    return efs_upload(h, EFS_FIELD_VEL, vel.data(), vel.size() * 4);
+}
+
+}  // extern "C"

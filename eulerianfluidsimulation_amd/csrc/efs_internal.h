@@ -1,0 +1,308 @@
+// Internal declarations of the HIP backend: device field layout, launch helpers and the
+// device functions shared by the stage kernels (trace, interpolation, corner weights).
+//
+// Device layout (HBM): structure-of-arrays FP32, cell (x,y,z) at z*W*H + y*W + x, the
+// reference's order (ScalField3D.h:19-25).  The three face-velocity components live in three
+// arrays (u = x-face at the cell's low-x side, v, w), so every stencil reads unit-stride rows.
+// Boundary classification is folded once per upload into a byte per cell (BF_*), so the
+// kernels never re-test the float boundary value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/efs.h"
+
+// boundary byte: the three ways the reference tests the float boundary value
+#define BF_EQ1 1u   // boundary == 1.f       (BOUNDARY tests in CPUSimulator3D.cpp, CGSolver.cpp:471)
+#define BF_GT05 2u  // boundary  > 0.5f      (Grid3D.cpp:79, CGSolver.cpp:484, scatter_values)
+#define BF_GE05 4u  // boundary >= 0.5f      (JacobiSolver.cpp:172)
+
+// solver row code (u16 per cell), rebuilt from rho and the boundary byte before each solve
+#define RC_NB_XM 0x01u
+#define RC_NB_XP 0x02u
+#define RC_NB_YM 0x04u
+#define RC_NB_YP 0x08u
+#define RC_NB_ZP 0x10u
+#define RC_NB_ZM 0x20u
+#define RC_CNT_SHIFT 8
+#define RC_CG_ROW 0x8000u
+#define RC_JAC_ROW 0x4000u
+
+// fixed-point accumulation of scattered float contributions: deterministic regardless of the
+// order the atomics land in.  2^36 gives 1.5e-11 resolution and a range of +-1.3e8.
+#define FIX_SCALE 68719476736.0
+#define FIX_INV (1.0 / 68719476736.0)
+
+struct DevGrid {
+    int W, H, D;
+    int N;
+    float *u, *v, *w;
+    float *fu, *fv, *fw;
+    float *rho, *gamma, *beta, *div, *p;
+    const uint8_t* bf;
+};
+
+__host__ __device__ inline int cidx(const DevGrid& g, int x, int y, int z) { return (z * g.H + y) * g.W + x; }
+
+template <class T> __device__ inline T clampv(T v, T lo, T hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+
+__device__ inline long long to_fix(float v) { return (long long)rint((double)v * FIX_SCALE); }
+__device__ inline void fix_add(unsigned long long* acc, int i, float v) {
+    const long long q = to_fix(v);
+    if (q != 0) atomicAdd(acc + i, (unsigned long long)q);
+}
+__device__ inline double from_fix(unsigned long long a) { return (double)(long long)a * FIX_INV; }
+
+// ---------------------------------------------------------------------------------------
+// VecField3D::interpolate on the staggered grid (VecField3D.cpp:20-86).  Low indices are
+// clamped to [0, n-2]; for the finite positions the traces produce ([1, n-2] - 0.5) the clamp
+// never binds, it only keeps non-finite velocities inside the arrays.
+// ---------------------------------------------------------------------------------------
+__device__ inline int lowIndex(float coord, float offset) { return (int)(floorf(coord - offset) + 1.0e-10f); }
+
+__device__ inline float trilin(const DevGrid& g, const float* __restrict__ f, float px, float py, float pz,
+                               float ox, float oy, float oz) {
+    const int lx = clampv(lowIndex(px, ox), 0, g.W - 2);
+    const int ly = clampv(lowIndex(py, oy), 0, g.H - 2);
+    const int lz = clampv(lowIndex(pz, oz), 0, g.D - 2);
+    const float x0 = (float)lx + ox, y0 = (float)ly + oy, z0 = (float)lz + oz;
+    const float tx = px - x0, ty = py - y0, tz = pz - z0;
+    const int b = cidx(g, lx, ly, lz);
+    const int sy = g.W, sz = g.W * g.H;
+    const float v000 = f[b], v001 = f[b + sz], v010 = f[b + sy], v011 = f[b + sy + sz];
+    const float v100 = f[b + 1], v101 = f[b + 1 + sz], v110 = f[b + 1 + sy], v111 = f[b + 1 + sy + sz];
+    const float v00 = v000 * (1.0f - tz) + v001 * tz;
+    const float v01 = v010 * (1.0f - tz) + v011 * tz;
+    const float v10 = v100 * (1.0f - tz) + v101 * tz;
+    const float v11 = v110 * (1.0f - tz) + v111 * tz;
+    const float vLow = v00 * (1.0f - ty) + v01 * ty;
+    const float vHigh = v10 * (1.0f - ty) + v11 * ty;
+    return vLow * (1.0f - tx) + vHigh * tx;
+}
+
+__device__ inline float3 vinterp(const DevGrid& g, float px, float py, float pz) {
+    float3 r;
+    r.x = trilin(g, g.u, px, py, pz, -1.0f, -0.5f, -0.5f);
+    r.y = trilin(g, g.v, px, py, pz, -0.5f, -1.0f, -0.5f);
+    r.z = trilin(g, g.w, px, py, pz, -0.5f, -0.5f, -1.0f);
+    return r;
+}
+
+// ScalField3D::interpolate (ScalField3D.cpp:18-107): cell values sit at i + 0.5.
+__device__ inline float sinterp(const DevGrid& g, const float* __restrict__ f, float px, float py, float pz) {
+    const int x = (int)px, y = (int)py, z = (int)pz;
+    int ix1 = x, ix2 = x + 1;
+    if (px - (float)x < 0.5f) { ix1 = x - 1; ix2 = x; }
+    int iy1 = y, iy2 = y + 1;
+    if (py - (float)y < 0.5f) { iy1 = y - 1; iy2 = y; }
+    int iz1 = z, iz2 = z + 1;
+    if (pz - (float)z < 0.5f) { iz1 = z - 1; iz2 = z; }
+    ix1 = clampv(ix1, 0, g.W - 1); ix2 = clampv(ix2, 0, g.W - 1);
+    iy1 = clampv(iy1, 0, g.H - 1); iy2 = clampv(iy2, 0, g.H - 1);
+    iz1 = clampv(iz1, 0, g.D - 1); iz2 = clampv(iz2, 0, g.D - 1);
+    const float v0 = f[cidx(g, ix1, iy1, iz1)], v1 = f[cidx(g, ix1, iy1, iz2)];
+    const float v2 = f[cidx(g, ix1, iy2, iz1)], v3 = f[cidx(g, ix1, iy2, iz2)];
+    const float v4 = f[cidx(g, ix2, iy1, iz1)], v5 = f[cidx(g, ix2, iy1, iz2)];
+    const float v6 = f[cidx(g, ix2, iy2, iz1)], v7 = f[cidx(g, ix2, iy2, iz2)];
+    const float x1 = (float)ix1 + 0.5f, x2 = (float)ix2 + 0.5f;
+    const float y1 = (float)iy1 + 0.5f, y2 = (float)iy2 + 0.5f;
+    const float z1 = (float)iz1 + 0.5f, z2 = (float)iz2 + 0.5f;
+    const float dz2z1 = z2 - z1;
+    float f00, f01, f10, f11;
+    if (dz2z1 < 1e-9f) {
+        f00 = v1; f01 = v3; f10 = v5; f11 = v7;
+    } else {
+        const float wz1 = (z2 - pz) / dz2z1, wz2 = (pz - z1) / dz2z1;
+        f00 = wz1 * v0 + wz2 * v1;
+        f01 = wz1 * v2 + wz2 * v3;
+        f10 = wz1 * v4 + wz2 * v5;
+        f11 = wz1 * v6 + wz2 * v7;
+    }
+    const float dy2y1 = y2 - y1;
+    float fx1, fx2;
+    if (dy2y1 < 1e-9f) {
+        fx1 = f01; fx2 = f11;
+    } else {
+        const float wy1 = (y2 - py) / dy2y1, wy2 = (py - y1) / dy2y1;
+        fx1 = wy1 * f00 + wy2 * f01;
+        fx2 = wy1 * f10 + wy2 * f11;
+    }
+    const float dx2x1 = x2 - x1;
+    if (dx2x1 < 1e-9f) return fx2;
+    const float wx1 = (x2 - px) / dx2x1, wx2 = (px - x1) / dx2x1;
+    return wx1 * fx1 + wx2 * fx2;
+}
+
+// Grid3D::traceParticle (Grid3D.cpp:66-122): RK2 with `substeps` substeps, velocity sampled at
+// p - 0.5, midpoint clamped to [1, n-2], 8-step bisection when a step lands in a wall.
+__device__ inline bool inWall(const DevGrid& g, float px, float py, float pz) {
+    const float Wf = (float)g.W, Hf = (float)g.H, Df = (float)g.D;
+    if (px < 1.f || px > Wf - 2.f || py < 1.f || py > Hf - 2.f || pz < 1.f || pz > Df - 2.f) return true;
+    const int ix = clampv((int)px, 0, g.W - 1);
+    const int iy = clampv((int)py, 0, g.H - 1);
+    const int iz = clampv((int)pz, 0, g.D - 1);
+    return (g.bf[cidx(g, ix, iy, iz)] & BF_GT05) != 0;
+}
+
+__device__ inline float3 traceParticle(const DevGrid& g, float sx, float sy, float sz, float dt, int substeps) {
+    if (inWall(g, sx, sy, sz)) return make_float3(sx, sy, sz);
+    const float Wf = (float)g.W, Hf = (float)g.H, Df = (float)g.D;
+    const float subDt = dt / (float)substeps;
+    const float half = subDt * 0.5f;
+    float px = sx, py = sy, pz = sz;
+    for (int s = 0; s < substeps; s++) {
+        const float3 k1 = vinterp(g, px - 0.5f, py - 0.5f, pz - 0.5f);
+        const float mx = clampv(px + k1.x * half, 1.f, Wf - 2.f);
+        const float my = clampv(py + k1.y * half, 1.f, Hf - 2.f);
+        const float mz = clampv(pz + k1.z * half, 1.f, Df - 2.f);
+        const float3 k2 = vinterp(g, mx - 0.5f, my - 0.5f, mz - 0.5f);
+        const float nx = px + k2.x * subDt, ny = py + k2.y * subDt, nz = pz + k2.z * subDt;
+        if (inWall(g, nx, ny, nz)) {
+            float sx_ = px, sy_ = py, sz_ = pz, ux = nx, uy = ny, uz = nz;
+            for (int b = 0; b < 8; b++) {
+                const float hx = (sx_ + ux) * 0.5f, hy = (sy_ + uy) * 0.5f, hz = (sz_ + uz) * 0.5f;
+                if (inWall(g, hx, hy, hz)) { ux = hx; uy = hy; uz = hz; }
+                else { sx_ = hx; sy_ = hy; sz_ = hz; }
+            }
+            return make_float3(sx_, sy_, sz_);
+        }
+        px = nx; py = ny; pz = nz;
+    }
+    return make_float3(px, py, pz);
+}
+
+// Corner weights of a traced position (CPUSimulator3D.cpp:140-163).  m[k] for corners in the
+// reference order 000,100,010,110,001,101,011,111 (x fastest); returns the sum.
+__device__ inline float cornerWeights(const DevGrid& g, float px, float py, float pz, int& p0, float m[8]) {
+    int p0x = (int)floorf(px), p0y = (int)floorf(py), p0z = (int)floorf(pz);
+    p0x = clampv(p0x, 0, g.W - 2); p0y = clampv(p0y, 0, g.H - 2); p0z = clampv(p0z, 0, g.D - 2);
+    const float dx0 = 1 - (px - p0x), dy0 = 1 - (py - p0y), dz0 = 1 - (pz - p0z);
+    const float dx1 = 1 - (p0x + 1 - px), dy1 = 1 - (p0y + 1 - py), dz1 = 1 - (p0z + 1 - pz);
+    p0 = cidx(g, p0x, p0y, p0z);
+    const int sy = g.W, sz = g.W * g.H;
+    const int off[8] = {0, 1, sy, sy + 1, sz, sz + 1, sz + sy, sz + sy + 1};
+    const float wx[2] = {dx0, dx1}, wy[2] = {dy0, dy1}, wz[2] = {dz0, dz1};
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        m[k] = 0.f;
+        if (!(g.bf[p0 + off[k]] & BF_EQ1)) m[k] = wx[k & 1] * wy[(k >> 1) & 1] * wz[(k >> 2) & 1];
+    }
+    float sum = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) sum += m[k];
+    return sum;
+}
+
+__device__ inline int cornerOffset(const DevGrid& g, int k) {
+    return (k & 1) + ((k >> 1) & 1) * g.W + ((k >> 2) & 1) * g.W * g.H;
+}
+
+// ---------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------
+struct TimerSlot {
+    std::vector<hipEvent_t> start, stop;
+    int64_t launches = 0;
+    double total_ms = 0;
+};
+
+struct CGState {
+    double gamma;      // r.r of the current residual
+    double gamma_old;
+    double dAd;        // d.Ad
+    double alpha;
+    double beta;
+    double bnorm2;     // b.b
+    double thr;        // stop when gamma < thr
+    double true_num;   // ||b - A x||^2 (FP64)
+    int iter;          // completed iterations
+    int done;
+    int converged;
+    int max_iter;
+    unsigned int counter;
+    int flags;         // bit0: a border cell is a fluid row
+    int stall;         // learned preconditioner: stall detected
+    int pad;
+};
+
+struct efs_sim {
+    int W, H, D, N;
+    efs_params P;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    // fields
+    float *u = nullptr, *v = nullptr, *w = nullptr, *fu = nullptr, *fv = nullptr, *fw = nullptr;
+    float *rho = nullptr, *gamma = nullptr, *beta = nullptr, *div = nullptr, *p = nullptr, *bndf = nullptr;
+    uint8_t* bf = nullptr;
+    // scratch
+    float *tu = nullptr, *tv = nullptr, *tw = nullptr;   // tmpVec (cell-centred advected velocity)
+    float *s0 = nullptr, *s1 = nullptr;                   // scalar scratch
+    uint8_t *valA = nullptr, *valB = nullptr;             // extrapolation validity
+    int *bwd_p0 = nullptr, *fwd_p0 = nullptr;             // advection matrix corner bases (-1 = none)
+    float *bwd_w = nullptr, *fwd_w = nullptr;             // 8 entries per cell, entry-major
+    unsigned long long *acc0 = nullptr, *acc1 = nullptr, *acc2 = nullptr;  // fixed-point accumulators
+    float* stage_bytes = nullptr;                         // AoS staging for upload/download
+    // solver
+    uint16_t* rowcode = nullptr;
+    float *cg_r = nullptr, *cg_d0 = nullptr, *cg_d1 = nullptr, *cg_x1 = nullptr, *cg_z = nullptr;
+    double* partials = nullptr;
+    int max_blocks = 0;
+    CGState* st = nullptr;
+    CGState* st_host = nullptr;  // pinned
+    // timing
+    bool timing = false;
+    bool ktiming = false;
+    TimerSlot ktimers[4];
+    hipEvent_t ev_stage[8] = {};
+    // learned preconditioner state (owned by the preconditioner module)
+    void* learned = nullptr;
+
+    DevGrid grid() const {
+        DevGrid g;
+        g.W = W; g.H = H; g.D = D; g.N = N;
+        g.u = u; g.v = v; g.w = w; g.fu = fu; g.fv = fv; g.fw = fw;
+        g.rho = rho; g.gamma = gamma; g.beta = beta; g.div = div; g.p = p; g.bf = bf;
+        return g;
+    }
+};
+
+// error reporting
+void efs_set_error(const std::string& msg);
+int efs_hip_fail(hipError_t e, const char* what);
+#define EFS_HIP(call)                                         \
+    do {                                                      \
+        hipError_t _e = (call);                               \
+        if (_e != hipSuccess) return efs_hip_fail(_e, #call); \
+    } while (0)
+
+// stage launchers (efs_step.hip)
+int efs_launch_decay(efs_sim* s);
+int efs_launch_extrapolate(efs_sim* s);
+int efs_launch_sharpen(efs_sim* s);
+int efs_launch_rho_advect(efs_sim* s);
+int efs_launch_vel_advect(efs_sim* s);
+int efs_launch_add_force(efs_sim* s);
+int efs_launch_walls_divergence(efs_sim* s);
+int efs_launch_gradient_walls(efs_sim* s);
+int efs_launch_boundary_flags(efs_sim* s);
+
+// solver launchers (efs_solver.hip)
+int efs_solver_alloc(efs_sim* s);
+void efs_solver_free(efs_sim* s);
+int efs_launch_rowcode(efs_sim* s, int* border_rows);
+int efs_cg_solve(efs_sim* s, efs_stats* st);
+int efs_jacobi_solve(efs_sim* s, efs_stats* st);
+int efs_launch_spmv(efs_sim* s, const float* x, float* y);
+int efs_true_residual(efs_sim* s, double* num, double* den);
+
+// learned preconditioner (efs_learned.hip)
+int efs_learned_cg_solve(efs_sim* s, efs_stats* st);
+void efs_learned_free(efs_sim* s);
+
+enum { KT_CG_DIR_SPMV = 0, KT_CG_UPDATE = 1, KT_JACOBI = 2, KT_OTHER = 3 };
+void efs_ktimer_begin(efs_sim* s, int k, hipEvent_t* a, hipEvent_t* b);
+void efs_ktimer_end(efs_sim* s, int k, hipEvent_t a, hipEvent_t b);

@@ -1,0 +1,631 @@
+// Stage kernels of the 3D liquid step on gfx950, following CPUSimulator3D's semantics
+// (src/simulation/simulator/CPUSimulator3D.cpp).  All float arithmetic is written in the
+// reference's operation order and the library is built without FMA contraction, so the
+// per-cell stages reproduce the CPU path bit for bit on the same inputs.  Two stages are
+// order-dependent on the CPU and are restated for a parallel device:
+//   * rho_sharpen (CPUSimulator3D.cpp:446-455) applies, traces and scatters cell by cell in
+//     place; here every change is applied first, then all traces read the same field, and the
+//     scatters accumulate in fixed point (deterministic in any atomic order).
+//   * rho_advect's sparse matrix A (std::unordered_map, CPUSimulator3D.cpp:133-272) is kept as
+//     8 backward entries per row and 8 forward entries per column; sums over the entries a
+//     cell does not own are scattered in fixed point, the rest are gathered.
+// The density diffusion sweeps (CPUSimulator3D.cpp:276-363) are sequential along a line and
+// independent across lines, so one thread per line reproduces them exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "efs_internal.h"
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+inline dim3 interior_grid(const efs_sim* s) { return dim3((s->W - 2 + BX - 1) / BX, (s->H - 2 + BY - 1) / BY, s->D - 2); }
+inline dim3 full_grid(const efs_sim* s) { return dim3((s->W + BX - 1) / BX, (s->H + BY - 1) / BY, s->D); }
+
+#define INTERIOR_XYZ(g)                                                             \
+    const int x = 1 + blockIdx.x * BX + threadIdx.x, y = 1 + blockIdx.y * BY + threadIdx.y; \
+    const int z = 1 + blockIdx.z;                                                   \
+    if (x > (g).W - 2 || y > (g).H - 2) return;                                     \
+    const int c = cidx((g), x, y, z)
+
+#define FULL_XYZ(g)                                                                 \
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y; \
+    const int z = blockIdx.z;                                                       \
+    if (x >= (g).W || y >= (g).H) return;                                           \
+    const int c = cidx((g), x, y, z)
+
+// ---- boundary byte -----------------------------------------------------------------------
+__global__ void k_boundary_flags(int n, const float* __restrict__ b, uint8_t* __restrict__ bf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = b[i];
+    bf[i] = (uint8_t)((v == 1.f ? BF_EQ1 : 0u) | (v > 0.5f ? BF_GT05 : 0u) | (v >= 0.5f ? BF_GE05 : 0u));
+}
+
+__global__ void k_fill_u64(int n, unsigned long long* a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0ull;
+}
+
+// ---- vel_decay (CPUSimulator3D.cpp:54-59) --------------------------------------------------
+__global__ __launch_bounds__(256) void k_decay(DevGrid g, float decay) {
+    INTERIOR_XYZ(g);
+    if (g.rho[c] < 0.5f) {
+        g.u[c] = g.u[c] * decay;
+        g.v[c] = g.v[c] * decay;
+        g.w[c] = g.w[c] * decay;
+    }
+}
+
+// ---- vel_extrapolate (CPUSimulator3D.cpp:61-111) -------------------------------------------
+__global__ __launch_bounds__(256) void k_extrap_init(DevGrid g, uint8_t* valid) {
+    INTERIOR_XYZ(g);
+    valid[c] = (g.rho[c] >= 0.5f && !(g.bf[c] & BF_EQ1)) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_extrap_pass(DevGrid g, const float* __restrict__ su, const float* __restrict__ sv,
+                                                     const float* __restrict__ sw, const uint8_t* __restrict__ sval,
+                                                     float* __restrict__ du, float* __restrict__ dv, float* __restrict__ dw,
+                                                     uint8_t* __restrict__ dval) {
+    INTERIOR_XYZ(g);
+    if (g.bf[c] & BF_EQ1) {
+        du[c] = su[c]; dv[c] = sv[c]; dw[c] = sw[c];
+        dval[c] = 0;
+        return;
+    }
+    if (sval[c]) {
+        du[c] = su[c]; dv[c] = sv[c]; dw[c] = sw[c];
+        dval[c] = 1;
+        return;
+    }
+    const int sy = g.W, sz = g.W * g.H;
+    const int nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
+    float ax = 0.f, ay = 0.f, az = 0.f, n = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int q = nb[k];
+        if (sval[q]) {
+            ax = ax + su[q]; ay = ay + sv[q]; az = az + sw[q];
+            n++;
+        }
+    }
+    if (n > 0) {
+        du[c] = ax / n; dv[c] = ay / n; dw[c] = az / n;
+        dval[c] = 1;
+    } else {
+        du[c] = su[c]; dv[c] = sv[c]; dw[c] = sw[c];
+        dval[c] = 0;
+    }
+}
+
+// ---- rho_sharpen (CPUSimulator3D.cpp:366-456) ----------------------------------------------
+__device__ inline double dsq(float v) { return (double)v * (double)v; }  // std::pow(float, 2) in double
+__device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }
+__device__ inline float fmaxr(float a, float b) { return (a < b) ? b : a; }  // std::max
+__device__ inline float fminr(float a, float b) { return (b < a) ? b : a; }  // std::min
+
+__global__ __launch_bounds__(256) void k_sharpen_delta(DevGrid g, float delT, float* __restrict__ del) {
+    INTERIOR_XYZ(g);
+    const int sy = g.W, sz = g.W * g.H;
+    const float r = g.rho[c];
+    const float rxm = g.rho[c - 1], rxp = g.rho[c + 1], rym = g.rho[c - sy], ryp = g.rho[c + sy];
+    const float rzm = g.rho[c - sz], rzp = g.rho[c + sz];
+    const float dpx = -(r - rxm) * delT, dmx = -(rxp - r) * delT;
+    const float dpy = -(r - rym) * delT, dmy = -(ryp - r) * delT;
+    const float dpz = -(r - rzm) * delT, dmz = -(rzp - r) * delT;
+    const float max_inc = (float)sqrt(dmax(dsq(fmaxr(dpx, 0.f)), dsq(fminr(dmx, 0.f))) +
+                                      dmax(dsq(fmaxr(dpy, 0.f)), dsq(fminr(dmy, 0.f))) +
+                                      dmax(dsq(fmaxr(dpz, 0.f)), dsq(fminr(dmz, 0.f))));
+    const float max_dec = (float)sqrt(dmax(dsq(fminr(dpx, 0.f)), dsq(fmaxr(dmx, 0.f))) +
+                                      dmax(dsq(fminr(dpy, 0.f)), dsq(fmaxr(dmy, 0.f))) +
+                                      dmax(dsq(fminr(dpz, 0.f)), dsq(fmaxr(dmz, 0.f))));
+    float mx = -INFINITY;
+    mx = fmaxr(mx, fabsf(r - rxm));
+    mx = fmaxr(mx, fabsf(r - rxp));
+    mx = fmaxr(mx, fabsf(r - rym));
+    mx = fmaxr(mx, fabsf(r - ryp));
+    mx = fmaxr(mx, fabsf(r - rzm));
+    mx = fmaxr(mx, fabsf(r - rzp));
+    const float a = r - 0.5f;
+    const float cube = (float)((double)a * (double)a * (double)a);  // powf(a, 3)
+    const float w = cube * (1 - fminr(1.f, mx / 0.2f));
+    float dr = (w >= 0) ? w * max_inc : w * max_dec;
+    if (r + dr < 0 || (double)r < 1e-5) dr = -r;
+    else if (r > 0.5f) dr = 0;
+    del[c] = dr;
+}
+
+__global__ __launch_bounds__(256) void k_sharpen_apply(DevGrid g, const float* __restrict__ del) {
+    INTERIOR_XYZ(g);
+    const float dr = del[c];
+    if (fabsf(dr) < 1e-6f) return;
+    const float r = g.rho[c] + dr;
+    g.rho[c] = fmaxr(r, 0.f);
+}
+
+// trace_along_field (CPUSimulator3D.cpp:568-607)
+__device__ inline float3 trace_along_field(const DevGrid& g, float sx, float sy, float sz, float Dm, int substeps) {
+    const float stepSize = Dm / (float)substeps;
+    const float Wf = (float)g.W, Hf = (float)g.H, Df = (float)g.D;
+    float px = sx, py = sy, pz = sz;
+    const int syo = g.W, szo = g.W * g.H;
+    for (int s = 0; s < substeps; s++) {
+        if (sinterp(g, g.rho, px, py, pz) >= 0.5f) break;
+        const int ix = clampv((int)px, 1, g.W - 2), iy = clampv((int)py, 1, g.H - 2), iz = clampv((int)pz, 1, g.D - 2);
+        const int q = cidx(g, ix, iy, iz);
+        const float gx = g.rho[q + 1] - g.rho[q - 1];
+        const float gy = g.rho[q + syo] - g.rho[q - syo];
+        const float gz = g.rho[q + szo] - g.rho[q - szo];
+        const float len = sqrtf(gx * gx + gy * gy + gz * gz);
+        if (len < 1e-6f) break;
+        const float dx = gx / len, dy = gy / len, dz = gz / len;
+        float nx = px + dx * stepSize, ny = py + dy * stepSize, nz = pz + dz * stepSize;
+        nx = clampv(nx, 1.f, Wf - 2.f);
+        ny = clampv(ny, 1.f, Hf - 2.f);
+        nz = clampv(nz, 1.f, Df - 2.f);
+        if (g.bf[cidx(g, (int)nx, (int)ny, (int)nz)] & BF_GT05) break;
+        px = nx; py = ny; pz = nz;
+    }
+    return make_float3(px, py, pz);
+}
+
+// scatter_values (CPUSimulator3D.cpp:609-655) into a fixed-point accumulator
+__device__ inline void scatter_fix(const DevGrid& g, float px, float py, float pz, float value, unsigned long long* acc) {
+    const int xs[2] = {clampv((int)px, 0, g.W - 1), clampv((int)px + 1, 0, g.W - 1)};
+    const int ys[2] = {clampv((int)py, 0, g.H - 1), clampv((int)py + 1, 0, g.H - 1)};
+    const int zs[2] = {clampv((int)pz, 0, g.D - 1), clampv((int)pz + 1, 0, g.D - 1)};
+    const float fx = px - (float)(int)px, fy = py - (float)(int)py, fz = pz - (float)(int)pz;
+    float w[8];
+    w[0] = (1.f - fx) * (1.f - fy) * (1.f - fz);
+    w[1] = fx * (1.f - fy) * (1.f - fz);
+    w[2] = (1.f - fx) * fy * (1.f - fz);
+    w[3] = fx * fy * (1.f - fz);
+    w[4] = (1.f - fx) * (1.f - fy) * fz;
+    w[5] = fx * (1.f - fy) * fz;
+    w[6] = (1.f - fx) * fy * fz;
+    w[7] = fx * fy * fz;
+    int q[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        q[k] = cidx(g, xs[k & 1], ys[(k >> 1) & 1], zs[(k >> 2) & 1]);
+        if (g.bf[q[k]] & BF_GT05) w[k] = 0.f;
+    }
+    const float total = w[0] + w[1] + w[2] + w[3] + w[4] + w[5] + w[6] + w[7];
+    if (total < 1e-6f) return;
+    const float inv = 1.f / total;
+#pragma unroll
+    for (int k = 0; k < 8; k++) fix_add(acc, q[k], value * w[k] * inv);
+}
+
+__global__ __launch_bounds__(256) void k_sharpen_scatter(DevGrid g, const float* __restrict__ del, float rhoD,
+                                                         unsigned long long* acc) {
+    INTERIOR_XYZ(g);
+    const float dr = del[c];
+    if (fabsf(dr) < 1e-6f) return;
+    const float3 p = trace_along_field(g, (float)x, (float)y, (float)z, rhoD, 10);
+    scatter_fix(g, p.x, p.y, p.z, -dr, acc);
+}
+
+__global__ void k_add_fix(int n, float* __restrict__ f, unsigned long long* __restrict__ acc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long a = acc[i];
+    if (a != 0ull) {
+        f[i] = (float)((double)f[i] + from_fix(a));
+        acc[i] = 0ull;
+    }
+}
+
+// ---- rho_advect (CPUSimulator3D.cpp:113-364) ----------------------------------------------
+// bwd_w[k * N + i] = A(i, p0_i + corner k) for row i; fwd_w[k * N + j] = A(q0_j + corner k, j).
+
+// Steps 1-4: backward trace, advected gamma, row weights scaled by it, column sums (beta).
+__global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __restrict__ gadv, int* __restrict__ bp0,
+                                                 float* __restrict__ bw, unsigned long long* beta_acc) {
+    INTERIOR_XYZ(g);
+    const float3 prev = traceParticle(g, (float)x, (float)y, (float)z, -dt, 4);
+    const float ga = (g.bf[c] & BF_EQ1) ? 0.f : sinterp(g, g.gamma, prev.x, prev.y, prev.z);
+    gadv[c] = ga;
+    float m[8];
+    int p0;
+    const float sum = cornerWeights(g, prev.x, prev.y, prev.z, p0, m);
+    if (sum == 0) {
+        bp0[c] = -1;
+        return;
+    }
+    bp0[c] = p0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        float a = m[k] / sum;
+        if (ga > 1.e-5f) a /= ga;
+        bw[(size_t)k * g.N + c] = a;
+        fix_add(beta_acc, p0 + cornerOffset(g, k), a);
+    }
+}
+
+// Step 5: forward trace for columns with beta < 1; the new entries are scattered into the
+// row sums of step 6 (gamma_acc).
+__global__ __launch_bounds__(256) void k_adv_fwd(DevGrid g, float dt, const unsigned long long* __restrict__ beta_acc,
+                                                 int* __restrict__ fq0, float* __restrict__ fw, unsigned long long* gamma_acc) {
+    INTERIOR_XYZ(g);
+    const float bj = (float)from_fix(beta_acc[c]);
+    if (bj >= 1) {
+        fq0[c] = -1;
+        return;
+    }
+    const float3 next = traceParticle(g, (float)x, (float)y, (float)z, +dt, 4);
+    float m[8];
+    int q0;
+    const float sum = cornerWeights(g, next.x, next.y, next.z, q0, m);
+    if (sum == 0) {
+        fq0[c] = -1;
+        return;
+    }
+    fq0[c] = q0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float a = (1 - bj) * (m[k] / sum);
+        fw[(size_t)k * g.N + c] = a;
+        fix_add(gamma_acc, q0 + cornerOffset(g, k), a);
+    }
+}
+
+// Step 6 (all cells): gamma_i = own backward row + scattered forward entries.
+__global__ __launch_bounds__(256) void k_adv_rowsum(DevGrid g, const int* __restrict__ bp0, const float* __restrict__ bw,
+                                                    unsigned long long* __restrict__ acc, float* __restrict__ out) {
+    FULL_XYZ(g);
+    long long s = (long long)acc[c];
+    acc[c] = 0ull;
+    if (bp0[c] >= 0) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) s += to_fix(bw[(size_t)k * g.N + c]);
+    }
+    out[c] = (float)((double)s * FIX_INV);
+}
+
+// Steps 7-8: divide by gamma (rows), scatter backward entries and add own forward entries into
+// the column sums (beta).
+__global__ __launch_bounds__(256) void k_adv_scale_rows(DevGrid g, const float* __restrict__ g6, const int* __restrict__ bp0,
+                                                        float* __restrict__ bw, const int* __restrict__ fq0,
+                                                        float* __restrict__ fw, unsigned long long* beta_acc) {
+    INTERIOR_XYZ(g);
+    const int p0 = bp0[c];
+    if (p0 >= 0) {
+        const float gi = g6[c];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float a = bw[(size_t)k * g.N + c];
+            if (gi > 1.e-5f) a /= gi;
+            bw[(size_t)k * g.N + c] = a;
+            fix_add(beta_acc, p0 + cornerOffset(g, k), a);
+        }
+    }
+    const int q0 = fq0[c];
+    if (q0 >= 0) {
+        long long own = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float gi = g6[q0 + cornerOffset(g, k)];
+            float a = fw[(size_t)k * g.N + c];
+            if (gi > 1.e-5f) a /= gi;
+            fw[(size_t)k * g.N + c] = a;
+            own += to_fix(a);
+        }
+        if (own != 0) atomicAdd(beta_acc + c, (unsigned long long)own);
+    }
+}
+
+// beta field (all cells) from the fixed-point column sums; clears the accumulator.
+__global__ void k_fix_to_field(int n, unsigned long long* __restrict__ acc, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (float)from_fix(acc[i]);
+    acc[i] = 0ull;
+}
+
+// Steps 9-11 (scatter half): divide by beta (columns); forward entries add to the next row
+// sums (gamma) and to rho' = A rho.
+__global__ __launch_bounds__(256) void k_adv_scale_cols(DevGrid g, const int* __restrict__ bp0, float* __restrict__ bw,
+                                                        const int* __restrict__ fq0, float* __restrict__ fw,
+                                                        unsigned long long* gamma_acc, unsigned long long* rho_acc) {
+    INTERIOR_XYZ(g);
+    const int p0 = bp0[c];
+    if (p0 >= 0) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float bj = g.beta[p0 + cornerOffset(g, k)];
+            float a = bw[(size_t)k * g.N + c];
+            if (bj > 1.e-5f) a /= bj;
+            bw[(size_t)k * g.N + c] = a;
+        }
+    }
+    const int q0 = fq0[c];
+    if (q0 >= 0) {
+        const float bj = g.beta[c];
+        const float rj = g.rho[c];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float a = fw[(size_t)k * g.N + c];
+            if (bj > 1.e-5f) a /= bj;
+            const int i = q0 + cornerOffset(g, k);
+            fix_add(gamma_acc, i, a);
+            fix_add(rho_acc, i, a * rj);
+        }
+    }
+}
+
+// Steps 10-11 (gather half), all cells: gamma = row sums; rho' = A rho on the interior.
+__global__ __launch_bounds__(256) void k_adv_final(DevGrid g, const int* __restrict__ bp0, const float* __restrict__ bw,
+                                                   unsigned long long* __restrict__ gamma_acc,
+                                                   unsigned long long* __restrict__ rho_acc, float* __restrict__ rho_out) {
+    FULL_XYZ(g);
+    long long gs = (long long)gamma_acc[c];
+    long long rs = (long long)rho_acc[c];
+    gamma_acc[c] = 0ull;
+    rho_acc[c] = 0ull;
+    const int p0 = bp0[c];
+    const bool interior = x > 0 && y > 0 && z > 0 && x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
+    if (interior && p0 >= 0) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float a = bw[(size_t)k * g.N + c];
+            gs += to_fix(a);
+            rs += to_fix(a * g.rho[p0 + cornerOffset(g, k)]);
+        }
+    }
+    g.gamma[c] = (float)((double)gs * FIX_INV);
+    rho_out[c] = interior ? (float)((double)rs * FIX_INV) : g.rho[c];
+}
+
+// Step 12: one pair (j = lower neighbour, i = cell) of the in-place diffusion sweep
+// (CPUSimulator3D.cpp:281-303), carried in registers along the line.
+__device__ inline void diffuse_pair(float& rj, float& gj, unsigned bj, float& ri, float& gi, unsigned bi) {
+    if ((bi & BF_EQ1) || (bj & BF_EQ1)) return;
+    if (gj < 1e-5f || gi < 1e-5f) return;
+    if (gj == gi) return;
+    const float avg = (gj + gi) / 2.f;
+    if (gj > gi) {
+        const float dif = rj * (gj - gi) / (2.f * gj);
+        rj -= dif;
+        ri += dif;
+    } else {
+        const float dif = ri * (gi - gj) / (2.f * gi);
+        ri -= dif;
+        rj += dif;
+    }
+    gj = avg;
+    gi = avg;
+}
+
+// axis 0: lines over (y, z); axis 1: lines over (x, z); axis 2: lines over (x, y)
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_diffuse(DevGrid g) {
+    const int a = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // fastest line coordinate
+    const int b = 1 + blockIdx.y;
+    int c0, stride, len;
+    if (AXIS == 0) {  // a = y, b = z
+        if (a > g.H - 2) return;
+        c0 = cidx(g, 1, a, b); stride = 1; len = g.W;
+    } else if (AXIS == 1) {  // a = x, b = z
+        if (a > g.W - 2) return;
+        c0 = cidx(g, a, 1, b); stride = g.W; len = g.H;
+    } else {  // a = x, b = y
+        if (a > g.W - 2) return;
+        c0 = cidx(g, a, b, 1); stride = g.W * g.H; len = g.D;
+    }
+    int cj = c0;
+    float rj = g.rho[cj], gj = g.gamma[cj];
+    unsigned bj = g.bf[cj];
+    for (int k = 2; k < len - 1; k++) {
+        const int ci = cj + stride;
+        float ri = g.rho[ci], gi = g.gamma[ci];
+        const unsigned bi = g.bf[ci];
+        diffuse_pair(rj, gj, bj, ri, gi, bi);
+        g.rho[cj] = rj;
+        g.gamma[cj] = gj;
+        cj = ci; rj = ri; gj = gi; bj = bi;
+    }
+    g.rho[cj] = rj;
+    g.gamma[cj] = gj;
+}
+
+// ---- vel_advect (CPUSimulator3D.cpp:478-504) ----------------------------------------------
+__global__ __launch_bounds__(256) void k_vadv_trace(DevGrid g, float dt, float* __restrict__ tu, float* __restrict__ tv,
+                                                    float* __restrict__ tw) {
+    INTERIOR_XYZ(g);
+    const float3 prev = traceParticle(g, (float)x, (float)y, (float)z, -dt, 4);
+    const float3 v = vinterp(g, prev.x - 0.5f, prev.y - 0.5f, prev.z - 0.5f);
+    tu[c] = v.x;
+    tv[c] = v.y;
+    tw[c] = v.z;
+}
+
+__global__ __launch_bounds__(256) void k_vadv_faces(DevGrid g, const float* __restrict__ tu, const float* __restrict__ tv,
+                                                    const float* __restrict__ tw) {
+    INTERIOR_XYZ(g);
+    const int sy = g.W, sz = g.W * g.H;
+    const bool bs = g.bf[c] & BF_EQ1;
+    g.u[c] = (x == 1 || bs || (g.bf[c - 1] & BF_EQ1)) ? tu[c] : 0.5f * (tu[c] + tu[c - 1]);
+    g.v[c] = (y == 1 || bs || (g.bf[c - sy] & BF_EQ1)) ? tv[c] : 0.5f * (tv[c] + tv[c - sy]);
+    g.w[c] = (z == 1 || bs || (g.bf[c - sz] & BF_EQ1)) ? tw[c] : 0.5f * (tw[c] + tw[c - sz]);
+}
+
+// ---- add_force (CPUSimulator3D.cpp:458-476) -----------------------------------------------
+__global__ __launch_bounds__(256) void k_add_force(DevGrid g, float dt, float gdt, int gravity) {
+    INTERIOR_XYZ(g);
+    if (g.bf[c] & BF_EQ1) return;
+    float u = g.u[c], v = g.v[c], w = g.w[c];
+    if (g.fu) {
+        u = u + g.fu[c] * dt;
+        v = v + g.fv[c] * dt;
+        w = w + g.fw[c] * dt;
+    } else {  // zero forces: vel + 0 * dt
+        u = u + 0.f * dt; v = v + 0.f * dt; w = w + 0.f * dt;
+    }
+    if (gravity) v -= gdt;
+    g.u[c] = u; g.v[c] = v; g.w[c] = w;
+}
+
+// ---- project (CPUSimulator3D.cpp:506-566, 657-679) ----------------------------------------
+// one-sided wall enforcement of the faces stored at cell c (x, y, z >= 1)
+__device__ inline float wall_face(float f, bool self_solid, bool lower_solid) {
+    if (self_solid && f > 0) f = 0.f;
+    if (lower_solid && f < 0) f = 0.f;
+    return f;
+}
+
+// enforce_wall_boundaries + divergence with density correction; divergence cleared elsewhere.
+// The divergence of cell c reads the +x/+y/+z faces, enforced on the fly (enforcement is
+// idempotent, so reading a face another thread has already enforced gives the same value).
+__global__ __launch_bounds__(256) void k_walls_div(DevGrid g, float liq, float lp, float lm, float eta) {
+    FULL_XYZ(g);
+    const int sy = g.W, sz = g.W * g.H;
+    if (x == 0 || y == 0 || z == 0) {
+        g.div[c] = 0.f;
+        return;
+    }
+    const bool s0 = g.bf[c] & BF_EQ1;
+    const float u0 = wall_face(g.u[c], s0, g.bf[c - 1] & BF_EQ1);
+    const float v0 = wall_face(g.v[c], s0, g.bf[c - sy] & BF_EQ1);
+    const float w0 = wall_face(g.w[c], s0, g.bf[c - sz] & BF_EQ1);
+    g.u[c] = u0; g.v[c] = v0; g.w[c] = w0;
+    float dv = 0.f;
+    const bool interior = x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
+    const float r = g.rho[c];
+    if (interior && !(r < liq) && !s0) {
+        const float u1 = wall_face(g.u[c + 1], g.bf[c + 1] & BF_EQ1, s0);
+        const float v1 = wall_face(g.v[c + sy], g.bf[c + sy] & BF_EQ1, s0);
+        const float w1 = wall_face(g.w[c + sz], g.bf[c + sz] & BF_EQ1, s0);
+        const float dx = u1 - u0, dy = v1 - v0, dz = w1 - w0;
+        dv = dx + dy + dz;
+        if (r > 1.f) dv -= fminr(lm * (r - 1.f), eta);
+        if (r < 1.f) dv += fminr(lp * (1.f - r), eta);
+    }
+    g.div[c] = dv;
+}
+
+// u -= grad p on non-solid interior cells (x>1 / y>1 / z>1 guards), then enforce the faces.
+__global__ __launch_bounds__(256) void k_gradient_walls(DevGrid g) {
+    FULL_XYZ(g);
+    if (x == 0 || y == 0 || z == 0) return;
+    const int sy = g.W, sz = g.W * g.H;
+    const bool s0 = g.bf[c] & BF_EQ1;
+    float u = g.u[c], v = g.v[c], w = g.w[c];
+    const bool interior = x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
+    if (interior && !s0) {
+        const float pc = g.p[c];
+        const float gx = (x > 1) ? pc - g.p[c - 1] : 0.f;
+        const float gy = (y > 1) ? pc - g.p[c - sy] : 0.f;
+        const float gz = (z > 1) ? pc - g.p[c - sz] : 0.f;
+        u = u - gx; v = v - gy; w = w - gz;
+    }
+    g.u[c] = wall_face(u, s0, g.bf[c - 1] & BF_EQ1);
+    g.v[c] = wall_face(v, s0, g.bf[c - sy] & BF_EQ1);
+    g.w[c] = wall_face(w, s0, g.bf[c - sz] & BF_EQ1);
+}
+
+inline int blocks1d(int n) { return (n + 255) / 256; }
+
+}  // namespace
+
+// =========================================================================================
+// launchers
+// =========================================================================================
+int efs_launch_boundary_flags(efs_sim* s) {
+    hipLaunchKernelGGL(k_boundary_flags, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->bndf, s->bf);
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_launch_decay(efs_sim* s) {
+    hipLaunchKernelGGL(k_decay, interior_grid(s), dim3(BX, BY), 0, s->stream, s->grid(), s->P.vel_decay);
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_launch_extrapolate(efs_sim* s) {
+    const DevGrid g = s->grid();
+    const dim3 grd = interior_grid(s), blk(BX, BY);
+    hipLaunchKernelGGL(k_extrap_init, grd, blk, 0, s->stream, g, s->valA);
+    for (int i = 0; i < 3; i++) {
+        hipLaunchKernelGGL(k_extrap_pass, grd, blk, 0, s->stream, g, s->u, s->v, s->w, s->valA, s->tu, s->tv, s->tw, s->valB);
+        hipLaunchKernelGGL(k_extrap_pass, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw, s->valB, s->u, s->v, s->w, s->valA);
+    }
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_launch_sharpen(efs_sim* s) {
+    const DevGrid g = s->grid();
+    const dim3 grd = interior_grid(s), blk(BX, BY);
+    const float delT = s->P.dt * 3;
+    hipLaunchKernelGGL(k_sharpen_delta, grd, blk, 0, s->stream, g, delT, s->s0);
+    hipLaunchKernelGGL(k_sharpen_apply, grd, blk, 0, s->stream, g, s->s0);
+    hipLaunchKernelGGL(k_sharpen_scatter, grd, blk, 0, s->stream, g, s->s0, s->P.rho_d, s->acc0);
+    hipLaunchKernelGGL(k_add_fix, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->rho, s->acc0);
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_launch_rho_advect(efs_sim* s) {
+    const DevGrid g = s->grid();
+    const dim3 gi = interior_grid(s), gf = full_grid(s), blk(BX, BY);
+    const dim3 g1(blocks1d(s->N)), b1(256);
+    const float dt = s->P.dt;
+    // steps 1-4
+    hipLaunchKernelGGL(k_adv_bwd, gi, blk, 0, s->stream, g, dt, s->s1, s->bwd_p0, s->bwd_w, s->acc0);
+    // step 5 (+ forward half of step 6)
+    hipLaunchKernelGGL(k_adv_fwd, gi, blk, 0, s->stream, g, dt, s->acc0, s->fwd_p0, s->fwd_w, s->acc1);
+    hipLaunchKernelGGL(k_fill_u64, g1, b1, 0, s->stream, s->N, s->acc0);
+    // step 6: gamma6 -> s1 (the advected gamma of step 1 is no longer needed)
+    hipLaunchKernelGGL(k_adv_rowsum, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->s1);
+    // steps 7-8
+    hipLaunchKernelGGL(k_adv_scale_rows, gi, blk, 0, s->stream, g, s->s1, s->bwd_p0, s->bwd_w, s->fwd_p0, s->fwd_w, s->acc0);
+    hipLaunchKernelGGL(k_fix_to_field, g1, b1, 0, s->stream, s->N, s->acc0, s->beta);
+    // steps 9-11
+    hipLaunchKernelGGL(k_adv_scale_cols, gi, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->fwd_p0, s->fwd_w, s->acc1, s->acc2);
+    hipLaunchKernelGGL(k_adv_final, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->acc2, s->s0);
+    EFS_HIP(hipGetLastError());
+    std::swap(s->rho, s->s0);
+    // step 12: 3 iterations of x, y, z sweeps
+    const DevGrid g2 = s->grid();
+    for (int it = 0; it < 3; it++) {
+        hipLaunchKernelGGL(k_diffuse<0>, dim3((s->H - 2 + 63) / 64, s->D - 2), dim3(64), 0, s->stream, g2);
+        hipLaunchKernelGGL(k_diffuse<1>, dim3((s->W - 2 + 63) / 64, s->D - 2), dim3(64), 0, s->stream, g2);
+        hipLaunchKernelGGL(k_diffuse<2>, dim3((s->W - 2 + 63) / 64, s->H - 2), dim3(64), 0, s->stream, g2);
+    }
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_launch_vel_advect(efs_sim* s) {
+    const DevGrid g = s->grid();
+    const dim3 grd = interior_grid(s), blk(BX, BY);
+    hipLaunchKernelGGL(k_vadv_trace, grd, blk, 0, s->stream, g, s->P.dt, s->tu, s->tv, s->tw);
+    hipLaunchKernelGGL(k_vadv_faces, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw);
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_launch_add_force(efs_sim* s) {
+    const float gdt = s->P.gravity_force * s->P.dt;
+    hipLaunchKernelGGL(k_add_force, interior_grid(s), dim3(BX, BY), 0, s->stream, s->grid(), s->P.dt, gdt,
+                       s->P.gravity_enabled);
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_launch_walls_divergence(efs_sim* s) {
+    hipLaunchKernelGGL(k_walls_div, full_grid(s), dim3(BX, BY), 0, s->stream, s->grid(), s->P.liquid_threshold,
+                       s->P.lambda_plus, s->P.lambda_minus, s->P.eta);
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_launch_gradient_walls(efs_sim* s) {
+    hipLaunchKernelGGL(k_gradient_walls, full_grid(s), dim3(BX, BY), 0, s->stream, s->grid());
+    EFS_HIP(hipGetLastError());
+    return 0;
+}

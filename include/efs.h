@@ -1,0 +1,161 @@
+/*
+ * efs.h — C ABI of the MI355X (gfx950) HIP backend for the Eulerian liquid step.
+ *
+ * The library replaces, for one GPU, the backend slot that the reference fills with
+ * CPUSimulator3D / CudaSimulator3D behind `ISimulator` (reference:
+ * include/simulation/simulator/ISimulator.h:8-53).  Every entry point takes plain pointers and
+ * sizes; fields cross the boundary in the reference layout: `float` per cell for scalar fields
+ * and `float[3]` (AoS, glm::vec3) per cell for velocity and forces, cell (x,y,z) at index
+ * z*W*H + y*W + x (include/simulation/data/ScalField3D.h:19-25).  Device memory holds the fields
+ * as structure-of-arrays; conversion happens inside the library.
+ *
+ * Return codes: 0 ok; 1 pressure solve did not converge (the caller sets PAUSED, as
+ * CPUSimulator3D.cpp:541-547 does); 2 non-finite residual; negative = error (-1 invalid
+ * argument, -2 out of memory, -3 library/extension unavailable, -(1000 + hipError_t) HIP error).
+ * efs_last_error() returns a message for the last failing call on the calling thread.
+ */
+#ifndef EFS_H
+#define EFS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct efs_sim* efs_handle;
+
+/* Field ids: the eight accessors of IGrid3D (include/simulation/data/IGrid.h:52-59). */
+enum efs_field {
+    EFS_FIELD_VEL = 0,        /* float[3] per cell: face velocities u (x-face), v, w      */
+    EFS_FIELD_RHO = 1,
+    EFS_FIELD_GAMMA = 2,
+    EFS_FIELD_BETA = 3,
+    EFS_FIELD_DIVERGENCE = 4,
+    EFS_FIELD_PRESSURE = 5,
+    EFS_FIELD_FORCES = 6,     /* float[3] per cell                                        */
+    EFS_FIELD_BOUNDARY = 7
+};
+
+/* Stages in ISimulator::update order (ISimulator.h:15-35); bit s of disable_mask skips stage s
+ * like the DISABLE_* properties (include/simulation/Properties.h:186-192). */
+enum efs_stage {
+    EFS_STAGE_VEL_DECAY = 0,
+    EFS_STAGE_VEL_EXTRAPOLATE = 1,
+    EFS_STAGE_RHO_SHARPEN = 2,
+    EFS_STAGE_RHO_ADVECT = 3,
+    EFS_STAGE_VEL_ADVECT = 4,
+    EFS_STAGE_ADD_FORCE = 5,
+    EFS_STAGE_PROJECT = 6
+};
+
+/* Solver ids keep the reference enum values (Properties.h:475-481). */
+enum efs_solver { EFS_SOLVER_SELF_CG = 2, EFS_SOLVER_JACOBI = 3 };
+
+/* CG stop criterion. */
+enum efs_criterion {
+    EFS_CRIT_REFERENCE = 0, /* r.r < solver_threshold (CGSolver.cpp:434-456)           */
+    EFS_CRIT_RELATIVE = 1   /* ||r||_2 / ||b||_2 <= solver_threshold                   */
+};
+
+enum efs_precond { EFS_PRECOND_NONE = 0, EFS_PRECOND_LEARNED = 1 };
+
+/* The step's parameter set (Properties.h:136-218).  gravity_force is GRAVITY_FORCE in
+ * cells/s^2 = GRAVITY_FORCE_PHYSICAL / CELL_SIZE (src/Application.cpp:89). */
+typedef struct {
+    float dt;
+    float gravity_force;
+    float lambda_minus;
+    float lambda_plus;
+    float eta;
+    float liquid_threshold;
+    float rho_d;
+    float vel_decay;
+    int32_t gravity_enabled;
+    int32_t solver_type;      /* efs_solver                                                */
+    int32_t max_steps;        /* MAX_STEPS: CG iteration cap; Jacobi runs max_steps/2 passes */
+    float solver_threshold;   /* SOLVER_THRESHOLD                                          */
+    uint32_t disable_mask;    /* bit per efs_stage                                         */
+    int32_t criterion;        /* efs_criterion                                             */
+    int32_t precond;          /* efs_precond                                               */
+    int32_t true_residual;    /* 1 = FP64 true residual ||b - A p|| / ||b|| after each solve */
+} efs_params;
+
+typedef struct {
+    int32_t status;           /* return code of the step / solve                           */
+    int32_t iterations;       /* CG iterations run (Jacobi: passes)                        */
+    double rr;                /* final r.r of the recurrence, FP64 (the reference's returned err) */
+    double rel_residual;      /* sqrt(rr) / ||b||                                          */
+    double true_rel_residual; /* ||b - A p||_2 / ||b||_2 in FP64, or -1 if not computed     */
+    double bnorm;             /* ||b||_2                                                   */
+    int32_t fallback;         /* learned preconditioner fell back to plain CG              */
+    int32_t warm_started;     /* learned warm start used                                   */
+    float stage_ms[7];        /* device time per stage (0 when timing is off)              */
+} efs_stats;
+
+/* Defaults of Properties.h:136-218 with the step's own parameters. */
+void efs_default_params(efs_params* p);
+
+/* Reference: CPUSimulator3D(w,h,d) / CudaSimulator3D ctor + Grid3D ctor (Grid3D.cpp:10-48):
+ * allocates the grid, gamma = 1, border cells solid with rho 0. */
+int efs_create_3d(int32_t width, int32_t height, int32_t depth, const efs_params* params, efs_handle* out);
+/* Reference: CudaSimulator3D::~CudaSimulator3D (CudaSimulator3D.cu:51-80). */
+int efs_destroy(efs_handle h);
+/* Reference: the PROPERTY_* reads each stage performs (Properties.h:16-32). */
+int efs_set_params(efs_handle h, const efs_params* params);
+int efs_get_params(efs_handle h, efs_params* params);
+int efs_dims(efs_handle h, int32_t dims[3]);
+
+/* Reference: CudaGrid3D::upload / download (CudaGrid3D.cu:247-269), one field at a time, in the
+ * reference layout; bytes must equal the field size (N*4, or N*12 for VEL/FORCES). */
+int efs_upload(efs_handle h, int32_t field, const void* host, size_t bytes);
+int efs_download(efs_handle h, int32_t field, void* host, size_t bytes);
+/* Device-to-device variants for callers that keep fields in HBM (same layout). */
+int efs_upload_device(efs_handle h, int32_t field, const void* device_ptr, size_t bytes);
+int efs_download_device(efs_handle h, int32_t field, void* device_ptr, size_t bytes);
+
+/* Reference: ISimulator::update (ISimulator.h:12-38): the seven stages with disable_mask. */
+int efs_step(efs_handle h, efs_stats* stats);
+/* One stage in isolation (the protected stage methods of ISimulator, ISimulator.h:43-50). */
+int efs_stage(efs_handle h, int32_t stage, efs_stats* stats);
+/* Reference: ISolver3D::solve (include/simulation/solver/ISolver.h:40), on the current
+ * divergence; pressure is cleared first as project() does (CPUSimulator3D.cpp:513). */
+int efs_solve_pressure(efs_handle h, efs_stats* stats);
+/* Reference: CGSolverCPU3D::impl_mat_mul (CGSolver.cpp:467-497) on device vectors of N floats. */
+int efs_apply_operator(efs_handle h, const float* x_device, float* y_device);
+
+/* Streams: the library launches on its own non-blocking stream unless the caller sets one
+ * (a hipStream_t passed as void*). */
+int efs_set_stream(efs_handle h, void* hip_stream);
+void* efs_get_stream(efs_handle h);
+int efs_synchronize(efs_handle h);
+/* 1 = record device time per stage into efs_stats.stage_ms (adds event pairs per stage). */
+int efs_set_timing(efs_handle h, int32_t on);
+/* Per-kernel event timing of the dominant CG kernels: 1 = bracket each launch of the tracked
+ * kernels with events; efs_kernel_time reads the accumulated count and milliseconds. */
+int efs_set_kernel_timing(efs_handle h, int32_t on);
+int efs_kernel_time(efs_handle h, int32_t kernel, int64_t* launches, double* total_ms);
+
+/* Seeded scene inputs (scenes are setup, not the step; they replace the editor rasterisation of
+ * box spawns, Editor.cpp:176-182, 246-257).  Primitives are evaluated at interior cell centres
+ * ((i - 0.5) h, h = 1/(n - 2)) in order; a solid primitive sets boundary 1 and rho 0, a fluid
+ * one sets rho 1 on non-solid cells.  The border stays solid. */
+typedef struct {
+    int32_t kind;  /* 0 box [a, b), 1 sphere centre a radius b[0] */
+    int32_t solid;
+    double a[3];
+    double b[3];
+} efs_prim;
+int efs_scene_rasterise(efs_handle h, const efs_prim* prims, int32_t nprims);
+/* N(0, sigma) noise on faces that touch no solid, counter-based from seed (stream 1). */
+int efs_scene_face_noise(efs_handle h, uint64_t seed, double sigma);
+
+const char* efs_last_error(void);
+const char* efs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EFS_H */

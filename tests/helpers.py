@@ -1,0 +1,51 @@
+"""Shared helpers for the parity tests: seeded states from the oracle, and moving a state into
+the HIP backend and into a fresh oracle instance."""
+import numpy as np
+
+from oracle.oracle import Oracle3D, params_from_scene
+
+FIELDS = ["vel", "rho", "gamma", "beta", "divergence", "pressure", "forces", "boundary"]
+
+
+def oracle_state(scene, nsteps, **over):
+    """Runs the FP32 oracle `nsteps` steps from the scene and returns a copy of every field."""
+    o = Oracle3D(*scene.dims, params=params_from_scene(scene, **over))
+    o.load_scene(scene)
+    for _ in range(nsteps):
+        o.step()
+    return {k: o.field(k).copy() for k in FIELDS}
+
+
+def to_oracle(state, scene, dbl=False, **over):
+    o = Oracle3D(*scene.dims, params=params_from_scene(scene, **over), dbl=dbl)
+    for k, v in state.items():
+        o.field(k)[...] = v
+    return o
+
+
+def to_gpu(sim, state):
+    g = sim.grid()
+    for k in ["boundary", "vel", "rho", "gamma", "beta", "divergence", "pressure", "forces"]:
+        g.set(k, state[k])
+
+
+def gpu_params(scene, **over):
+    from eulerianfluidsimulation_amd import Params
+    p = scene.params(Params)
+    p.criterion = 0
+    p.precond = 0
+    p.true_residual = 0
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+def maxdiff(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b))) if a.size else 0.0
+
+
+def nbits_equal(a, b):
+    """True when every element matches exactly (+0 and -0 compare equal)."""
+    return bool(np.array_equal(np.asarray(a), np.asarray(b)))
