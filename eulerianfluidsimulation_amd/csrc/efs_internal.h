@@ -212,6 +212,8 @@ struct TimerSlot {
 
 struct CGState {
     double gamma;      // r.r of the current residual
+    double gam[2];     // r.r by iteration parity (written by block 0 while other blocks read the other slot)
+    double thr_in;     // tolerance as given
     double gamma_old;
     double dAd;        // d.Ad
     double alpha;
@@ -249,7 +251,11 @@ struct efs_sim {
     // solver
     uint16_t* rowcode = nullptr;
     float *cg_r = nullptr, *cg_d0 = nullptr, *cg_d1 = nullptr, *cg_x1 = nullptr, *cg_z = nullptr;
+    float* cg_xd = nullptr;     // FP32 increment of the iterate since the last fold
+    double* cg_x64 = nullptr;   // FP64 iterate (reliable-update accumulator)
+    bool p64_valid = false;     // the last solve left its FP64 iterate in cg_x64
     double* partials = nullptr;
+    double *p_rr = nullptr, *p_bb = nullptr, *p_dad = nullptr;  // per-block partial sums of the CG kernels
     int max_blocks = 0;
     CGState* st = nullptr;
     CGState* st_host = nullptr;  // pinned

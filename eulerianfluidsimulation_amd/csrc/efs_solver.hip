@@ -1,22 +1,32 @@
 // Pressure Poisson solve on gfx950: the matrix-free 7-point operator of CGSolverCPU3D
-// (CGSolver.cpp:467-497), CG with FP32 vectors and FP64 dot products / alpha / beta
-// (CGSolver.cpp:416-465 restated), Jacobi (JacobiSolver.cpp:154-212) and an FP64 true residual.
+// (CGSolver.cpp:467-497), a mixed-precision CG (CGSolver.cpp:416-465 restated), Jacobi
+// (JacobiSolver.cpp:154-212) and an FP64 true residual.
 //
-// Kernel shape: a block is a 64 x 4 tile of an xy plane (one wavefront per row) that marches
-// along z over a chunk of planes, keeping the centre column z-1, z, z+1 in registers; lateral
-// neighbours come from L1/L2.  Blocks are renumbered so each XCD owns a contiguous run of tiles
-// (workgroups are dealt to the 8 XCDs round-robin), which keeps the shared halo rows in one L2.
+// Precision.  Search direction d, residual r and the operator run in FP32; dot products,
+// alpha and beta run in FP64.  The iterate is kept as an FP64 accumulator x64 plus an FP32
+// increment xd: each iteration adds alpha d to xd, and at every residual replacement
+// (i % 64 == 0, where the reference recomputes r = b - A x) xd is folded into x64 and r is
+// recomputed from x64 in FP64.  An FP32 iterate alone cannot reach ||b - A x|| / ||b|| <= 1e-5
+// at 256^3: rounding the exact solution to FP32 already leaves a larger residual.
 //
-// CG iteration = 2 kernels, each with one grid-level FP64 reduction finished by the last block:
-//   K1  d_i = r_i + beta d_{i-1} (computed on the fly for the stencil, stored once), t = A d_i,
-//       partial d.t                          -> last block: alpha = gamma / d.t
-//   K2  t = A d_i recomputed from d_i, x_{i+1} = x_i + alpha d_i,
-//       r_{i+1} = r_i - alpha t  (or b - A x_{i+1} when i % 64 == 0, CGSolver.cpp:444-451),
-//       partial r.r                          -> last block: convergence test, beta, iter++
-// x and d are double-buffered by iteration parity, so the residual replacement can read
-// neighbour x_{i+1} = x_i + alpha d_i without racing with the writes.  Every launch has the same
-// arguments (the iteration lives in device memory), and a converged solve turns the remaining
-// launches of a batch into no-ops, so the host checks for convergence once per batch.
+// Kernel shape.  A block is a 64 x 8 tile of an xy plane (8 wavefronts, one per row) marching
+// along z over a chunk of planes.  The plane's lateral neighbours are staged in LDS with a
+// one-cell halo (double-buffered, one barrier per plane); the z neighbours of the thread's own
+// column stay in registers; the loads for plane z+1 are issued before plane z is computed.
+// Blocks are renumbered so each XCD owns a contiguous run of tiles (workgroups are dealt to
+// the 8 XCDs round-robin), keeping the halo rows shared by neighbouring tiles in one L2.
+//
+// Iteration = 2 kernels.  Neither uses a fence or an atomic: a release fence at device scope
+// writes back the XCD's L2 on every block and cost more than the stencil itself.  Each block
+// stores its partial sum; every block of the next kernel reduces all partials in the same fixed
+// order, so all blocks hold bit-identical FP64 scalars:
+//   K1(i)  gamma_i = sum(r.r partials), convergence test, beta = gamma_i / gamma_{i-1},
+//          d_i = r_i + beta d_{i-1} (formed while staging, stored once), t = A d_i, partial d.t
+//   K2(i)  alpha = gamma_i / sum(d.t partials), xd += alpha d_i, r_{i+1} = r_i - alpha t,
+//          partial r.r
+// On replacement iterations K2 is replaced by a fold (x64 += xd + alpha d) and an FP64 residual
+// kernel.  A converged solve turns the remaining launches of a batch into no-ops, so the host
+// synchronises once per batch of 64 iterations.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,19 +37,19 @@
 
 namespace {
 
-constexpr int TX = 64, TY = 4, NT = TX * TY;
+constexpr int TX = 64;
 
 struct Dom {
     int W, H, D;        // grid dims
     int x0, y0, z0;     // first cell of the launch domain
     int nx, ny, nz;     // extent of the launch domain
-    int tiles_x, tiles_y, chunks_z, zc;  // tiling
-    int nblocks;        // real blocks (the launch may round up to a multiple of 8)
-    int full;           // 1 = domain touches the outer layer (neighbour indices need clamping)
+    int ty;             // tile height (rows per block)
+    int tiles_x, tiles_y, chunks_z, zc;
+    int nblocks;        // real blocks (the launch rounds up to a multiple of 8)
 };
 
 __device__ inline bool block_coords(const Dom& d, int& bx, int& by, int& bz) {
-    // XCD-aware renumbering: hardware block b runs on XCD b % 8; give XCD k the contiguous
+    // XCD-aware renumbering: hardware block b runs on XCD b % 8; XCD k gets the contiguous
     // logical range [k * per, (k + 1) * per).
     const int b = blockIdx.x;
     const int per = (d.nblocks + 7) / 8;
@@ -58,13 +68,13 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// Block sum (256 threads = 4 waves) deposited in partials[b]; the last block to arrive reduces
-// all partials in a fixed order and runs `fin` on thread 0.  Returns true in the last block.
-template <int NV>
-__device__ inline bool grid_reduce(double (&v)[NV], double* partials, unsigned int* counter, int nb, double (&out)[NV]) {
+// Block sum deposited in partials[k * gridDim + block]; the last block to arrive reduces all
+// partials in a fixed order (deterministic).  Returns true in the last block (all threads).
+template <int NT, int NV>
+__device__ inline bool grid_reduce(double (&v)[NV], double* partials, unsigned int* counter, double (&out)[NV]) {
     __shared__ double red[NV][NT / 64];
     __shared__ bool last;
-    const int lane = threadIdx.x + threadIdx.y * TX;
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         const double s = wave_sum(v[k]);
@@ -85,12 +95,12 @@ __device__ inline bool grid_reduce(double (&v)[NV], double* partials, unsigned i
     __syncthreads();
     if (!last) return false;
     __threadfence();
-    // deterministic: each thread sums a fixed strided subset, then a fixed tree
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         double s = 0;
         for (int i = lane; i < (int)gridDim.x; i += NT) s += __builtin_nontemporal_load(&partials[(size_t)k * gridDim.x + i]);
         s = wave_sum(s);
+        __syncthreads();
         if ((lane & 63) == 0) red[k][lane >> 6] = s;
     }
     __syncthreads();
@@ -103,7 +113,6 @@ __device__ inline bool grid_reduce(double (&v)[NV], double* partials, unsigned i
         }
         *counter = 0;
     }
-    (void)nb;
     return true;
 }
 
@@ -121,7 +130,6 @@ __device__ inline float apply_row(unsigned rc, float xc, float xm, float xp, flo
     return sum - (float)((rc >> RC_CNT_SHIFT) & 7u) * xc;
 }
 
-// Same row in FP64 (true residual).
 __device__ inline double apply_row_d(unsigned rc, double xc, double xm, double xp, double ym, double yp, double zp, double zm) {
     if (!(rc & RC_CG_ROW)) return xc;
     double sum = 0.0;
@@ -150,12 +158,32 @@ __device__ inline Nb nb_offsets(const Dom& d, int x, int y, int z) {
 }
 
 // ---------------------------------------------------------------------------------------
+// Plane staging: thread (tx, ty) owns LDS slot (ty+1, tx+1); edge threads also fill the
+// one-cell halo.  Cells outside the array read 0.
+// ---------------------------------------------------------------------------------------
+template <int TY>
+struct Stager {
+    int hx_slot, hy_slot;  // LDS column / row of this thread's halo cell (-1 = none)
+    int hx_off, hy_off;    // array offset of the halo cell from the own cell
+    bool own_ok, hx_ok, hy_ok;
+    __device__ void init(const Dom& d, int x, int y) {
+        const int tx = threadIdx.x, ty = threadIdx.y;
+        own_ok = x < d.W && y < d.H;
+        hx_slot = -1; hy_slot = -1; hx_off = 0; hy_off = 0; hx_ok = false; hy_ok = false;
+        if (tx == 0) { hx_slot = 0; hx_off = -1; hx_ok = x - 1 >= 0 && x - 1 < d.W && y < d.H; }
+        if (tx == TX - 1) { hx_slot = TX + 1; hx_off = 1; hx_ok = x + 1 < d.W && y < d.H; }
+        if (ty == 0) { hy_slot = 0; hy_off = -d.W; hy_ok = y - 1 >= 0 && y - 1 < d.H && x < d.W; }
+        if (ty == TY - 1) { hy_slot = TY + 1; hy_off = d.W; hy_ok = y + 1 < d.H && x < d.W; }
+    }
+};
+
+// ---------------------------------------------------------------------------------------
 // row codes: neighbour counted = in range, not on the outer layer, boundary <= 0.5;
 // contributing = counted and rho > 0.5; CG row = !(rho <= 0.5 || boundary == 1);
 // Jacobi row = !(rho <= 0.5 || boundary >= 0.5).
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_rowcode(DevGrid g, uint16_t* rc, CGState* st) {
-    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y, z = blockIdx.z;
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
     if (x >= g.W || y >= g.H) return;
     const int c = cidx(g, x, y, z);
     const float r = g.rho[c];
@@ -184,172 +212,337 @@ __global__ __launch_bounds__(256) void k_rowcode(DevGrid g, uint16_t* rc, CGStat
 }
 
 // ---------------------------------------------------------------------------------------
-// CG
+// CG.  No kernel on the iteration path uses a fence or an atomic: each block writes its
+// partial sum with a plain store, and every block of the next kernel reduces all partials in
+// the same fixed order (bit-identical scalars in every block).  Block 0 records the scalars
+// in CGState for the host; the iteration index is a launch argument.
 // ---------------------------------------------------------------------------------------
-// init: x0 = b, r0 = b - A b, d0 = r0; reduce r.r and b.b
-__global__ __launch_bounds__(256) void k_cg_init(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
-                                                 float* __restrict__ r, float* __restrict__ d0, CGState* st,
-                                                 double* partials) {
+constexpr int CG_TY = 8, CG_NT = TX * CG_TY;
+
+// sum of the n partials of the previous kernel, identical in every block
+__device__ inline double sum_partials(const double* __restrict__ p, int n, double* red) {
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    double s = 0.0;
+    for (int i = lane; i < n; i += CG_NT) s += p[i];
+    s = wave_sum(s);
+    if ((lane & 63) == 0) red[lane >> 6] = s;
+    __syncthreads();
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < CG_NT / 64; w++) tot += red[w];
+    __syncthreads();
+    return tot;
+}
+
+// this block's total of v, written to p[blockIdx.x] (lane 0)
+__device__ inline void write_partial(double v, double* __restrict__ p, double* red) {
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    v = wave_sum(v);
+    if ((lane & 63) == 0) red[lane >> 6] = v;
+    __syncthreads();
+    if (lane == 0) {
+        double tot = 0.0;
+#pragma unroll
+        for (int w = 0; w < CG_NT / 64; w++) tot += red[w];
+        p[blockIdx.x] = tot;
+    }
+}
+
+// whole array: x64 = b, xd = 0 (identity rows keep x0 = b, CGSolver.cpp:429)
+__global__ void k_cg_setup(int n, const float* __restrict__ b, double* __restrict__ x64, float* __restrict__ xd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x64[i] = (double)b[i];
+    xd[i] = 0.f;
+}
+
+// Convergence test at the start of iteration `it` (gamma_it = r_it . r_it from the partials).
+// Returns true when the solve stops; block 0 records the outcome.
+__device__ inline bool cg_check(CGState* st, int it, const double* __restrict__ p_rr, const double* __restrict__ p_bb,
+                                double* red, double& g_it) {
+    g_it = sum_partials(p_rr, gridDim.x, red);
+    double thr = st->thr;
+    double bb = 0.0;
+    if (it == 0) {
+        thr = st->thr_in;
+        bb = sum_partials(p_bb, gridDim.x, red);
+        if (st->flags & 2) thr = thr * thr * bb;
+    }
+    const bool conv = g_it < thr;
+    const bool stop = conv || !isfinite(g_it) || it >= st->max_iter;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+        st->gam[it & 1] = g_it;
+        st->gamma = g_it;
+        if (it == 0) {
+            st->thr = thr;
+            st->bnorm2 = bb;
+        }
+        if (stop) {
+            st->iter = it;
+            st->converged = conv ? 1 : 0;
+            st->done = 1;
+        }
+    }
+    return stop;
+}
+
+// r0 = b - A b (FP64, rounded), d0 = r0; partials of r.r and b.b
+__global__ __launch_bounds__(CG_NT) void k_cg_init(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
+                                                   float* __restrict__ r, float* __restrict__ d0, double* p_rr, double* p_bb) {
+    __shared__ double red[CG_NT / 64];
     int bx, by, bz;
-    double v[2] = {0.0, 0.0};
+    double vr = 0.0, vb = 0.0;
     if (block_coords(d, bx, by, bz)) {
-        const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * TY + threadIdx.y;
+        const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * d.ty + threadIdx.y;
         const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
         if (x < d.x0 + d.nx && y < d.y0 + d.ny) {
             for (int z = zb; z < ze; z++) {
                 const int c = (z * d.H + y) * d.W + x;
                 const Nb n = nb_offsets(d, x, y, z);
-                const float bc = b[c];
-                const float t = apply_row(rcode[c], bc, b[c + n.xm], b[c + n.xp], b[c + n.ym], b[c + n.yp], b[c + n.zp], b[c + n.zm]);
-                const float rr = bc - t;
+                const double bc = b[c];
+                const double t = apply_row_d(rcode[c], bc, b[c + n.xm], b[c + n.xp], b[c + n.ym], b[c + n.yp], b[c + n.zp], b[c + n.zm]);
+                const float rr = (float)(bc - t);
                 r[c] = rr;
                 d0[c] = rr;
-                v[0] += (double)rr * (double)rr;
-                v[1] += (double)bc * (double)bc;
+                vr += (double)rr * (double)rr;
+                vb += bc * bc;
             }
         }
     }
-    double out[2];
-    if (grid_reduce<2>(v, partials, &st->counter, d.nblocks, out)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            st->gamma = out[0];
-            st->bnorm2 = out[1];
-            st->thr = (st->flags & 2) ? st->thr * st->thr * out[1] : st->thr;
-            st->iter = 0;
-            st->converged = (out[0] < st->thr) ? 1 : 0;
-            st->done = st->converged || st->max_iter <= 0 || !isfinite(out[0]);
-        }
-    }
+    write_partial(vr, p_rr, red);
+    __syncthreads();
+    write_partial(vb, p_bb, red);
 }
 
-// K1: direction update + A d + d.Ad
-__global__ __launch_bounds__(256) void k_cg_dir_spmv(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ r,
-                                                     float* d0, float* d1, CGState* st, double* partials) {
+// K1 (iteration it): convergence test, d_it = r_it + beta d_{it-1}, t = A d_it, partial d.t
+__global__ __launch_bounds__(CG_NT) void k_cg_dir_spmv(Dom d, int it, const uint16_t* __restrict__ rcode, const float* __restrict__ r,
+                                                       float* d0, float* d1, CGState* st, const double* p_rr,
+                                                       const double* p_bb, double* p_dad) {
     if (st->done) return;
-    const int it = st->iter;
-    const double beta = st->beta;
+    __shared__ double red[CG_NT / 64];
+    double g_it;
+    if (cg_check(st, it, p_rr, p_bb, red, g_it)) return;
+    const double beta = (it == 0) ? 0.0 : g_it / st->gam[(it + 1) & 1];
     const bool first = (it == 0);
     float* dnew = (it & 1) ? d1 : d0;
     const float* dold = (it & 1) ? d0 : d1;
-    auto D = [&](int c) -> float {
+    __shared__ float sh[2][CG_TY + 2][TX + 2];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    auto Dv = [&](int c) -> float {
         if (first) return dnew[c];
         return (float)((double)dold[c] * beta + (double)r[c]);
     };
     int bx, by, bz;
-    double v[1] = {0.0};
+    double v = 0.0;
     if (block_coords(d, bx, by, bz)) {
-        const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * TY + threadIdx.y;
+        const int x = d.x0 + bx * TX + tx, y = d.y0 + by * CG_TY + ty;
+        const bool in_dom = x < d.x0 + d.nx && y < d.y0 + d.ny;
+        Stager<CG_TY> S;
+        S.init(d, x, y);
         const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
-        if (x < d.x0 + d.nx && y < d.y0 + d.ny) {
-            const int sz = d.W * d.H;
-            int c = (zb * d.H + y) * d.W + x;
-            float cm = (zb > 0) ? D(c - sz) : 0.f;
-            float cc = D(c);
-            for (int z = zb; z < ze; z++, c += sz) {
-                const Nb n = nb_offsets(d, x, y, z);
-                const float cp = D(c + n.zp);
+        const int sz = d.W * d.H;
+        int c = (zb * d.H + y) * d.W + x;
+        float dm = (zb > 0 && S.own_ok) ? Dv(c - sz) : 0.f;
+        float dc = S.own_ok ? Dv(c) : 0.f;
+        sh[0][ty + 1][tx + 1] = dc;
+        if (S.hx_slot >= 0) sh[0][ty + 1][S.hx_slot] = S.hx_ok ? Dv(c + S.hx_off) : 0.f;
+        if (S.hy_slot >= 0) sh[0][S.hy_slot][tx + 1] = S.hy_ok ? Dv(c + S.hy_off) : 0.f;
+        __syncthreads();
+        int cur = 0;
+        for (int z = zb; z < ze; z++, c += sz) {
+            const bool has_next = z + 1 < d.D;
+            const float np = (has_next && S.own_ok) ? Dv(c + sz) : 0.f;
+            const float nhx = (has_next && S.hx_ok) ? Dv(c + sz + S.hx_off) : 0.f;
+            const float nhy = (has_next && S.hy_ok) ? Dv(c + sz + S.hy_off) : 0.f;
+            if (in_dom) {
                 const unsigned rc = rcode[c];
-                float t = cc;
-                if (rc & RC_CG_ROW) t = apply_row(rc, cc, D(c + n.xm), D(c + n.xp), D(c + n.ym), D(c + n.yp), cp, cm);
-                if (!first) dnew[c] = cc;
-                v[0] += (double)cc * (double)t;
-                cm = cc;
-                cc = cp;
+                float t = dc;
+                if (rc & RC_CG_ROW)
+                    t = apply_row(rc, dc, sh[cur][ty + 1][tx], sh[cur][ty + 1][tx + 2], sh[cur][ty][tx + 1],
+                                  sh[cur][ty + 2][tx + 1], np, dm);
+                if (!first) dnew[c] = dc;
+                v += (double)dc * (double)t;
             }
+            sh[cur ^ 1][ty + 1][tx + 1] = np;
+            if (S.hx_slot >= 0) sh[cur ^ 1][ty + 1][S.hx_slot] = nhx;
+            if (S.hy_slot >= 0) sh[cur ^ 1][S.hy_slot][tx + 1] = nhy;
+            __syncthreads();
+            cur ^= 1;
+            dm = dc;
+            dc = np;
         }
     }
-    double out[1];
-    if (grid_reduce<1>(v, partials, &st->counter, d.nblocks, out)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            st->dAd = out[0];
-            st->alpha = st->gamma / out[0];
-        }
-    }
+    write_partial(v, p_dad, red);
+    if (blockIdx.x == 0 && tx == 0 && ty == 0) st->beta = beta;
 }
 
-// K2: x, r update + r.r
-__global__ __launch_bounds__(256) void k_cg_update(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
-                                                   float* r, const float* d0, const float* d1, float* x0b, float* x1b,
-                                                   CGState* st, double* partials) {
+// K2 (iteration it): alpha = gamma / d.t, xd += alpha d, r -= alpha A d, partial r.r
+__global__ __launch_bounds__(CG_NT) void k_cg_update(Dom d, int it, const uint16_t* __restrict__ rcode, float* r, const float* d0,
+                                                     const float* d1, float* xd, CGState* st, const double* p_dad,
+                                                     double* p_rr) {
     if (st->done) return;
-    const int it = st->iter;
-    const double alpha = st->alpha;
+    __shared__ double red[CG_NT / 64];
+    const double alpha = st->gam[it & 1] / sum_partials(p_dad, gridDim.x, red);
     const float* dd = (it & 1) ? d1 : d0;
-    const float* xin = (it & 1) ? x1b : x0b;
-    float* xout = (it & 1) ? x0b : x1b;
-    const bool replace = (it & 0x3F) == 0;
-    auto X = [&](int c) -> float { return (float)((double)xin[c] + alpha * (double)dd[c]); };
+    __shared__ float sh[2][CG_TY + 2][TX + 2];
+    const int tx = threadIdx.x, ty = threadIdx.y;
     int bx, by, bz;
-    double v[1] = {0.0};
+    double v = 0.0;
     if (block_coords(d, bx, by, bz)) {
-        const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * TY + threadIdx.y;
+        const int x = d.x0 + bx * TX + tx, y = d.y0 + by * CG_TY + ty;
+        const bool in_dom = x < d.x0 + d.nx && y < d.y0 + d.ny;
+        Stager<CG_TY> S;
+        S.init(d, x, y);
         const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
-        if (x < d.x0 + d.nx && y < d.y0 + d.ny) {
-            const int sz = d.W * d.H;
-            int c = (zb * d.H + y) * d.W + x;
-            if (!replace) {
-                float cm = (zb > 0) ? dd[c - sz] : 0.f;
-                float cc = dd[c];
-                for (int z = zb; z < ze; z++, c += sz) {
-                    const Nb n = nb_offsets(d, x, y, z);
-                    const float cp = dd[c + n.zp];
-                    const unsigned rc = rcode[c];
-                    float t = cc;
-                    if (rc & RC_CG_ROW) t = apply_row(rc, cc, dd[c + n.xm], dd[c + n.xp], dd[c + n.ym], dd[c + n.yp], cp, cm);
-                    xout[c] = (float)((double)xin[c] + alpha * (double)cc);
-                    const float rn = (float)((double)r[c] - alpha * (double)t);
-                    r[c] = rn;
-                    v[0] += (double)rn * (double)rn;
-                    cm = cc;
-                    cc = cp;
-                }
-            } else {
-                float cm = (zb > 0) ? X(c - sz) : 0.f;
-                float cc = X(c);
-                for (int z = zb; z < ze; z++, c += sz) {
-                    const Nb n = nb_offsets(d, x, y, z);
-                    const float cp = X(c + n.zp);
-                    const unsigned rc = rcode[c];
-                    float t = cc;
-                    if (rc & RC_CG_ROW) t = apply_row(rc, cc, X(c + n.xm), X(c + n.xp), X(c + n.ym), X(c + n.yp), cp, cm);
-                    xout[c] = cc;
-                    const float rn = b[c] - t;
-                    r[c] = rn;
-                    v[0] += (double)rn * (double)rn;
-                    cm = cc;
-                    cc = cp;
-                }
+        const int sz = d.W * d.H;
+        int c = (zb * d.H + y) * d.W + x;
+        float dm = (zb > 0 && S.own_ok) ? dd[c - sz] : 0.f;
+        float dc = S.own_ok ? dd[c] : 0.f;
+        sh[0][ty + 1][tx + 1] = dc;
+        if (S.hx_slot >= 0) sh[0][ty + 1][S.hx_slot] = S.hx_ok ? dd[c + S.hx_off] : 0.f;
+        if (S.hy_slot >= 0) sh[0][S.hy_slot][tx + 1] = S.hy_ok ? dd[c + S.hy_off] : 0.f;
+        __syncthreads();
+        int cur = 0;
+        for (int z = zb; z < ze; z++, c += sz) {
+            const bool has_next = z + 1 < d.D;
+            const float np = (has_next && S.own_ok) ? dd[c + sz] : 0.f;
+            const float nhx = (has_next && S.hx_ok) ? dd[c + sz + S.hx_off] : 0.f;
+            const float nhy = (has_next && S.hy_ok) ? dd[c + sz + S.hy_off] : 0.f;
+            if (in_dom) {
+                const unsigned rc = rcode[c];
+                const float xdo = xd[c], ro = r[c];
+                float t = dc;
+                if (rc & RC_CG_ROW)
+                    t = apply_row(rc, dc, sh[cur][ty + 1][tx], sh[cur][ty + 1][tx + 2], sh[cur][ty][tx + 1],
+                                  sh[cur][ty + 2][tx + 1], np, dm);
+                xd[c] = (float)((double)xdo + alpha * (double)dc);
+                const float rn = (float)((double)ro - alpha * (double)t);
+                r[c] = rn;
+                v += (double)rn * (double)rn;
             }
+            sh[cur ^ 1][ty + 1][tx + 1] = np;
+            if (S.hx_slot >= 0) sh[cur ^ 1][ty + 1][S.hx_slot] = nhx;
+            if (S.hy_slot >= 0) sh[cur ^ 1][S.hy_slot][tx + 1] = nhy;
+            __syncthreads();
+            cur ^= 1;
+            dm = dc;
+            dc = np;
         }
     }
-    double out[1];
-    if (grid_reduce<1>(v, partials, &st->counter, d.nblocks, out)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            const double gnew = out[0];
-            st->iter = it + 1;
-            st->gamma_old = st->gamma;
-            st->beta = gnew / st->gamma;
-            st->gamma = gnew;
-            st->converged = (gnew < st->thr) ? 1 : 0;
-            st->done = st->converged || (it + 1) >= st->max_iter || !isfinite(gnew);
-        }
+    write_partial(v, p_rr, red);
+    if (blockIdx.x == 0 && tx == 0 && ty == 0) st->alpha = alpha;
+}
+
+// Replacement iteration, part 1: x64 += xd + alpha d (own cell), xd = 0.
+__global__ __launch_bounds__(CG_NT) void k_cg_fold(Dom d, int it, const float* d0, const float* d1, float* __restrict__ xd,
+                                                   double* __restrict__ x64, const CGState* st, const double* p_dad) {
+    if (st->done) return;
+    __shared__ double red[CG_NT / 64];
+    const double alpha = st->gam[it & 1] / sum_partials(p_dad, gridDim.x, red);
+    const float* dd = (it & 1) ? d1 : d0;
+    int bx, by, bz;
+    if (!block_coords(d, bx, by, bz)) return;
+    const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * d.ty + threadIdx.y;
+    if (x >= d.x0 + d.nx || y >= d.y0 + d.ny) return;
+    const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
+    for (int z = zb; z < ze; z++) {
+        const int c = (z * d.H + y) * d.W + x;
+        const float xn = (float)((double)xd[c] + alpha * (double)dd[c]);
+        x64[c] = x64[c] + (double)xn;
+        xd[c] = 0.f;
     }
 }
 
-// FP64 true residual ||b - A x||^2 and ||b||^2 over the launch domain.
-__global__ __launch_bounds__(256) void k_true_residual(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
-                                                       const float* __restrict__ xv, CGState* st, double* partials) {
+// Replacement iteration, part 2: r = b - A x64 in FP64, rounded to FP32; partial r.r.
+__global__ __launch_bounds__(CG_NT) void k_cg_residual(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
+                                                       const double* __restrict__ x64, float* __restrict__ r, const CGState* st,
+                                                       double* p_rr) {
+    if (st->done) return;
+    __shared__ double red[CG_NT / 64];
     int bx, by, bz;
-    double v[2] = {0.0, 0.0};
+    double v = 0.0;
     if (block_coords(d, bx, by, bz)) {
-        const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * TY + threadIdx.y;
+        const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * d.ty + threadIdx.y;
         const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
         if (x < d.x0 + d.nx && y < d.y0 + d.ny) {
             for (int z = zb; z < ze; z++) {
                 const int c = (z * d.H + y) * d.W + x;
                 const Nb n = nb_offsets(d, x, y, z);
-                const double t = apply_row_d(rcode[c], xv[c], xv[c + n.xm], xv[c + n.xp], xv[c + n.ym], xv[c + n.yp],
-                                             xv[c + n.zp], xv[c + n.zm]);
+                const double t = apply_row_d(rcode[c], x64[c], x64[c + n.xm], x64[c + n.xp], x64[c + n.ym], x64[c + n.yp],
+                                             x64[c + n.zp], x64[c + n.zm]);
+                const float rn = (float)((double)b[c] - t);
+                r[c] = rn;
+                v += (double)rn * (double)rn;
+            }
+        }
+    }
+    write_partial(v, p_rr, red);
+}
+
+// Convergence test for iteration `it` after a batch (one block, the CG grid's partial count).
+__global__ __launch_bounds__(CG_NT) void k_cg_check(int nparts, int it, CGState* st, const double* p_rr, const double* p_bb) {
+    if (st->done) return;
+    __shared__ double red[CG_NT / 64];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    auto sum = [&](const double* p) {
+        double s = 0.0;
+        for (int i = lane; i < nparts; i += CG_NT) s += p[i];
+        s = wave_sum(s);
+        if ((lane & 63) == 0) red[lane >> 6] = s;
+        __syncthreads();
+        double tot = 0.0;
+        for (int w = 0; w < CG_NT / 64; w++) tot += red[w];
+        __syncthreads();
+        return tot;
+    };
+    const double g = sum(p_rr);
+    double thr = st->thr, bb = st->bnorm2;
+    if (it == 0) {
+        bb = sum(p_bb);
+        thr = st->thr_in;
+        if (st->flags & 2) thr = thr * thr * bb;
+    }
+    if (lane == 0) {
+        const bool conv = g < thr;
+        st->gamma = g;
+        st->gam[it & 1] = g;
+        if (it == 0) {
+            st->thr = thr;
+            st->bnorm2 = bb;
+        }
+        if (conv || !isfinite(g) || it >= st->max_iter) {
+            st->iter = it;
+            st->converged = conv ? 1 : 0;
+            st->done = 1;
+        }
+    }
+}
+
+// whole array: x64 += xd; pressure = (float)x64
+__global__ void k_cg_finalize(int n, double* __restrict__ x64, const float* __restrict__ xd, float* __restrict__ p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = x64[i] + (double)xd[i];
+    x64[i] = x;
+    p[i] = (float)x;
+}
+
+// FP64 true residual ||b - A x||^2 and ||b||^2 (x in FP64 if given, else the FP32 field).
+__global__ __launch_bounds__(256) void k_true_residual(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
+                                                       const float* __restrict__ xf, const double* __restrict__ xdbl,
+                                                       CGState* st, double* partials) {
+    int bx, by, bz;
+    double v[2] = {0.0, 0.0};
+    if (block_coords(d, bx, by, bz)) {
+        const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * d.ty + threadIdx.y;
+        const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
+        if (x < d.x0 + d.nx && y < d.y0 + d.ny) {
+            for (int z = zb; z < ze; z++) {
+                const int c = (z * d.H + y) * d.W + x;
+                const Nb n = nb_offsets(d, x, y, z);
+                auto X = [&](int q) -> double { return xdbl ? xdbl[q] : (double)xf[q]; };
+                const double t = apply_row_d(rcode[c], X(c), X(c + n.xm), X(c + n.xp), X(c + n.ym), X(c + n.yp), X(c + n.zp),
+                                             X(c + n.zm));
                 const double rr = (double)b[c] - t;
                 v[0] += rr * rr;
                 v[1] += (double)b[c] * (double)b[c];
@@ -357,23 +550,11 @@ __global__ __launch_bounds__(256) void k_true_residual(Dom d, const uint16_t* __
         }
     }
     double out[2];
-    if (grid_reduce<2>(v, partials, &st->counter, d.nblocks, out)) {
+    if (grid_reduce<256, 2>(v, partials, &st->counter, out)) {
         if (threadIdx.x == 0 && threadIdx.y == 0) {
             st->true_num = out[0];
             st->bnorm2 = out[1];
         }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_copy_domain(Dom d, const float* __restrict__ src, float* __restrict__ dst) {
-    int bx, by, bz;
-    if (!block_coords(d, bx, by, bz)) return;
-    const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * TY + threadIdx.y;
-    if (x >= d.x0 + d.nx || y >= d.y0 + d.ny) return;
-    const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
-    for (int z = zb; z < ze; z++) {
-        const int c = (z * d.H + y) * d.W + x;
-        dst[c] = src[c];
     }
 }
 
@@ -382,7 +563,7 @@ __global__ __launch_bounds__(256) void k_spmv(Dom d, const uint16_t* __restrict_
                                               float* __restrict__ yv) {
     int bx, by, bz;
     if (!block_coords(d, bx, by, bz)) return;
-    const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * TY + threadIdx.y;
+    const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * d.ty + threadIdx.y;
     if (x >= d.x0 + d.nx || y >= d.y0 + d.ny) return;
     const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
     for (int z = zb; z < ze; z++) {
@@ -394,42 +575,60 @@ __global__ __launch_bounds__(256) void k_spmv(Dom d, const uint16_t* __restrict_
 
 // ---------------------------------------------------------------------------------------
 // Jacobi pass (JacobiSolver.cpp:165-196): neighbour order x-, x+, y-, y+, z-, z+;
-// p' = (sum - b) / nCount, 0 when nCount == 0; identity rows copy p.
+// p' = (sum - b) / nCount, 0 when nCount == 0; identity rows copy p.  Same tiling as K2.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_jacobi(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
-                                                const float* __restrict__ pin, float* __restrict__ pout) {
+__global__ __launch_bounds__(CG_NT) void k_jacobi(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
+                                                  const float* __restrict__ pin, float* __restrict__ pout) {
+    __shared__ float sh[2][CG_TY + 2][TX + 2];
+    const int tx = threadIdx.x, ty = threadIdx.y;
     int bx, by, bz;
     if (!block_coords(d, bx, by, bz)) return;
-    const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * TY + threadIdx.y;
-    if (x >= d.x0 + d.nx || y >= d.y0 + d.ny) return;
+    const int x = d.x0 + bx * TX + tx, y = d.y0 + by * CG_TY + ty;
+    const bool in_dom = x < d.x0 + d.nx && y < d.y0 + d.ny;
+    Stager<CG_TY> S;
+    S.init(d, x, y);
     const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
     const int sz = d.W * d.H;
     int c = (zb * d.H + y) * d.W + x;
-    float cm = (zb > 0) ? pin[c - sz] : 0.f;
-    float cc = pin[c];
+    float pm = (zb > 0 && S.own_ok) ? pin[c - sz] : 0.f;
+    float pc = S.own_ok ? pin[c] : 0.f;
+    sh[0][ty + 1][tx + 1] = pc;
+    if (S.hx_slot >= 0) sh[0][ty + 1][S.hx_slot] = S.hx_ok ? pin[c + S.hx_off] : 0.f;
+    if (S.hy_slot >= 0) sh[0][S.hy_slot][tx + 1] = S.hy_ok ? pin[c + S.hy_off] : 0.f;
+    __syncthreads();
+    int cur = 0;
     for (int z = zb; z < ze; z++, c += sz) {
-        const Nb n = nb_offsets(d, x, y, z);
-        const float cp = pin[c + n.zp];
-        const unsigned rc = rcode[c];
-        float out = cc;
-        if (rc & RC_JAC_ROW) {
-            float sum = 0.0f;
-            if (rc & RC_NB_XM) sum += pin[c + n.xm];
-            if (rc & RC_NB_XP) sum += pin[c + n.xp];
-            if (rc & RC_NB_YM) sum += pin[c + n.ym];
-            if (rc & RC_NB_YP) sum += pin[c + n.yp];
-            if (rc & RC_NB_ZM) sum += cm;
-            if (rc & RC_NB_ZP) sum += cp;
-            const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
-            out = cnt > 0 ? (sum - b[c]) / (float)cnt : 0.0f;
+        const bool has_next = z + 1 < d.D;
+        const float np = (has_next && S.own_ok) ? pin[c + sz] : 0.f;
+        const float nhx = (has_next && S.hx_ok) ? pin[c + sz + S.hx_off] : 0.f;
+        const float nhy = (has_next && S.hy_ok) ? pin[c + sz + S.hy_off] : 0.f;
+        if (in_dom) {
+            const unsigned rc = rcode[c];
+            float out = pc;
+            if (rc & RC_JAC_ROW) {
+                float sum = 0.0f;
+                if (rc & RC_NB_XM) sum += sh[cur][ty + 1][tx];
+                if (rc & RC_NB_XP) sum += sh[cur][ty + 1][tx + 2];
+                if (rc & RC_NB_YM) sum += sh[cur][ty][tx + 1];
+                if (rc & RC_NB_YP) sum += sh[cur][ty + 2][tx + 1];
+                if (rc & RC_NB_ZM) sum += pm;
+                if (rc & RC_NB_ZP) sum += np;
+                const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
+                out = cnt > 0 ? (sum - b[c]) / (float)cnt : 0.0f;
+            }
+            pout[c] = out;
         }
-        pout[c] = out;
-        cm = cc;
-        cc = cp;
+        sh[cur ^ 1][ty + 1][tx + 1] = np;
+        if (S.hx_slot >= 0) sh[cur ^ 1][ty + 1][S.hx_slot] = nhx;
+        if (S.hy_slot >= 0) sh[cur ^ 1][S.hy_slot][tx + 1] = nhy;
+        __syncthreads();
+        cur ^= 1;
+        pm = pc;
+        pc = np;
     }
 }
 
-Dom make_dom(const efs_sim* s, bool full) {
+Dom make_dom(const efs_sim* s, bool full, int ty, int target_blocks) {
     Dom d;
     d.W = s->W; d.H = s->H; d.D = s->D;
     if (full) {
@@ -439,37 +638,42 @@ Dom make_dom(const efs_sim* s, bool full) {
         d.x0 = d.y0 = d.z0 = 1;
         d.nx = s->W - 2; d.ny = s->H - 2; d.nz = s->D - 2;
     }
-    d.full = full ? 1 : 0;
+    d.ty = ty;
     d.tiles_x = (d.nx + TX - 1) / TX;
-    d.tiles_y = (d.ny + TY - 1) / TY;
+    d.tiles_y = (d.ny + ty - 1) / ty;
     const int txy = d.tiles_x * d.tiles_y;
-    // aim for >= 2048 blocks (8 per CU) with chunks of 2..32 planes
-    int zc = (int)std::max<long long>(2, std::min<long long>(32, (long long)d.nz * txy / 2048));
+    int zc = (int)std::max<long long>(2, std::min<long long>(32, (long long)d.nz * txy / target_blocks));
     zc = std::min(zc, std::max(1, d.nz));
     d.zc = zc;
     d.chunks_z = (d.nz + zc - 1) / zc;
     d.nblocks = txy * d.chunks_z;
     return d;
 }
+Dom dom_simple(const efs_sim* s, bool full) { return make_dom(s, full, 4, 2048); }
+Dom dom_tiled(const efs_sim* s, bool full) { return make_dom(s, full, CG_TY, 2048); }
 
 int launch_blocks(const Dom& d) { return ((d.nblocks + 7) / 8) * 8; }
+inline int blocks1d(int n) { return (n + 255) / 256; }
 
 }  // namespace
 
 int efs_solver_alloc(efs_sim* s) {
     const size_t N = (size_t)s->N;
     EFS_HIP(hipMalloc(&s->rowcode, N * sizeof(uint16_t)));
-    EFS_HIP(hipMalloc(&s->cg_r, N * 4));
-    EFS_HIP(hipMalloc(&s->cg_d0, N * 4));
-    EFS_HIP(hipMalloc(&s->cg_d1, N * 4));
-    EFS_HIP(hipMalloc(&s->cg_x1, N * 4));
-    EFS_HIP(hipMemset(s->cg_r, 0, N * 4));
-    EFS_HIP(hipMemset(s->cg_d0, 0, N * 4));
-    EFS_HIP(hipMemset(s->cg_d1, 0, N * 4));
-    EFS_HIP(hipMemset(s->cg_x1, 0, N * 4));
-    const Dom d = make_dom(s, true);
-    s->max_blocks = launch_blocks(d) + 64;
+    float** fl[] = {&s->cg_r, &s->cg_d0, &s->cg_d1, &s->cg_x1, &s->cg_xd};
+    for (float** p : fl) {
+        EFS_HIP(hipMalloc(p, N * 4));
+        EFS_HIP(hipMemset(*p, 0, N * 4));
+    }
+    EFS_HIP(hipMalloc(&s->cg_x64, N * 8));
+    EFS_HIP(hipMemset(s->cg_x64, 0, N * 8));
+    s->max_blocks = std::max(launch_blocks(dom_simple(s, true)), launch_blocks(dom_tiled(s, true))) + 64;
     EFS_HIP(hipMalloc(&s->partials, (size_t)s->max_blocks * 2 * sizeof(double)));
+    double** pp[] = {&s->p_rr, &s->p_bb, &s->p_dad};
+    for (double** p : pp) {
+        EFS_HIP(hipMalloc(p, (size_t)s->max_blocks * sizeof(double)));
+        EFS_HIP(hipMemset(*p, 0, (size_t)s->max_blocks * sizeof(double)));
+    }
     EFS_HIP(hipMalloc(&s->st, sizeof(CGState)));
     EFS_HIP(hipMemset(s->st, 0, sizeof(CGState)));
     EFS_HIP(hipHostMalloc(&s->st_host, sizeof(CGState), hipHostMallocDefault));
@@ -483,19 +687,22 @@ void efs_solver_free(efs_sim* s) {
     hipFree(s->cg_d0);
     hipFree(s->cg_d1);
     hipFree(s->cg_x1);
+    hipFree(s->cg_xd);
+    hipFree(s->cg_x64);
     hipFree(s->partials);
+    hipFree(s->p_rr);
+    hipFree(s->p_bb);
+    hipFree(s->p_dad);
     hipFree(s->st);
     if (s->st_host) hipHostFree(s->st_host);
 }
 
 int efs_launch_rowcode(efs_sim* s, int* border_rows) {
     const DevGrid g = s->grid();
-    // flags cleared on the device so the whole solve stays ordered on the stream
     CGState init{};
-    init.max_iter = 0;
     std::memcpy(s->st_host, &init, sizeof(CGState));
     EFS_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(CGState), hipMemcpyHostToDevice, s->stream));
-    dim3 blk(TX, TY, 1), grd((g.W + TX - 1) / TX, (g.H + TY - 1) / TY, g.D);
+    dim3 blk(64, 4, 1), grd((g.W + 63) / 64, (g.H + 3) / 4, g.D);
     hipLaunchKernelGGL(k_rowcode, grd, blk, 0, s->stream, g, s->rowcode, s->st);
     EFS_HIP(hipGetLastError());
     EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
@@ -507,7 +714,10 @@ int efs_launch_rowcode(efs_sim* s, int* border_rows) {
 void efs_ktimer_begin(efs_sim* s, int k, hipEvent_t* a, hipEvent_t* b) {
     *a = *b = nullptr;
     if (!s->ktiming) return;
-    if (hipEventCreate(a) != hipSuccess || hipEventCreate(b) != hipSuccess) { *a = *b = nullptr; return; }
+    if (hipEventCreate(a) != hipSuccess || hipEventCreate(b) != hipSuccess) {
+        *a = *b = nullptr;
+        return;
+    }
     hipEventRecord(*a, s->stream);
     (void)k;
 }
@@ -535,33 +745,16 @@ int efs_launch_spmv(efs_sim* s, const float* x, float* y) {
     int border = 0;
     int rc = efs_launch_rowcode(s, &border);
     if (rc) return rc;
-    const Dom d = make_dom(s, true);
-    hipLaunchKernelGGL(k_spmv, dim3(launch_blocks(d)), dim3(TX, TY), 0, s->stream, d, s->rowcode, x, y);
+    const Dom d = dom_simple(s, true);
+    hipLaunchKernelGGL(k_spmv, dim3(launch_blocks(d)), dim3(TX, d.ty), 0, s->stream, d, s->rowcode, x, y);
     EFS_HIP(hipGetLastError());
     return 0;
 }
 
-// Prepares the state, the launch domain and the initial guess x0 = b (pressure = divergence,
-// CGSolver.cpp:429).  Returns the domain through *dom.
-static int cg_prepare(efs_sim* s, Dom* dom) {
-    int border = 0;
-    int rc = efs_launch_rowcode(s, &border);
-    if (rc) return rc;
-    *dom = make_dom(s, border != 0);
-    EFS_HIP(hipMemcpyAsync(s->p, s->div, (size_t)s->N * 4, hipMemcpyDeviceToDevice, s->stream));
-    CGState init{};
-    init.thr = (double)s->P.solver_threshold;
-    init.flags = (s->P.criterion == EFS_CRIT_RELATIVE) ? 2 : 0;
-    init.max_iter = s->P.max_steps;
-    std::memcpy(s->st_host, &init, sizeof(CGState));
-    EFS_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(CGState), hipMemcpyHostToDevice, s->stream));
-    return 0;
-}
-
 int efs_true_residual(efs_sim* s, double* num, double* den) {
-    const Dom d = make_dom(s, true);
-    hipLaunchKernelGGL(k_true_residual, dim3(launch_blocks(d)), dim3(TX, TY), 0, s->stream, d, s->rowcode, s->div, s->p, s->st,
-                       s->partials);
+    const Dom d = dom_simple(s, true);
+    hipLaunchKernelGGL(k_true_residual, dim3(launch_blocks(d)), dim3(TX, d.ty), 0, s->stream, d, s->rowcode, s->div, s->p,
+                       s->p64_valid ? s->cg_x64 : nullptr, s->st, s->partials);
     EFS_HIP(hipGetLastError());
     EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
     EFS_HIP(hipStreamSynchronize(s->stream));
@@ -571,46 +764,67 @@ int efs_true_residual(efs_sim* s, double* num, double* den) {
 }
 
 int efs_cg_solve(efs_sim* s, efs_stats* out) {
-    Dom d;
-    int rc = cg_prepare(s, &d);
+    int border = 0;
+    int rc = efs_launch_rowcode(s, &border);
     if (rc) return rc;
-    const int nb = launch_blocks(d);
-    if (nb > s->max_blocks) { efs_set_error("launch exceeds reduction buffer"); return -1; }
-    const dim3 blk(TX, TY);
-    hipLaunchKernelGGL(k_cg_init, dim3(nb), blk, 0, s->stream, d, s->rowcode, s->div, s->cg_r, s->cg_d0, s->st, s->partials);
+    s->p64_valid = false;
+    const Dom dt = dom_tiled(s, border != 0);
+    const int nbt = launch_blocks(dt);
+    if (nbt > s->max_blocks) {
+        efs_set_error("launch exceeds the partial-sum buffers");
+        return -1;
+    }
+    CGState init{};
+    init.thr_in = (double)s->P.solver_threshold;
+    init.thr = init.thr_in;
+    init.flags = (s->P.criterion == EFS_CRIT_RELATIVE) ? 2 : 0;
+    init.max_iter = s->P.max_steps;
+    std::memcpy(s->st_host, &init, sizeof(CGState));
+    EFS_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(CGState), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_cg_setup, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->div, s->cg_x64, s->cg_xd);
+    const dim3 bt(TX, CG_TY);
+    hipLaunchKernelGGL(k_cg_init, dim3(nbt), bt, 0, s->stream, dt, s->rowcode, s->div, s->cg_r, s->cg_d0, s->p_rr, s->p_bb);
     EFS_HIP(hipGetLastError());
     const int max_it = s->P.max_steps;
-    const int batch = 16;
     int launched = 0;
     const size_t kt0 = s->ktimers[KT_CG_DIR_SPMV].start.size(), kt1 = s->ktimers[KT_CG_UPDATE].start.size();
     while (true) {
-        const int n = std::min(batch, max_it - launched);
+        const int n = std::min(64, max_it - launched);
         for (int k = 0; k < n; k++) {
+            const int i = launched + k;
             hipEvent_t a, b;
             efs_ktimer_begin(s, KT_CG_DIR_SPMV, &a, &b);
-            hipLaunchKernelGGL(k_cg_dir_spmv, dim3(nb), blk, 0, s->stream, d, s->rowcode, s->cg_r, s->cg_d0, s->cg_d1, s->st,
-                               s->partials);
+            hipLaunchKernelGGL(k_cg_dir_spmv, dim3(nbt), bt, 0, s->stream, dt, i, s->rowcode, s->cg_r, s->cg_d0, s->cg_d1,
+                               s->st, s->p_rr, s->p_bb, s->p_dad);
             efs_ktimer_end(s, KT_CG_DIR_SPMV, a, b);
-            efs_ktimer_begin(s, KT_CG_UPDATE, &a, &b);
-            hipLaunchKernelGGL(k_cg_update, dim3(nb), blk, 0, s->stream, d, s->rowcode, s->div, s->cg_r, s->cg_d0, s->cg_d1,
-                               s->p, s->cg_x1, s->st, s->partials);
-            efs_ktimer_end(s, KT_CG_UPDATE, a, b);
+            if ((i & 0x3F) == 0) {
+                hipLaunchKernelGGL(k_cg_fold, dim3(nbt), bt, 0, s->stream, dt, i, s->cg_d0, s->cg_d1, s->cg_xd, s->cg_x64, s->st,
+                                   s->p_dad);
+                hipLaunchKernelGGL(k_cg_residual, dim3(nbt), bt, 0, s->stream, dt, s->rowcode, s->div, s->cg_x64, s->cg_r, s->st,
+                                   s->p_rr);
+            } else {
+                efs_ktimer_begin(s, KT_CG_UPDATE, &a, &b);
+                hipLaunchKernelGGL(k_cg_update, dim3(nbt), bt, 0, s->stream, dt, i, s->rowcode, s->cg_r, s->cg_d0, s->cg_d1,
+                                   s->cg_xd, s->st, s->p_dad, s->p_rr);
+                efs_ktimer_end(s, KT_CG_UPDATE, a, b);
+            }
         }
-        EFS_HIP(hipGetLastError());
         launched += n;
+        // convergence test of iteration `launched` (no-op if a K1 already stopped the solve)
+        hipLaunchKernelGGL(k_cg_check, dim3(1), bt, 0, s->stream, nbt, launched, s->st, s->p_rr, s->p_bb);
+        EFS_HIP(hipGetLastError());
         EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
         EFS_HIP(hipStreamSynchronize(s->stream));
         if (s->st_host->done || launched >= max_it) break;
     }
     const CGState fin = *s->st_host;
-    // launches after convergence exit at once; drop their events so kernel timings count real
-    // iterations only
+    // launches after convergence exit at once; keep the events of real iterations only
     trim_timer(s->ktimers[KT_CG_DIR_SPMV], kt0 + fin.iter);
-    trim_timer(s->ktimers[KT_CG_UPDATE], kt1 + fin.iter);
-    // the final iterate lives in x[iter & 1]; x[0] is the pressure field
-    if (fin.iter & 1)
-        hipLaunchKernelGGL(k_copy_domain, dim3(nb), blk, 0, s->stream, d, s->cg_x1, s->p);
+    const size_t repl = (size_t)(fin.iter + 63) / 64;  // replacement iterations have no K2
+    trim_timer(s->ktimers[KT_CG_UPDATE], kt1 + (size_t)fin.iter - std::min<size_t>(repl, fin.iter));
+    hipLaunchKernelGGL(k_cg_finalize, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->cg_x64, s->cg_xd, s->p);
     EFS_HIP(hipGetLastError());
+    s->p64_valid = true;
     if (out) {
         out->iterations = fin.iter;
         out->rr = fin.gamma;
@@ -632,7 +846,8 @@ int efs_jacobi_solve(efs_sim* s, efs_stats* out) {
     int border = 0;
     int rc = efs_launch_rowcode(s, &border);
     if (rc) return rc;
-    const Dom d = make_dom(s, border != 0);
+    s->p64_valid = false;
+    const Dom d = dom_tiled(s, border != 0);
     const int nb = launch_blocks(d);
     const size_t bytes = (size_t)s->N * 4;
     // p was cleared by project(); both ping-pong buffers start from it
@@ -643,7 +858,7 @@ int efs_jacobi_solve(efs_sim* s, efs_stats* out) {
         float* pout = (i & 1) ? s->p : s->cg_x1;
         hipEvent_t a, b;
         efs_ktimer_begin(s, KT_JACOBI, &a, &b);
-        hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(TX, TY), 0, s->stream, d, s->rowcode, s->div, pin, pout);
+        hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(TX, CG_TY), 0, s->stream, d, s->rowcode, s->div, pin, pout);
         efs_ktimer_end(s, KT_JACOBI, a, b);
     }
     EFS_HIP(hipGetLastError());

@@ -507,7 +507,8 @@ __global__ __launch_bounds__(256) void k_walls_div(DevGrid g, float liq, float l
 }
 
 // u -= grad p on non-solid interior cells (x>1 / y>1 / z>1 guards), then enforce the faces.
-__global__ __launch_bounds__(256) void k_gradient_walls(DevGrid g) {
+// p64 (optional) is the CG solver's FP64 iterate; the FP32 pressure field is its rounding.
+__global__ __launch_bounds__(256) void k_gradient_walls(DevGrid g, const double* __restrict__ p64) {
     FULL_XYZ(g);
     if (x == 0 || y == 0 || z == 0) return;
     const int sy = g.W, sz = g.W * g.H;
@@ -515,11 +516,19 @@ __global__ __launch_bounds__(256) void k_gradient_walls(DevGrid g) {
     float u = g.u[c], v = g.v[c], w = g.w[c];
     const bool interior = x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
     if (interior && !s0) {
-        const float pc = g.p[c];
-        const float gx = (x > 1) ? pc - g.p[c - 1] : 0.f;
-        const float gy = (y > 1) ? pc - g.p[c - sy] : 0.f;
-        const float gz = (z > 1) ? pc - g.p[c - sz] : 0.f;
-        u = u - gx; v = v - gy; w = w - gz;
+        if (p64) {
+            const double pc = p64[c];
+            const double gx = (x > 1) ? pc - p64[c - 1] : 0.0;
+            const double gy = (y > 1) ? pc - p64[c - sy] : 0.0;
+            const double gz = (z > 1) ? pc - p64[c - sz] : 0.0;
+            u = (float)((double)u - gx); v = (float)((double)v - gy); w = (float)((double)w - gz);
+        } else {
+            const float pc = g.p[c];
+            const float gx = (x > 1) ? pc - g.p[c - 1] : 0.f;
+            const float gy = (y > 1) ? pc - g.p[c - sy] : 0.f;
+            const float gz = (z > 1) ? pc - g.p[c - sz] : 0.f;
+            u = u - gx; v = v - gy; w = w - gz;
+        }
     }
     g.u[c] = wall_face(u, s0, g.bf[c - 1] & BF_EQ1);
     g.v[c] = wall_face(v, s0, g.bf[c - sy] & BF_EQ1);
@@ -625,7 +634,8 @@ int efs_launch_walls_divergence(efs_sim* s) {
 }
 
 int efs_launch_gradient_walls(efs_sim* s) {
-    hipLaunchKernelGGL(k_gradient_walls, full_grid(s), dim3(BX, BY), 0, s->stream, s->grid());
+    hipLaunchKernelGGL(k_gradient_walls, full_grid(s), dim3(BX, BY), 0, s->stream, s->grid(),
+                       s->p64_valid ? (const double*)s->cg_x64 : nullptr);
     EFS_HIP(hipGetLastError());
     return 0;
 }

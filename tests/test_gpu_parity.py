@@ -179,3 +179,20 @@ def test_free_running_steps(case):
     m_gpu, m_ref = float(g.rho().astype(np.float64).sum()), float(o.field("rho").astype(np.float64).sum())
     assert abs(m_gpu - m_ref) <= 1e-3 * max(1.0, m_ref)
     sim.close()
+
+
+@pytest.mark.parametrize("n", [48, 96])
+def test_cg_relative_criterion_true_residual(hiplib, n):
+    """Mixed-precision CG (FP32 stencil / FP64 dots and iterate) reaches ||b - A p|| / ||b||
+    <= 1e-5 in FP64, where an FP32 iterate cannot (its rounding floor is above 1e-5 at large n)."""
+    sc = scenes.sphere_pool(n, seed=0)
+    state = oracle_state(sc, 1)
+    sim = HipSimulator3D(*sc.dims, params=gpu_params(sc, criterion=1, true_residual=1, max_steps=4000))
+    to_gpu(sim, state)
+    rc = sim.project()
+    st = sim.last_stats
+    print(f"[rel n={n}] rc={rc} its={st.iterations} rel(recurrence)={st.rel_residual:.3e} true={st.true_rel_residual:.3e}")
+    assert rc == 0
+    assert st.rel_residual < 1e-5
+    assert st.true_rel_residual < 1.5e-5
+    sim.close()
