@@ -31,6 +31,23 @@ FLAGS = [
 ]
 
 
+HOST_DIR = os.path.join(HERE, "host")
+HOST_BIN = os.path.join(HOST_DIR, "efs_run")
+
+
+def build_host(verbose: bool = False) -> str:
+    """C++ host driver over the C++ adapter (HipSimulator3D.h), linked against the library."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", os.path.join(HOST_DIR, "efs_run.cpp"), "-o", HOST_BIN,
+           "-L", HERE, "-lefs_hip", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building the host driver")
+    return HOST_BIN
+
+
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -42,6 +59,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libefs_hip.so")
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
 
 

@@ -1,0 +1,158 @@
+// C++ host side of the HIP backend: HipSimulator3D / HipGrid3D mirror the reference's backend
+// interface (ISimulator3D, include/simulation/simulator/ISimulator.h:8-53 and ISimulator3D.h;
+// IGrid3D, include/simulation/data/IGrid.h:44-68) over the C ABI of include/efs.h.
+//
+// The device is authoritative: fields stay resident in HBM across steps.  Host mirrors are
+// downloaded on first access after a step (the lazy download CudaGrid3D does with setDirty(),
+// CudaGrid3D.h:16) and uploaded before the next step only when the caller marked them modified,
+// instead of the reference CUDA backend's 8-field upload and download every step
+// (CudaSimulator3D.cu:86-98).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/efs.h"
+
+namespace efs {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& what) : std::runtime_error(what + " (efs code " + std::to_string(code) + ")"), code(code) {}
+    int code;
+};
+
+inline int check(int rc, const char* what, bool allow_status = false) {
+    if (rc < 0 || (!allow_status && rc != 0)) throw Error(rc, std::string(what) + ": " + efs_last_error());
+    return rc;
+}
+
+class HipGrid3D {
+public:
+    HipGrid3D(efs_handle h, unsigned w, unsigned hgt, unsigned d) : _h(h), _w(w), _hgt(hgt), _d(d) {
+        for (int f = 0; f < 8; f++) {
+            _mirror[f].resize((size_t)w * hgt * d * ncomp(f));
+            _host_stale[f] = true;
+            _host_modified[f] = false;
+        }
+    }
+    unsigned width() const { return _w; }
+    unsigned height() const { return _hgt; }
+    unsigned depth() const { return _d; }
+    unsigned size() const { return _w * _hgt * _d; }
+
+    // IGrid3D accessors (reference layout: float per cell, float[3] per cell for vel/forces)
+    std::vector<float>& vel() { return field(EFS_FIELD_VEL); }
+    std::vector<float>& rho() { return field(EFS_FIELD_RHO); }
+    std::vector<float>& gamma() { return field(EFS_FIELD_GAMMA); }
+    std::vector<float>& beta() { return field(EFS_FIELD_BETA); }
+    std::vector<float>& divergence() { return field(EFS_FIELD_DIVERGENCE); }
+    std::vector<float>& pressure() { return field(EFS_FIELD_PRESSURE); }
+    std::vector<float>& forces() { return field(EFS_FIELD_FORCES); }
+    std::vector<float>& boundary() { return field(EFS_FIELD_BOUNDARY); }
+
+    // The caller edited a mirror (editor, scene nodes): upload it before the next step.
+    void markModified(int f) { _host_modified[f] = true; }
+    // The device changed every field (after a step).
+    void setDirty() {
+        for (int f = 0; f < 8; f++) _host_stale[f] = true;
+    }
+    void uploadModified() {
+        // boundary first: the other stages read the boundary byte derived from it
+        static const int order[8] = {EFS_FIELD_BOUNDARY, EFS_FIELD_VEL, EFS_FIELD_RHO, EFS_FIELD_GAMMA,
+                                     EFS_FIELD_BETA, EFS_FIELD_DIVERGENCE, EFS_FIELD_PRESSURE, EFS_FIELD_FORCES};
+        for (int f : order) {
+            if (!_host_modified[f]) continue;
+            check(efs_upload(_h, f, _mirror[f].data(), _mirror[f].size() * sizeof(float)), "efs_upload");
+            _host_modified[f] = false;
+        }
+    }
+
+private:
+    static int ncomp(int f) { return (f == EFS_FIELD_VEL || f == EFS_FIELD_FORCES) ? 3 : 1; }
+    std::vector<float>& field(int f) {
+        if (_host_stale[f] && !_host_modified[f]) {
+            check(efs_download(_h, f, _mirror[f].data(), _mirror[f].size() * sizeof(float)), "efs_download");
+            _host_stale[f] = false;
+        }
+        return _mirror[f];
+    }
+    efs_handle _h;
+    unsigned _w, _hgt, _d;
+    std::array<std::vector<float>, 8> _mirror;
+    std::array<bool, 8> _host_stale, _host_modified;
+};
+
+// ISimulator3D on one GPU.  update() runs the seven stages of ISimulator::update in order with
+// the DISABLE_* gating (efs_params::disable_mask); the stage methods run one stage each.
+class HipSimulator3D {
+public:
+    struct RenderData3D {  // SimulationDisplay3D::RenderData3D (dim, grid)
+        int dim = 3;
+        HipGrid3D* grid = nullptr;
+    };
+
+    HipSimulator3D(unsigned width, unsigned height, unsigned depth, const efs_params* params = nullptr)
+        : _width(width), _height(height), _depth(depth) {
+        efs_params p;
+        if (params) p = *params;
+        else efs_default_params(&p);
+        check(efs_create_3d((int32_t)width, (int32_t)height, (int32_t)depth, &p, &_h), "efs_create_3d");
+        _grid = new HipGrid3D(_h, width, height, depth);
+        _data.grid = _grid;
+    }
+    ~HipSimulator3D() {
+        delete _grid;
+        efs_destroy(_h);
+    }
+    HipSimulator3D(const HipSimulator3D&) = delete;
+    HipSimulator3D& operator=(const HipSimulator3D&) = delete;
+
+    void setParams(const efs_params& p) { check(efs_set_params(_h, &p), "efs_set_params"); }
+
+    // ISimulator::update (ISimulator.h:12-38).  Returns the step status: 1 = the pressure solve
+    // did not converge, which pauses the simulation as CPUSimulator3D::project does.
+    int update() {
+        _grid->uploadModified();
+        const int rc = check(efs_step(_h, &_stats), "efs_step", true);
+        _grid->setDirty();
+        if (rc == 1) _paused = true;
+        _frames++;
+        return rc;
+    }
+    void* getRenderData() { return &_data; }
+    HipGrid3D& grid() { return *_grid; }
+    const efs_stats& lastStats() const { return _stats; }
+    bool paused() const { return _paused; }
+    long frames() const { return _frames; }
+    efs_handle handle() const { return _h; }
+
+    // the protected stages of ISimulator (ISimulator.h:43-50)
+    void vel_decay() { stage(EFS_STAGE_VEL_DECAY); }
+    void vel_extrapolate() { stage(EFS_STAGE_VEL_EXTRAPOLATE); }
+    void rho_sharpen() { stage(EFS_STAGE_RHO_SHARPEN); }
+    void rho_advect() { stage(EFS_STAGE_RHO_ADVECT); }
+    void vel_advect() { stage(EFS_STAGE_VEL_ADVECT); }
+    void add_force() { stage(EFS_STAGE_ADD_FORCE); }
+    int project() { return stage(EFS_STAGE_PROJECT); }
+
+private:
+    int stage(int s) {
+        _grid->uploadModified();
+        const int rc = check(efs_stage(_h, s, &_stats), "efs_stage", true);
+        _grid->setDirty();
+        return rc;
+    }
+    unsigned _width, _height, _depth;
+    efs_handle _h = nullptr;
+    HipGrid3D* _grid = nullptr;
+    RenderData3D _data;
+    efs_stats _stats{};
+    bool _paused = false;
+    long _frames = 0;
+};
+
+}  // namespace efs

@@ -54,3 +54,10 @@ def test_version_and_defaults_without_gpu():
     lib.efs_default_params(ctypes.byref(p))  # host-only
     assert abs(p.dt - 0.0167) < 1e-7 and p.max_steps == 2000 and p.solver_type == 2
     assert abs(p.solver_threshold - 1e-5) < 1e-12 and abs(p.lambda_minus - 0.5) < 1e-7
+
+
+def test_host_adapter_builds():
+    """The C++ host adapter compiles against include/efs.h and links the library."""
+    from eulerianfluidsimulation_amd import build as b
+    path = b.build_host()
+    assert os.path.exists(path)
