@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--config", default="sphere_pool", choices=["sphere_pool", "dam_spheres", "dam3d"])
     ap.add_argument("--precond", default="none", choices=["none", "learned"])
     ap.add_argument("--criterion", default="relative", choices=["relative", "reference"])
+    ap.add_argument("--guess", default="previous", choices=["previous", "divergence"],
+                    help="CG initial guess: the previous step's pressure (the reference CUDA backend) or b "
+                         "(the reference CPU backend)")
+    ap.add_argument("--cold-steps", type=int, default=3, help="extra timed steps with x0 = b, reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128, help="interior resolution of the CPU sample")
     return ap.parse_args()
@@ -89,6 +93,7 @@ def main():
     p.criterion = _abi.CRIT_RELATIVE if ARGS.criterion == "relative" else _abi.CRIT_REFERENCE
     p.precond = _abi.PRECOND_LEARNED if ARGS.precond == "learned" else _abi.PRECOND_NONE
     p.true_residual = 1
+    p.initial_guess = _abi.GUESS_PREVIOUS if ARGS.guess == "previous" else _abi.GUESS_DIVERGENCE
     sim = HipSimulator3D(*sc.dims, params=p)
     t0 = time.perf_counter()
     sim.load_scene(sc)
@@ -115,6 +120,21 @@ def main():
         lib.efs_kernel_time(sim.h, k, ctypes.byref(n), ctypes.byref(ms))
         kt[name] = (n.value, ms.value)
     lib.efs_set_kernel_timing(sim.h, 0)
+    # the same simulation continued with x0 = b (the CPU backend's initial guess), for reference
+    cold = None
+    if ARGS.cold_steps > 0:
+        pc = sim.params
+        pc.initial_guess = _abi.GUESS_DIVERGENCE
+        sim.set_params(pc)
+        sim.synchronize()
+        cits = []
+        t1 = time.perf_counter()
+        for i in range(ARGS.cold_steps):
+            sim.update()
+            cits.append(sim.last_stats.iterations)
+        sim.synchronize()
+        cold = {"steps_per_s": ARGS.cold_steps / (time.perf_counter() - t1), "cg_iters_per_step": float(np.mean(cits)),
+                "steps": ARGS.cold_steps}
     interior = ARGS.n ** 3
     # dominant kernel = larger total time
     dom = max(kt, key=lambda k: kt[k][1])
@@ -141,9 +161,13 @@ def main():
         "dtype": "f32 (stencils/advection) + f64 (CG dots, alpha/beta, true residual)",
         "data": "synthetic seeded scene",
         "config": {"workload": f"{sc.name}: {ARGS.n}^3 interior ({sc.dims[0]}^3 with border), sphere r=0.15 over pool "
-                               f"depth 0.3, 8 seeded solid boxes, dt=1/60, CG to rel. residual 1e-5",
-                   "criterion": ARGS.criterion, "precond": ARGS.precond},
+                               f"depth 0.3, 8 seeded solid boxes, dt=1/60, CG to rel. residual 1e-5 (FP64-verified)",
+                   "criterion": ARGS.criterion, "precond": ARGS.precond,
+                   "initial_guess": "previous step pressure (CUDA backend, CudaSolver.cu:427)" if ARGS.guess == "previous"
+                   else "b (CPU backend, CGSolver.cpp:429)"},
         "cg_iters_per_step": its,
+        "cg_iters_each_step": iters,
+        "x0_b_reference": cold,
         "true_rel_residual_max": float(max(true_res)) if true_res else None,
         "kernel_events": {k: {"launches": v[0], "avg_us": 1e3 * v[1] / max(1, v[0])} for k, v in kt.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -150,6 +150,10 @@ class HipSimulator3D:
     def project(self):
         return self._stage(_abi.STAGE_PROJECT)
 
+    def project_divergence(self):
+        """Wall enforcement and divergence only (the first half of project)."""
+        check(self.lib.efs_project_divergence(self.h))
+
     # -- ISolver3D::solve on the current divergence --
     def solve_pressure(self) -> Stats:
         st = Stats()

@@ -14,6 +14,7 @@ STAGE_NAMES = ["0 Vel Decay", "1 Vel Extrapolate", "2 Rho Sharpen", "3 Rho Advec
 SOLVER_SELF_CG, SOLVER_JACOBI = 2, 3
 CRIT_REFERENCE, CRIT_RELATIVE = 0, 1
 PRECOND_NONE, PRECOND_LEARNED = 0, 1
+GUESS_DIVERGENCE, GUESS_PREVIOUS = 0, 1
 KT_CG_DIR_SPMV, KT_CG_UPDATE, KT_JACOBI, KT_OTHER = range(4)
 
 
@@ -35,6 +36,7 @@ class Params(ctypes.Structure):
         ("criterion", ctypes.c_int32),
         ("precond", ctypes.c_int32),
         ("true_residual", ctypes.c_int32),
+        ("initial_guess", ctypes.c_int32),
     ]
 
 
@@ -49,6 +51,18 @@ class Stats(ctypes.Structure):
         ("fallback", ctypes.c_int32),
         ("warm_started", ctypes.c_int32),
         ("stage_ms", ctypes.c_float * 7),
+    ]
+
+
+class LearnedOpts(ctypes.Structure):
+    _fields_ = [
+        ("out_scale", ctypes.c_float),
+        ("jacobi_weight", ctypes.c_float),
+        ("warm_start", ctypes.c_int32),
+        ("max_iters", ctypes.c_int32),
+        ("every", ctypes.c_int32),
+        ("stall_window", ctypes.c_int32),
+        ("stall_ratio", ctypes.c_float),
     ]
 
 
@@ -76,6 +90,7 @@ SIGNATURES = {
     "efs_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     "efs_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Stats)]),
     "efs_solve_pressure": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    "efs_project_divergence": (ctypes.c_int, [ctypes.c_void_p]),
     "efs_apply_operator": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "efs_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "efs_get_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
@@ -85,6 +100,23 @@ SIGNATURES = {
     "efs_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]),
     "efs_scene_rasterise": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prim), ctypes.c_int32]),
     "efs_scene_face_noise": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double]),
+    "efs_unet_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "efs_unet_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "efs_unet_num_params": (ctypes.c_int64, [ctypes.c_void_p]),
+    "efs_unet_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "efs_unet_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "efs_unet_get_grads": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "efs_unet_forward_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]),
+    "efs_unet_loss_grad_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    "efs_unet_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
+    "efs_unet_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "efs_unet_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "efs_default_learned_opts": (None, [ctypes.POINTER(LearnedOpts)]),
+    "efs_set_learned_preconditioner": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(LearnedOpts)]),
+    "efs_learned_coarse_dims": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "efs_learned_coarse_pair": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.POINTER(ctypes.c_double)]),
     "efs_last_error": (ctypes.c_char_p, []),
     "efs_version": (ctypes.c_char_p, []),
 }

@@ -274,6 +274,7 @@ void efs_default_params(efs_params* p) {
     p->criterion = EFS_CRIT_REFERENCE;
     p->precond = EFS_PRECOND_NONE;
     p->true_residual = 0;
+    p->initial_guess = EFS_GUESS_DIVERGENCE;
 }
 
 int efs_create_3d(int32_t width, int32_t height, int32_t depth, const efs_params* params, efs_handle* out) {
@@ -414,6 +415,14 @@ int efs_solve_pressure(efs_handle h, efs_stats* st) {
     EFS_HIP(hipStreamSynchronize(h->stream));
     out->status = rc;
     return rc;
+}
+
+int efs_project_divergence(efs_handle h) {
+    if (!valid_handle(h)) return -1;
+    const int rc = efs_launch_walls_divergence(h);
+    if (rc) return rc;
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    return 0;
 }
 
 int efs_apply_operator(efs_handle h, const float* x, float* y) {

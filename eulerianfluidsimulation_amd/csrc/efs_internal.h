@@ -229,6 +229,8 @@ struct CGState {
     int flags;         // bit0: a border cell is a fluid row
     int stall;         // learned preconditioner: stall detected
     int pad;
+    double sigma;      // learned preconditioner: RMS of the restricted vector
+    double dq;         // flexible CG: d_k . A d_k of the last iteration
 };
 
 struct efs_sim {
@@ -254,6 +256,7 @@ struct efs_sim {
     float* cg_xd = nullptr;     // FP32 increment of the iterate since the last fold
     double* cg_x64 = nullptr;   // FP64 iterate (reliable-update accumulator)
     bool p64_valid = false;     // the last solve left its FP64 iterate in cg_x64
+    bool x64_has_previous = false;  // cg_x64 holds a finished solve (usable as the next initial guess)
     double* partials = nullptr;
     double *p_rr = nullptr, *p_bb = nullptr, *p_dad = nullptr;  // per-block partial sums of the CG kernels
     int max_blocks = 0;
@@ -301,6 +304,11 @@ int efs_solver_alloc(efs_sim* s);
 void efs_solver_free(efs_sim* s);
 int efs_launch_rowcode(efs_sim* s, int* border_rows);
 int efs_cg_solve(efs_sim* s, efs_stats* st);
+int efs_cg_run(efs_sim* s, efs_stats* st, bool border, int max_it);
+int efs_cg_setup_x0(efs_sim* s);                   // x64 = b, xd = 0
+int efs_cg_init_residual(efs_sim* s, bool border);  // r = b - A x64 (FP64), partials of r.r, b.b
+int efs_cg_fold_all(efs_sim* s);                    // x64 += xd, xd = 0
+int efs_cg_finalize(efs_sim* s);                    // x64 += xd, pressure = (float)x64
 int efs_jacobi_solve(efs_sim* s, efs_stats* st);
 int efs_launch_spmv(efs_sim* s, const float* x, float* y);
 int efs_true_residual(efs_sim* s, double* num, double* den);

@@ -34,148 +34,9 @@
 #include <cstring>
 
 #include "efs_internal.h"
+#include "efs_stencil.h"
 
 namespace {
-
-constexpr int TX = 64;
-
-struct Dom {
-    int W, H, D;        // grid dims
-    int x0, y0, z0;     // first cell of the launch domain
-    int nx, ny, nz;     // extent of the launch domain
-    int ty;             // tile height (rows per block)
-    int tiles_x, tiles_y, chunks_z, zc;
-    int nblocks;        // real blocks (the launch rounds up to a multiple of 8)
-};
-
-__device__ inline bool block_coords(const Dom& d, int& bx, int& by, int& bz) {
-    // XCD-aware renumbering: hardware block b runs on XCD b % 8; XCD k gets the contiguous
-    // logical range [k * per, (k + 1) * per).
-    const int b = blockIdx.x;
-    const int per = (d.nblocks + 7) / 8;
-    const int L = (b % 8) * per + b / 8;
-    if (L >= d.nblocks) return false;
-    bx = L % d.tiles_x;
-    const int r = L / d.tiles_x;
-    by = r % d.tiles_y;
-    bz = r / d.tiles_y;
-    return true;
-}
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-// Block sum deposited in partials[k * gridDim + block]; the last block to arrive reduces all
-// partials in a fixed order (deterministic).  Returns true in the last block (all threads).
-template <int NT, int NV>
-__device__ inline bool grid_reduce(double (&v)[NV], double* partials, unsigned int* counter, double (&out)[NV]) {
-    __shared__ double red[NV][NT / 64];
-    __shared__ bool last;
-    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        const double s = wave_sum(v[k]);
-        if ((lane & 63) == 0) red[k][lane >> 6] = s;
-    }
-    __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = 0;
-            for (int wv = 0; wv < NT / 64; wv++) s += red[k][wv];
-            partials[(size_t)k * gridDim.x + blockIdx.x] = s;
-        }
-        __threadfence();
-        const unsigned int t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double s = 0;
-        for (int i = lane; i < (int)gridDim.x; i += NT) s += __builtin_nontemporal_load(&partials[(size_t)k * gridDim.x + i]);
-        s = wave_sum(s);
-        __syncthreads();
-        if ((lane & 63) == 0) red[k][lane >> 6] = s;
-    }
-    __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = 0;
-            for (int wv = 0; wv < NT / 64; wv++) s += red[k][wv];
-            out[k] = s;
-        }
-        *counter = 0;
-    }
-    return true;
-}
-
-// Row of the operator.  The reference sums the contributing neighbours in the order
-// x-, x+, y-, y+, z+, z- starting from 0.0f, then subtracts nCount * x (no contraction).
-__device__ inline float apply_row(unsigned rc, float xc, float xm, float xp, float ym, float yp, float zp, float zm) {
-    if (!(rc & RC_CG_ROW)) return xc;
-    float sum = 0.0f;
-    if (rc & RC_NB_XM) sum += xm;
-    if (rc & RC_NB_XP) sum += xp;
-    if (rc & RC_NB_YM) sum += ym;
-    if (rc & RC_NB_YP) sum += yp;
-    if (rc & RC_NB_ZP) sum += zp;
-    if (rc & RC_NB_ZM) sum += zm;
-    return sum - (float)((rc >> RC_CNT_SHIFT) & 7u) * xc;
-}
-
-__device__ inline double apply_row_d(unsigned rc, double xc, double xm, double xp, double ym, double yp, double zp, double zm) {
-    if (!(rc & RC_CG_ROW)) return xc;
-    double sum = 0.0;
-    if (rc & RC_NB_XM) sum += xm;
-    if (rc & RC_NB_XP) sum += xp;
-    if (rc & RC_NB_YM) sum += ym;
-    if (rc & RC_NB_YP) sum += yp;
-    if (rc & RC_NB_ZP) sum += zp;
-    if (rc & RC_NB_ZM) sum += zm;
-    return sum - (double)((rc >> RC_CNT_SHIFT) & 7u) * xc;
-}
-
-struct Nb {  // neighbour offsets, clamped into the array on the outer layer (values unused there)
-    int xm, xp, ym, yp, zm, zp;
-};
-__device__ inline Nb nb_offsets(const Dom& d, int x, int y, int z) {
-    const int sy = d.W, sz = d.W * d.H;
-    Nb n;
-    n.xm = (x > 0) ? -1 : 0;
-    n.xp = (x < d.W - 1) ? 1 : 0;
-    n.ym = (y > 0) ? -sy : 0;
-    n.yp = (y < d.H - 1) ? sy : 0;
-    n.zm = (z > 0) ? -sz : 0;
-    n.zp = (z < d.D - 1) ? sz : 0;
-    return n;
-}
-
-// ---------------------------------------------------------------------------------------
-// Plane staging: thread (tx, ty) owns LDS slot (ty+1, tx+1); edge threads also fill the
-// one-cell halo.  Cells outside the array read 0.
-// ---------------------------------------------------------------------------------------
-template <int TY>
-struct Stager {
-    int hx_slot, hy_slot;  // LDS column / row of this thread's halo cell (-1 = none)
-    int hx_off, hy_off;    // array offset of the halo cell from the own cell
-    bool own_ok, hx_ok, hy_ok;
-    __device__ void init(const Dom& d, int x, int y) {
-        const int tx = threadIdx.x, ty = threadIdx.y;
-        own_ok = x < d.W && y < d.H;
-        hx_slot = -1; hy_slot = -1; hx_off = 0; hy_off = 0; hx_ok = false; hy_ok = false;
-        if (tx == 0) { hx_slot = 0; hx_off = -1; hx_ok = x - 1 >= 0 && x - 1 < d.W && y < d.H; }
-        if (tx == TX - 1) { hx_slot = TX + 1; hx_off = 1; hx_ok = x + 1 < d.W && y < d.H; }
-        if (ty == 0) { hy_slot = 0; hy_off = -d.W; hy_ok = y - 1 >= 0 && y - 1 < d.H && x < d.W; }
-        if (ty == TY - 1) { hy_slot = TY + 1; hy_off = d.W; hy_ok = y + 1 < d.H && x < d.W; }
-    }
-};
 
 // ---------------------------------------------------------------------------------------
 // row codes: neighbour counted = in range, not on the outer layer, boundary <= 0.5;
@@ -217,42 +78,13 @@ __global__ __launch_bounds__(256) void k_rowcode(DevGrid g, uint16_t* rc, CGStat
 // the same fixed order (bit-identical scalars in every block).  Block 0 records the scalars
 // in CGState for the host; the iteration index is a launch argument.
 // ---------------------------------------------------------------------------------------
-constexpr int CG_TY = 8, CG_NT = TX * CG_TY;
-
-// sum of the n partials of the previous kernel, identical in every block
-__device__ inline double sum_partials(const double* __restrict__ p, int n, double* red) {
-    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-    double s = 0.0;
-    for (int i = lane; i < n; i += CG_NT) s += p[i];
-    s = wave_sum(s);
-    if ((lane & 63) == 0) red[lane >> 6] = s;
-    __syncthreads();
-    double tot = 0.0;
-#pragma unroll
-    for (int w = 0; w < CG_NT / 64; w++) tot += red[w];
-    __syncthreads();
-    return tot;
-}
-
-// this block's total of v, written to p[blockIdx.x] (lane 0)
-__device__ inline void write_partial(double v, double* __restrict__ p, double* red) {
-    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-    v = wave_sum(v);
-    if ((lane & 63) == 0) red[lane >> 6] = v;
-    __syncthreads();
-    if (lane == 0) {
-        double tot = 0.0;
-#pragma unroll
-        for (int w = 0; w < CG_NT / 64; w++) tot += red[w];
-        p[blockIdx.x] = tot;
-    }
-}
-
-// whole array: x64 = b, xd = 0 (identity rows keep x0 = b, CGSolver.cpp:429)
-__global__ void k_cg_setup(int n, const float* __restrict__ b, double* __restrict__ x64, float* __restrict__ xd) {
+// whole array: x64 = b, xd = 0 (identity rows keep x0 = b, CGSolver.cpp:429); with `prev`, the
+// fluid rows keep the previous iterate (the previous step's pressure)
+__global__ void k_cg_setup(int n, const float* __restrict__ b, const uint16_t* __restrict__ rcode, int prev,
+                           double* __restrict__ x64, float* __restrict__ xd) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    x64[i] = (double)b[i];
+    if (!(prev && (rcode[i] & RC_CG_ROW))) x64[i] = (double)b[i];
     xd[i] = 0.f;
 }
 
@@ -268,7 +100,7 @@ __device__ inline bool cg_check(CGState* st, int it, const double* __restrict__ 
         bb = sum_partials(p_bb, gridDim.x, red);
         if (st->flags & 2) thr = thr * thr * bb;
     }
-    const bool conv = g_it < thr;
+    const bool conv = g_it < thr || g_it == 0.0;
     const bool stop = conv || !isfinite(g_it) || it >= st->max_iter;
     if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
         st->gam[it & 1] = g_it;
@@ -286,9 +118,11 @@ __device__ inline bool cg_check(CGState* st, int it, const double* __restrict__ 
     return stop;
 }
 
-// r0 = b - A b (FP64, rounded), d0 = r0; partials of r.r and b.b
+// r0 = b - A x0 (FP64, rounded; x0 = x64, which is b for the reference's initial guess), d0 = r0;
+// partials of r.r and b.b
 __global__ __launch_bounds__(CG_NT) void k_cg_init(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
-                                                   float* __restrict__ r, float* __restrict__ d0, double* p_rr, double* p_bb) {
+                                                   const double* __restrict__ x64, float* __restrict__ r, float* __restrict__ d0,
+                                                   double* p_rr, double* p_bb) {
     __shared__ double red[CG_NT / 64];
     int bx, by, bz;
     double vr = 0.0, vb = 0.0;
@@ -300,7 +134,8 @@ __global__ __launch_bounds__(CG_NT) void k_cg_init(Dom d, const uint16_t* __rest
                 const int c = (z * d.H + y) * d.W + x;
                 const Nb n = nb_offsets(d, x, y, z);
                 const double bc = b[c];
-                const double t = apply_row_d(rcode[c], bc, b[c + n.xm], b[c + n.xp], b[c + n.ym], b[c + n.yp], b[c + n.zp], b[c + n.zm]);
+                const double t = apply_row_d(rcode[c], x64[c], x64[c + n.xm], x64[c + n.xp], x64[c + n.ym], x64[c + n.yp],
+                                             x64[c + n.zp], x64[c + n.zm]);
                 const float rr = (float)(bc - t);
                 r[c] = rr;
                 d0[c] = rr;
@@ -503,7 +338,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_check(int nparts, int it, CGState*
         if (st->flags & 2) thr = thr * thr * bb;
     }
     if (lane == 0) {
-        const bool conv = g < thr;
+        const bool conv = g < thr || g == 0.0;
         st->gamma = g;
         st->gam[it & 1] = g;
         if (it == 0) {
@@ -525,6 +360,14 @@ __global__ void k_cg_finalize(int n, double* __restrict__ x64, const float* __re
     const double x = x64[i] + (double)xd[i];
     x64[i] = x;
     p[i] = (float)x;
+}
+
+// whole array: x64 += xd; xd = 0
+__global__ void k_cg_fold_all(int n, double* __restrict__ x64, float* __restrict__ xd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x64[i] = x64[i] + (double)xd[i];
+    xd[i] = 0.f;
 }
 
 // FP64 true residual ||b - A x||^2 and ||b||^2 (x in FP64 if given, else the FP32 field).
@@ -627,33 +470,6 @@ __global__ __launch_bounds__(CG_NT) void k_jacobi(Dom d, const uint16_t* __restr
         pc = np;
     }
 }
-
-Dom make_dom(const efs_sim* s, bool full, int ty, int target_blocks) {
-    Dom d;
-    d.W = s->W; d.H = s->H; d.D = s->D;
-    if (full) {
-        d.x0 = d.y0 = d.z0 = 0;
-        d.nx = s->W; d.ny = s->H; d.nz = s->D;
-    } else {
-        d.x0 = d.y0 = d.z0 = 1;
-        d.nx = s->W - 2; d.ny = s->H - 2; d.nz = s->D - 2;
-    }
-    d.ty = ty;
-    d.tiles_x = (d.nx + TX - 1) / TX;
-    d.tiles_y = (d.ny + ty - 1) / ty;
-    const int txy = d.tiles_x * d.tiles_y;
-    int zc = (int)std::max<long long>(2, std::min<long long>(32, (long long)d.nz * txy / target_blocks));
-    zc = std::min(zc, std::max(1, d.nz));
-    d.zc = zc;
-    d.chunks_z = (d.nz + zc - 1) / zc;
-    d.nblocks = txy * d.chunks_z;
-    return d;
-}
-Dom dom_simple(const efs_sim* s, bool full) { return make_dom(s, full, 4, 2048); }
-Dom dom_tiled(const efs_sim* s, bool full) { return make_dom(s, full, CG_TY, 2048); }
-
-int launch_blocks(const Dom& d) { return ((d.nblocks + 7) / 8) * 8; }
-inline int blocks1d(int n) { return (n + 255) / 256; }
 
 }  // namespace
 
@@ -763,12 +579,52 @@ int efs_true_residual(efs_sim* s, double* num, double* den) {
     return 0;
 }
 
+int efs_cg_setup_x0(efs_sim* s) {
+    // the previous iterate is usable only if the last solve left one (CG, same grid)
+    const int prev = (s->P.initial_guess == EFS_GUESS_PREVIOUS && s->x64_has_previous) ? 1 : 0;
+    hipLaunchKernelGGL(k_cg_setup, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->div, s->rowcode, prev, s->cg_x64,
+                       s->cg_xd);
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_cg_init_residual(efs_sim* s, bool border) {
+    const Dom dt = dom_tiled(s, border);
+    hipLaunchKernelGGL(k_cg_init, dim3(launch_blocks(dt)), dim3(TX, CG_TY), 0, s->stream, dt, s->rowcode, s->div, s->cg_x64, s->cg_r,
+                       s->cg_d0, s->p_rr, s->p_bb);
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_cg_fold_all(efs_sim* s) {
+    hipLaunchKernelGGL(k_cg_fold_all, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->cg_x64, s->cg_xd);
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
+
+int efs_cg_finalize(efs_sim* s) {
+    hipLaunchKernelGGL(k_cg_finalize, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->cg_x64, s->cg_xd, s->p);
+    EFS_HIP(hipGetLastError());
+    s->p64_valid = true;
+    s->x64_has_previous = true;
+    return 0;
+}
+
 int efs_cg_solve(efs_sim* s, efs_stats* out) {
     int border = 0;
     int rc = efs_launch_rowcode(s, &border);
     if (rc) return rc;
+    // x0 = b (CGSolver.cpp:429)
+    if ((rc = efs_cg_setup_x0(s))) return rc;
+    return efs_cg_run(s, out, border != 0, s->P.max_steps);
+}
+
+// Plain mixed-precision CG from the FP64 iterate already in cg_x64 (xd must be 0), with the row
+// codes current; runs at most max_it iterations.
+int efs_cg_run(efs_sim* s, efs_stats* out, bool border, int max_it) {
+    int rc = 0;
     s->p64_valid = false;
-    const Dom dt = dom_tiled(s, border != 0);
+    const Dom dt = dom_tiled(s, border);
     const int nbt = launch_blocks(dt);
     if (nbt > s->max_blocks) {
         efs_set_error("launch exceeds the partial-sum buffers");
@@ -778,14 +634,11 @@ int efs_cg_solve(efs_sim* s, efs_stats* out) {
     init.thr_in = (double)s->P.solver_threshold;
     init.thr = init.thr_in;
     init.flags = (s->P.criterion == EFS_CRIT_RELATIVE) ? 2 : 0;
-    init.max_iter = s->P.max_steps;
+    init.max_iter = max_it;
     std::memcpy(s->st_host, &init, sizeof(CGState));
     EFS_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(CGState), hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_cg_setup, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->div, s->cg_x64, s->cg_xd);
     const dim3 bt(TX, CG_TY);
-    hipLaunchKernelGGL(k_cg_init, dim3(nbt), bt, 0, s->stream, dt, s->rowcode, s->div, s->cg_r, s->cg_d0, s->p_rr, s->p_bb);
-    EFS_HIP(hipGetLastError());
-    const int max_it = s->P.max_steps;
+    if ((rc = efs_cg_init_residual(s, border))) return rc;
     int launched = 0;
     const size_t kt0 = s->ktimers[KT_CG_DIR_SPMV].start.size(), kt1 = s->ktimers[KT_CG_UPDATE].start.size();
     while (true) {
@@ -822,9 +675,7 @@ int efs_cg_solve(efs_sim* s, efs_stats* out) {
     trim_timer(s->ktimers[KT_CG_DIR_SPMV], kt0 + fin.iter);
     const size_t repl = (size_t)(fin.iter + 63) / 64;  // replacement iterations have no K2
     trim_timer(s->ktimers[KT_CG_UPDATE], kt1 + (size_t)fin.iter - std::min<size_t>(repl, fin.iter));
-    hipLaunchKernelGGL(k_cg_finalize, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->cg_x64, s->cg_xd, s->p);
-    EFS_HIP(hipGetLastError());
-    s->p64_valid = true;
+    if ((rc = efs_cg_finalize(s))) return rc;
     if (out) {
         out->iterations = fin.iter;
         out->rr = fin.gamma;
@@ -847,6 +698,7 @@ int efs_jacobi_solve(efs_sim* s, efs_stats* out) {
     int rc = efs_launch_rowcode(s, &border);
     if (rc) return rc;
     s->p64_valid = false;
+    s->x64_has_previous = false;
     const Dom d = dom_tiled(s, border != 0);
     const int nb = launch_blocks(d);
     const size_t bytes = (size_t)s->N * 4;

@@ -1,0 +1,782 @@
+// 3D U-Net of the learned pressure preconditioner: forward, backward and Adam on gfx950.
+//
+// Layout: activations channels-last (NDHWC), channels a multiple of 16.  Convolutions are
+// implicit GEMMs on the matrix cores: v_mfma_f32_16x16x16_bf16 with BF16 operands and FP32
+// accumulation.  A wavefront computes 16 consecutive voxels of one row x every output channel;
+// per tap and per 16-channel input chunk it loads one A fragment (16 voxels x 16 channels) and
+// one B fragment (16 channels x 16 outputs) per output tile.
+//   forward        y[v][co] = b[co] + sum_tap sum_ci x[v + off(tap)][ci] * W[tap][co][ci]
+//   backward data  dx = the same kernel on dy with W flipped in space and transposed in channels
+//   backward W     dW[tap][co][ci] = sum_v dy[v][co] * x[v + off(tap)][ci]: per-block partials,
+//                  then a fixed-order reduction (deterministic)
+// Master weights, gradients and Adam moments are FP32; BF16 copies feed the matrix cores.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "efs_internal.h"
+#include "efs_unet.h"
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+__device__ inline uint16_t f2bf(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+template <typename T> __device__ inline bf16x4 load4(const T* p);
+template <> __device__ inline bf16x4 load4<uint16_t>(const uint16_t* p) {
+    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    bf16x4 r;
+    r[0] = (short)(v.x & 0xffff); r[1] = (short)(v.x >> 16);
+    r[2] = (short)(v.y & 0xffff); r[3] = (short)(v.y >> 16);
+    return r;
+}
+template <> __device__ inline bf16x4 load4<float>(const float* p) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    bf16x4 r;
+    r[0] = (short)f2bf(v.x); r[1] = (short)f2bf(v.y); r[2] = (short)f2bf(v.z); r[3] = (short)f2bf(v.w);
+    return r;
+}
+
+constexpr int MAXNT = 8;  // up to 128 output channels per wavefront
+
+// Convolution (kernel size 3 with zero padding, or 1).  One wavefront per 16-voxel row tile.
+template <typename TI>
+__global__ __launch_bounds__(256) void k_conv_fwd(const TI* __restrict__ in, int Cin, const uint16_t* __restrict__ w,
+                                                  const float* __restrict__ bias, int Cout, int D, int H, int W, int ks,
+                                                  int relu, int ch0, float* __restrict__ out_f, uint16_t* __restrict__ out_b) {
+    const int lane = threadIdx.x & 63;
+    const int tiles_x = (W + 15) >> 4;
+    const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long long ntiles = (long long)tiles_x * H * D;
+    if (t >= ntiles) return;
+    const int tx = (int)(t % tiles_x);
+    const long long rest = t / tiles_x;
+    const int y = (int)(rest % H), z = (int)(rest / H);
+    const int x0 = tx * 16;
+    const int row = lane & 15, kq = lane >> 4;
+    const int nt = Cout >> 4, nc = Cin >> 4;
+    floatx4 acc[MAXNT];
+#pragma unroll
+    for (int i = 0; i < MAXNT; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int ntap = ks == 3 ? 27 : 1;
+    for (int tap = 0; tap < ntap; tap++) {
+        const int dz = ks == 3 ? tap / 9 - 1 : 0, dy = ks == 3 ? (tap / 3) % 3 - 1 : 0, dx = ks == 3 ? tap % 3 - 1 : 0;
+        const int xx = x0 + row + dx, yy = y + dy, zz = z + dz;
+        const bool ok = xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D;
+        const long long base = (((long long)zz * H + yy) * W + xx) * Cin + kq * 4;
+        const uint16_t* wt = w + ((size_t)tap * Cout) * Cin + (size_t)row * Cin + kq * 4;
+        for (int cc = 0; cc < nc; cc++) {
+            bf16x4 a = ok ? load4<TI>(in + base + cc * 16) : bf16x4{0, 0, 0, 0};
+#pragma unroll
+            for (int n = 0; n < MAXNT; n++) {
+                if (n < nt) {
+                    const bf16x4 b = load4<uint16_t>(wt + (size_t)n * 16 * Cin + cc * 16);
+                    acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D[i][j]: lane j + 16 * (i / 4), element i % 4  (i = voxel row, j = output channel)
+#pragma unroll
+    for (int n = 0; n < MAXNT; n++) {
+        if (n >= nt) break;
+        const int co = n * 16 + row;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int x = x0 + kq * 4 + e;
+            if (x >= W) continue;
+            float v = acc[n][e] + bv;
+            if (relu) v = v > 0.f ? v : 0.f;
+            const long long vox = ((long long)z * H + y) * W + x;
+            const long long o = vox * Cout + co;
+            if (ch0) {  // compact output of channel 0 only
+                if (co == 0) out_f[vox] = v;
+                continue;
+            }
+            if (out_f) out_f[o] = v;
+            if (out_b) out_b[o] = f2bf(v);
+        }
+    }
+}
+
+// Weight gradient partials: block (tap, co tile, chunk) sums over its voxel rows.
+// dW[tap][co][ci] = sum_v dy[v][co] * x[v + off][ci]; A = dy^T (co x voxel), B = x (voxel x ci).
+template <typename TX>
+__global__ __launch_bounds__(64) void k_conv_wgrad(const uint16_t* __restrict__ dy, int Cout, const TX* __restrict__ x, int Cin,
+                                                   int D, int H, int W, int ks, int nchunks, float* __restrict__ part) {
+    const int lane = threadIdx.x;
+    const int ntap = ks == 3 ? 27 : 1;
+    const int tap = blockIdx.x % ntap;
+    const int cot = (blockIdx.x / ntap) % (Cout >> 4);
+    const int chunk = blockIdx.x / (ntap * (Cout >> 4));
+    const int dz = ks == 3 ? tap / 9 - 1 : 0, dyy = ks == 3 ? (tap / 3) % 3 - 1 : 0, dx = ks == 3 ? tap % 3 - 1 : 0;
+    const int row = lane & 15, kq = lane >> 4;
+    const int nci = Cin >> 4;
+    floatx4 acc[MAXNT];
+#pragma unroll
+    for (int i = 0; i < MAXNT; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int tiles_x = (W + 15) >> 4;
+    const long long ntiles = (long long)tiles_x * H * D;
+    for (long long t = chunk; t < ntiles; t += nchunks) {
+        const int tx = (int)(t % tiles_x);
+        const long long rest = t / tiles_x;
+        const int y = (int)(rest % H), z = (int)(rest / H);
+        // the 4 voxels (k = kq*4 + e) this lane supplies
+        bf16x4 a;
+        int src[4];
+        bool okx[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int xv = tx * 16 + kq * 4 + e;
+            const bool inb = xv < W;
+            const long long v = (((long long)z * H + y) * W + xv);
+            a[e] = inb ? (short)dy[v * Cout + cot * 16 + row] : (short)0;
+            const int xs = xv + dx, ys = y + dyy, zs = z + dz;
+            okx[e] = inb && xs >= 0 && xs < W && ys >= 0 && ys < H && zs >= 0 && zs < D;
+            src[e] = okx[e] ? (int)(((long long)zs * H + ys) * W + xs) : 0;
+        }
+#pragma unroll
+        for (int n = 0; n < MAXNT; n++) {
+            if (n >= nci) break;
+            bf16x4 b;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                float xv = 0.f;
+                if (okx[e]) {
+                    if constexpr (sizeof(TX) == 2) xv = bf2f(x[(size_t)src[e] * Cin + n * 16 + row]);
+                    else xv = x[(size_t)src[e] * Cin + n * 16 + row];
+                }
+                b[e] = (short)f2bf(xv);
+            }
+            acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+        }
+    }
+    // D[i = co][j = ci]: lane j + 16 * (i / 4), element i % 4
+#pragma unroll
+    for (int n = 0; n < MAXNT; n++) {
+        if (n >= nci) break;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int co = cot * 16 + kq * 4 + e, ci = n * 16 + row;
+            part[(((size_t)chunk * ntap + tap) * Cout + co) * Cin + ci] = acc[n][e];
+        }
+    }
+}
+
+__global__ void k_reduce_chunks(int n, int nchunks, const float* __restrict__ part, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunks; c++) s += part[(size_t)c * n + i];
+    out[i] = s;
+}
+
+// bias gradient: sum over voxels of dy[v][co] (fixed-order per channel)
+__global__ void k_bias_grad(const uint16_t* __restrict__ dy, int C, long long nvox, float* __restrict__ g) {
+    const int co = blockIdx.x;
+    float s = 0.f;
+    for (long long v = threadIdx.x; v < nvox; v += blockDim.x) s += bf2f(dy[v * C + co]);
+    __shared__ float red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) g[co] = red[0];
+}
+
+// dpre = dy * (y > 0) (y is the post-ReLU activation), as BF16 MFMA operand
+__global__ void k_relu_bwd(long long n, const float* __restrict__ dy, const uint16_t* __restrict__ y, int relu,
+                           uint16_t* __restrict__ dpre) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = dy[i];
+    dpre[i] = f2bf((!relu || bf2f(y[i]) > 0.f) ? g : 0.f);
+}
+
+// 2x2x2 average pool (dims of the output), channels-last
+__global__ void k_pool_fwd(const uint16_t* __restrict__ in, int C, int D, int H, int W, uint16_t* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)D * H * W * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    long long v = i / C;
+    const int x = (int)(v % W); v /= W;
+    const int y = (int)(v % H);
+    const int z = (int)(v / H);
+    const int W2 = W * 2, H2 = H * 2;
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) {
+        const long long q = (((long long)(2 * z + (k >> 2)) * H2 + (2 * y + ((k >> 1) & 1))) * W2 + (2 * x + (k & 1))) * C + c;
+        s += bf2f(in[q]);
+    }
+    out[i] = f2bf(s * 0.125f);
+}
+// backward: d_in[child] += d_out[parent] / 8   (dims of the input / fine level)
+__global__ void k_pool_bwd(const float* __restrict__ dout, int C, int D, int H, int W, float* __restrict__ din) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)D * H * W * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    long long v = i / C;
+    const int x = (int)(v % W); v /= W;
+    const int y = (int)(v % H);
+    const int z = (int)(v / H);
+    din[i] += 0.125f * dout[(((long long)(z >> 1) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1)) * C + c];
+}
+// nearest 2x upsample of `a` (coarse, Ca channels) concatenated with `b` (fine, Cb channels)
+__global__ void k_up_concat(const uint16_t* __restrict__ a, int Ca, const uint16_t* __restrict__ b, int Cb, int D, int H, int W,
+                            uint16_t* __restrict__ out) {
+    const int C = Ca + Cb;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)D * H * W * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    long long v = i / C;
+    const int x = (int)(v % W); v /= W;
+    const int y = (int)(v % H);
+    const int z = (int)(v / H);
+    if (c < Ca) out[i] = a[(((long long)(z >> 1) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1)) * Ca + c];
+    else out[i] = b[(((long long)z * H + y) * W + x) * Cb + (c - Ca)];
+}
+// backward of k_up_concat: d_a[parent] += sum of children, d_b += own
+__global__ void k_up_concat_bwd_a(const float* __restrict__ dout, int Ca, int Cb, int D, int H, int W, float* __restrict__ da) {
+    // one thread per coarse element
+    const int C = Ca + Cb;
+    const int Dc = D >> 1, Hc = H >> 1, Wc = W >> 1;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)Dc * Hc * Wc * Ca;
+    if (i >= n) return;
+    const int c = (int)(i % Ca);
+    long long v = i / Ca;
+    const int x = (int)(v % Wc); v /= Wc;
+    const int y = (int)(v % Hc);
+    const int z = (int)(v / Hc);
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) {
+        const long long q = (((long long)(2 * z + (k >> 2)) * H + (2 * y + ((k >> 1) & 1))) * W + (2 * x + (k & 1))) * C + c;
+        s += dout[q];
+    }
+    da[i] += s;
+}
+__global__ void k_concat_bwd_b(const float* __restrict__ dout, int Ca, int Cb, long long nvox, float* __restrict__ db) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvox * Cb) return;
+    const int c = (int)(i % Cb);
+    const long long v = i / Cb;
+    db[i] += dout[v * (Ca + Cb) + Ca + c];
+}
+
+__global__ void k_to_bf16(long long n, const float* __restrict__ a, uint16_t* __restrict__ b) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = f2bf(a[i]);
+}
+__global__ void k_zero(long long n, float* a) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.f;
+}
+// W[tap][co][ci] (fp32) -> BF16 copy and the flipped/transposed copy Wt[26 - tap][ci][co]
+__global__ void k_prep_weights(int ntap, int Cout, int Cin, const float* __restrict__ w, uint16_t* __restrict__ wb,
+                               uint16_t* __restrict__ wt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = ntap * Cout * Cin;
+    if (i >= n) return;
+    const int ci = i % Cin, co = (i / Cin) % Cout, tap = i / (Cin * Cout);
+    const uint16_t v = f2bf(w[i]);
+    wb[i] = v;
+    wt[((size_t)(ntap - 1 - tap) * Cin + ci) * Cout + co] = v;
+}
+// loss on output channel 0: sum over voxels with mask of (y - t)^2 / count; dy = 2 (y - t) / count
+__global__ void k_mse(long long nvox, int C, const float* __restrict__ y, const float* __restrict__ t,
+                      const float* __restrict__ m, float inv_count, float* __restrict__ dy, double* __restrict__ part) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long long)gridDim.x * blockDim.x) {
+        const float w = m ? m[v] : 1.f;
+        const float e = (y[v * C] - t[v]) * w;
+        s += (double)e * e;
+        for (int c = 0; c < C; c++) dy[v * C + c] = (c == 0) ? 2.f * e * w * inv_count : 0.f;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void k_adam(int n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       float lr, float b1, float b2, float eps, float c1, float c2, const float* __restrict__ frozen) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (frozen && frozen[i] != 0.f) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+inline unsigned nb(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+// ======================================================================================
+// host side
+// ======================================================================================
+namespace efsu {
+
+static int hipfail(hipError_t e, const char* what) { return efs_hip_fail(e, what); }
+#define UHIP(call)                                            \
+    do {                                                      \
+        hipError_t _e = (call);                               \
+        if (_e != hipSuccess) return hipfail(_e, #call);      \
+    } while (0)
+
+struct Conv {
+    int cin, cout, ks, relu;
+    size_t woff, boff;  // offsets into the parameter vector (weights [tap][co][ci], bias [co])
+    size_t nw() const { return (size_t)(ks == 3 ? 27 : 1) * cout * cin; }
+};
+
+// Layer table of the 4-level U-Net (channels widths c0..c3, input 16 channels, output
+// channel 0 of the last layer).
+static std::vector<Conv> make_layers(const int c[4]) {
+    std::vector<Conv> L;
+    auto add = [&](int ci, int co, int ks, int relu) { L.push_back({ci, co, ks, relu, 0, 0}); };
+    add(16, c[0], 3, 1);            // 0 enc0a
+    add(c[0], c[0], 3, 1);          // 1 enc0b  -> e0
+    add(c[0], c[1], 3, 1);          // 2 enc1a (after pool)
+    add(c[1], c[1], 3, 1);          // 3 enc1b  -> e1
+    add(c[1], c[2], 3, 1);          // 4 enc2a
+    add(c[2], c[2], 3, 1);          // 5 enc2b  -> e2
+    add(c[2], c[3], 3, 1);          // 6 enc3a
+    add(c[3], c[3], 3, 1);          // 7 enc3b  -> e3
+    add(c[3] + c[2], c[2], 3, 1);   // 8 dec2 (up(e3) ++ e2)
+    add(c[2] + c[1], c[1], 3, 1);   // 9 dec1 (up(d2) ++ e1)
+    add(c[1] + c[0], c[0], 3, 1);   // 10 dec0 (up(d1) ++ e0)
+    add(c[0], 16, 3, 0);            // 11 out (channel 0 used)
+    size_t off = 0;
+    for (auto& l : L) {
+        l.woff = off;
+        off += l.nw();
+        l.boff = off;
+        off += l.cout;
+    }
+    return L;
+}
+
+}  // namespace efsu
+
+using namespace efsu;
+
+struct efs_unet_s {
+    int widths[4];
+    std::vector<Conv> layers;
+    size_t nparams = 0;
+    hipStream_t stream = nullptr;
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *frozen = nullptr;  // FP32 params / grads / moments
+    uint16_t *wb = nullptr, *wt = nullptr;                                          // BF16 copies (fwd / transposed)
+    bool wdirty = true;
+    long long adam_t = 0;
+    // activation buffers for the current dims
+    int D = 0, H = 0, W = 0;
+    std::vector<void*> bufs;
+    uint16_t* act[12] = {};     // post-activation outputs of layers 0..10 (BF16)
+    float* out = nullptr;       // layer 11 output (FP32, 16 channels)
+    float* out0 = nullptr;      // layer 11 channel 0, compact (inference)
+    uint16_t *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;      // pooled inputs of enc1a/2a/3a
+    uint16_t *cat2 = nullptr, *cat1 = nullptr, *cat0 = nullptr;  // decoder inputs
+    uint16_t* inb = nullptr;    // BF16 input (16 channels)
+    // backward scratch
+    float *ga = nullptr, *gb = nullptr;  // FP32 gradient buffers (largest activation)
+    float *g_e[4] = {};                  // accumulated gradients of the skip tensors e0..e3
+    uint16_t* dpre = nullptr;
+    float* wpart = nullptr;
+    size_t wpart_n = 0;
+    double* lpart = nullptr;
+};
+
+namespace {
+
+int unet_alloc_dims(efs_unet_s* u, int D, int H, int W) {
+    if (u->D == D && u->H == H && u->W == W) return 0;
+    if (D % 8 || H % 8 || W % 8) {
+        efs_set_error("U-Net dims must be multiples of 8");
+        return -1;
+    }
+    for (void* p : u->bufs) hipFree(p);
+    u->bufs.clear();
+    u->D = D; u->H = H; u->W = W;
+    const int* c = u->widths;
+    auto A = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        u->bufs.push_back(p);
+        return p;
+    };
+    const size_t v0 = (size_t)D * H * W, v1 = v0 / 8, v2 = v1 / 8, v3 = v2 / 8;
+    const size_t lv[12] = {v0, v0, v1, v1, v2, v2, v3, v3, v2, v1, v0, v0};
+    for (int i = 0; i < 11; i++) u->act[i] = (uint16_t*)A(lv[i] * u->layers[i].cout * 2);
+    u->out = (float*)A(v0 * 16 * 4);
+    u->out0 = (float*)A(v0 * 4);
+    u->inb = (uint16_t*)A(v0 * 16 * 2);
+    u->p1 = (uint16_t*)A(v1 * c[0] * 2);
+    u->p2 = (uint16_t*)A(v2 * c[1] * 2);
+    u->p3 = (uint16_t*)A(v3 * c[2] * 2);
+    u->cat2 = (uint16_t*)A(v2 * (c[3] + c[2]) * 2);
+    u->cat1 = (uint16_t*)A(v1 * (c[2] + c[1]) * 2);
+    u->cat0 = (uint16_t*)A(v0 * (c[1] + c[0]) * 2);
+    size_t maxel = 0;
+    for (int i = 0; i < 12; i++) maxel = std::max(maxel, lv[i] * (size_t)u->layers[i].cin);
+    maxel = std::max(maxel, v0 * 16);
+    u->ga = (float*)A(maxel * 4);
+    u->gb = (float*)A(maxel * 4);
+    u->dpre = (uint16_t*)A(maxel * 2);
+    u->g_e[0] = (float*)A(v0 * c[0] * 4);
+    u->g_e[1] = (float*)A(v1 * c[1] * 4);
+    u->g_e[2] = (float*)A(v2 * c[2] * 4);
+    u->g_e[3] = (float*)A(v3 * c[3] * 4);
+    u->lpart = (double*)A(1024 * 8);
+    for (void* p : u->bufs)
+        if (!p) {
+            efs_set_error("U-Net activation allocation failed");
+            return -2;
+        }
+    return 0;
+}
+
+int prep_weights(efs_unet_s* u) {
+    if (!u->wdirty) return 0;
+    for (auto& l : u->layers) {
+        const int nt = l.ks == 3 ? 27 : 1;
+        hipLaunchKernelGGL(k_prep_weights, dim3(nb(l.nw())), dim3(256), 0, u->stream, nt, l.cout, l.cin, u->w + l.woff,
+                           u->wb + l.woff, u->wt + l.woff);
+    }
+    UHIP(hipGetLastError());
+    u->wdirty = false;
+    return 0;
+}
+
+template <typename TI>
+void conv_fwd(efs_unet_s* u, const Conv& l, const TI* in, int D, int H, int W, float* of, uint16_t* ob, int ch0 = 0) {
+    const long long tiles = (long long)((W + 15) / 16) * H * D;
+    hipLaunchKernelGGL(k_conv_fwd<TI>, dim3(nb(tiles, 4)), dim3(256), 0, u->stream, in, l.cin, u->wb + l.woff, u->w + l.boff,
+                       l.cout, D, H, W, l.ks, l.relu, ch0, of, ob);
+}
+
+// forward from the BF16 input already in u->inb; compact = write only channel 0 into u->out0
+int forward(efs_unet_s* u, bool compact = false) {
+    int rc = prep_weights(u);
+    if (rc) return rc;
+    const int D = u->D, H = u->H, W = u->W;
+    const int* c = u->widths;
+    const auto& L = u->layers;
+    conv_fwd<uint16_t>(u, L[0], u->inb, D, H, W, nullptr, u->act[0]);
+    conv_fwd<uint16_t>(u, L[1], u->act[0], D, H, W, nullptr, u->act[1]);
+    hipLaunchKernelGGL(k_pool_fwd, dim3(nb((long long)D / 2 * H / 2 * W / 2 * c[0])), dim3(256), 0, u->stream, u->act[1], c[0],
+                       D / 2, H / 2, W / 2, u->p1);
+    conv_fwd<uint16_t>(u, L[2], u->p1, D / 2, H / 2, W / 2, nullptr, u->act[2]);
+    conv_fwd<uint16_t>(u, L[3], u->act[2], D / 2, H / 2, W / 2, nullptr, u->act[3]);
+    hipLaunchKernelGGL(k_pool_fwd, dim3(nb((long long)D / 4 * H / 4 * W / 4 * c[1])), dim3(256), 0, u->stream, u->act[3], c[1],
+                       D / 4, H / 4, W / 4, u->p2);
+    conv_fwd<uint16_t>(u, L[4], u->p2, D / 4, H / 4, W / 4, nullptr, u->act[4]);
+    conv_fwd<uint16_t>(u, L[5], u->act[4], D / 4, H / 4, W / 4, nullptr, u->act[5]);
+    hipLaunchKernelGGL(k_pool_fwd, dim3(nb((long long)D / 8 * H / 8 * W / 8 * c[2])), dim3(256), 0, u->stream, u->act[5], c[2],
+                       D / 8, H / 8, W / 8, u->p3);
+    conv_fwd<uint16_t>(u, L[6], u->p3, D / 8, H / 8, W / 8, nullptr, u->act[6]);
+    conv_fwd<uint16_t>(u, L[7], u->act[6], D / 8, H / 8, W / 8, nullptr, u->act[7]);
+    hipLaunchKernelGGL(k_up_concat, dim3(nb((long long)D / 4 * H / 4 * W / 4 * (c[3] + c[2]))), dim3(256), 0, u->stream,
+                       u->act[7], c[3], u->act[5], c[2], D / 4, H / 4, W / 4, u->cat2);
+    conv_fwd<uint16_t>(u, L[8], u->cat2, D / 4, H / 4, W / 4, nullptr, u->act[8]);
+    hipLaunchKernelGGL(k_up_concat, dim3(nb((long long)D / 2 * H / 2 * W / 2 * (c[2] + c[1]))), dim3(256), 0, u->stream,
+                       u->act[8], c[2], u->act[3], c[1], D / 2, H / 2, W / 2, u->cat1);
+    conv_fwd<uint16_t>(u, L[9], u->cat1, D / 2, H / 2, W / 2, nullptr, u->act[9]);
+    hipLaunchKernelGGL(k_up_concat, dim3(nb((long long)D * H * W * (c[1] + c[0]))), dim3(256), 0, u->stream, u->act[9], c[1],
+                       u->act[1], c[0], D, H, W, u->cat0);
+    conv_fwd<uint16_t>(u, L[10], u->cat0, D, H, W, nullptr, u->act[10]);
+    if (compact) conv_fwd<uint16_t>(u, L[11], u->act[10], D, H, W, u->out0, nullptr, 1);
+    else conv_fwd<uint16_t>(u, L[11], u->act[10], D, H, W, u->out, nullptr);
+    UHIP(hipGetLastError());
+    return 0;
+}
+
+// Backward of one conv layer given dL/dy (FP32, post-activation gradient) in `gy`:
+// weight/bias gradients accumulate into u->g; if gx != null, dL/dx is written (not accumulated).
+template <typename TX>
+int conv_bwd(efs_unet_s* u, const Conv& l, const TX* x, const uint16_t* y_act, const float* gy, int D, int H, int W, float* gx) {
+    const long long nvox = (long long)D * H * W;
+    hipLaunchKernelGGL(k_relu_bwd, dim3(nb(nvox * l.cout)), dim3(256), 0, u->stream, nvox * l.cout, gy, y_act, l.relu, u->dpre);
+    const int ntap = l.ks == 3 ? 27 : 1;
+    const long long tiles = (long long)((W + 15) / 16) * H * D;
+    const int nchunks = (int)std::max<long long>(1, std::min<long long>(64, tiles / 8));
+    const size_t need = (size_t)nchunks * l.nw();
+    if (need > u->wpart_n) {
+        if (u->wpart) hipFree(u->wpart);
+        UHIP(hipMalloc(&u->wpart, need * 4));
+        u->wpart_n = need;
+    }
+    hipLaunchKernelGGL(k_conv_wgrad<TX>, dim3(ntap * (l.cout / 16) * nchunks), dim3(64), 0, u->stream, u->dpre, l.cout, x, l.cin,
+                       D, H, W, l.ks, nchunks, u->wpart);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(nb(l.nw())), dim3(256), 0, u->stream, (int)l.nw(), nchunks, u->wpart, u->g + l.woff);
+    hipLaunchKernelGGL(k_bias_grad, dim3(l.cout), dim3(256), 0, u->stream, u->dpre, l.cout, nvox, u->g + l.boff);
+    if (gx) {
+        // dx = conv(dpre, W flipped / transposed), no bias, no activation
+        hipLaunchKernelGGL(k_conv_fwd<uint16_t>, dim3(nb(tiles, 4)), dim3(256), 0, u->stream, u->dpre, l.cout, u->wt + l.woff,
+                           (const float*)nullptr, l.cin, D, H, W, l.ks, 0, 0, gx, (uint16_t*)nullptr);
+    }
+    UHIP(hipGetLastError());
+    return 0;
+}
+
+// backward from dL/dout (FP32, 16 channels, in u->ga); fills u->g
+int backward(efs_unet_s* u) {
+    const int D = u->D, H = u->H, W = u->W;
+    const int* c = u->widths;
+    const auto& L = u->layers;
+    const size_t v0 = (size_t)D * H * W, v1 = v0 / 8, v2 = v1 / 8, v3 = v2 / 8;
+    auto zero = [&](float* p, size_t n) { hipLaunchKernelGGL(k_zero, dim3(nb(n)), dim3(256), 0, u->stream, (long long)n, p); };
+    zero(u->g_e[0], v0 * c[0]);
+    zero(u->g_e[1], v1 * c[1]);
+    zero(u->g_e[2], v2 * c[2]);
+    zero(u->g_e[3], v3 * c[3]);
+    int rc;
+    // out layer (11): x = act[10]
+    if ((rc = conv_bwd<uint16_t>(u, L[11], u->act[10], nullptr, u->ga, D, H, W, u->gb))) return rc;
+    // dec0 (10): y = act[10], x = cat0 (up(d1) ++ e0); gb = dL/d act[10]
+    if ((rc = conv_bwd<uint16_t>(u, L[10], u->cat0, u->act[10], u->gb, D, H, W, u->ga))) return rc;
+    zero(u->gb, v1 * c[1]);
+    hipLaunchKernelGGL(k_up_concat_bwd_a, dim3(nb((long long)v1 * c[1])), dim3(256), 0, u->stream, u->ga, c[1], c[0], D, H, W, u->gb);
+    hipLaunchKernelGGL(k_concat_bwd_b, dim3(nb((long long)v0 * c[0])), dim3(256), 0, u->stream, u->ga, c[1], c[0], (long long)v0,
+                       u->g_e[0]);
+    // dec1 (9): y = act[9] (gb holds dL/d act[9]), x = cat1 (up(d2) ++ e1)
+    if ((rc = conv_bwd<uint16_t>(u, L[9], u->cat1, u->act[9], u->gb, D / 2, H / 2, W / 2, u->ga))) return rc;
+    zero(u->gb, v2 * c[2]);
+    hipLaunchKernelGGL(k_up_concat_bwd_a, dim3(nb((long long)v2 * c[2])), dim3(256), 0, u->stream, u->ga, c[2], c[1], D / 2, H / 2,
+                       W / 2, u->gb);
+    hipLaunchKernelGGL(k_concat_bwd_b, dim3(nb((long long)v1 * c[1])), dim3(256), 0, u->stream, u->ga, c[2], c[1], (long long)v1,
+                       u->g_e[1]);
+    // dec2 (8): y = act[8], x = cat2 (up(e3) ++ e2)
+    if ((rc = conv_bwd<uint16_t>(u, L[8], u->cat2, u->act[8], u->gb, D / 4, H / 4, W / 4, u->ga))) return rc;
+    hipLaunchKernelGGL(k_up_concat_bwd_a, dim3(nb((long long)v3 * c[3])), dim3(256), 0, u->stream, u->ga, c[3], c[2], D / 4, H / 4,
+                       W / 4, u->g_e[3]);
+    hipLaunchKernelGGL(k_concat_bwd_b, dim3(nb((long long)v2 * c[2])), dim3(256), 0, u->stream, u->ga, c[3], c[2], (long long)v2,
+                       u->g_e[2]);
+    // enc3b (7): y = act[7] = e3, x = act[6]
+    if ((rc = conv_bwd<uint16_t>(u, L[7], u->act[6], u->act[7], u->g_e[3], D / 8, H / 8, W / 8, u->ga))) return rc;
+    // enc3a (6): y = act[6], x = p3
+    if ((rc = conv_bwd<uint16_t>(u, L[6], u->p3, u->act[6], u->ga, D / 8, H / 8, W / 8, u->gb))) return rc;
+    hipLaunchKernelGGL(k_pool_bwd, dim3(nb((long long)v2 * c[2])), dim3(256), 0, u->stream, u->gb, c[2], D / 4, H / 4, W / 4,
+                       u->g_e[2]);
+    // enc2b (5), enc2a (4)
+    if ((rc = conv_bwd<uint16_t>(u, L[5], u->act[4], u->act[5], u->g_e[2], D / 4, H / 4, W / 4, u->ga))) return rc;
+    if ((rc = conv_bwd<uint16_t>(u, L[4], u->p2, u->act[4], u->ga, D / 4, H / 4, W / 4, u->gb))) return rc;
+    hipLaunchKernelGGL(k_pool_bwd, dim3(nb((long long)v1 * c[1])), dim3(256), 0, u->stream, u->gb, c[1], D / 2, H / 2, W / 2,
+                       u->g_e[1]);
+    // enc1b (3), enc1a (2)
+    if ((rc = conv_bwd<uint16_t>(u, L[3], u->act[2], u->act[3], u->g_e[1], D / 2, H / 2, W / 2, u->ga))) return rc;
+    if ((rc = conv_bwd<uint16_t>(u, L[2], u->p1, u->act[2], u->ga, D / 2, H / 2, W / 2, u->gb))) return rc;
+    hipLaunchKernelGGL(k_pool_bwd, dim3(nb((long long)v0 * c[0])), dim3(256), 0, u->stream, u->gb, c[0], D, H, W, u->g_e[0]);
+    // enc0b (1), enc0a (0) (no input gradient needed)
+    if ((rc = conv_bwd<uint16_t>(u, L[1], u->act[0], u->act[1], u->g_e[0], D, H, W, u->ga))) return rc;
+    if ((rc = conv_bwd<uint16_t>(u, L[0], u->inb, u->act[0], u->ga, D, H, W, nullptr))) return rc;
+    UHIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+// ======================================================================================
+// C ABI
+// ======================================================================================
+extern "C" {
+
+int efs_unet_create(const int32_t* widths, uint64_t seed, efs_unet* out) {
+    if (!out) return -1;
+    *out = nullptr;
+    efs_unet_s* u = new efs_unet_s();
+    const int def[4] = {16, 32, 48, 64};
+    for (int i = 0; i < 4; i++) {
+        u->widths[i] = widths ? widths[i] : def[i];
+        if (u->widths[i] <= 0 || u->widths[i] % 16 || u->widths[i] > 128) {
+            delete u;
+            efs_set_error("U-Net widths must be positive multiples of 16, at most 128");
+            return -1;
+        }
+    }
+    u->layers = make_layers(u->widths);
+    for (auto& l : u->layers)
+        if (l.cin > 128 || l.cout > 128) {
+            delete u;
+            efs_set_error("U-Net layer wider than 128 channels");
+            return -1;
+        }
+    u->nparams = u->layers.back().boff + u->layers.back().cout;
+    hipError_t e = hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete u;
+        return efs_hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    const size_t n = u->nparams;
+    if (hipMalloc(&u->w, n * 4) || hipMalloc(&u->g, n * 4) || hipMalloc(&u->m, n * 4) || hipMalloc(&u->v, n * 4) ||
+        hipMalloc(&u->frozen, n * 4) || hipMalloc(&u->wb, n * 2) || hipMalloc(&u->wt, n * 2)) {
+        efs_set_error("U-Net parameter allocation failed");
+        return -2;
+    }
+    // He-style init from the counter generator; the unused output channels 1..15 of the last
+    // layer start at zero and are frozen (their gradients are zero anyway)
+    std::vector<float> h(n, 0.f), fr(n, 0.f);
+    uint64_t ctr = 0;
+    auto uni = [&]() {
+        uint64_t z = seed * 0x9E3779B97F4A7C15ull + (++ctr) * 0xD1B54A32D192ED03ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * 0x1.0p-53;
+    };
+    for (size_t li = 0; li < u->layers.size(); li++) {
+        const Conv& l = u->layers[li];
+        const double fan_in = (double)(l.ks == 3 ? 27 : 1) * l.cin;
+        const double a = std::sqrt(6.0 / fan_in) * (l.relu ? 1.0 : 0.5);
+        for (size_t i = 0; i < l.nw(); i++) {
+            const int co = (int)((i / l.cin) % l.cout);
+            const bool dead = (li == u->layers.size() - 1) && co != 0;
+            h[l.woff + i] = dead ? 0.f : (float)((2.0 * uni() - 1.0) * a);
+            if (dead) fr[l.woff + i] = 1.f;
+        }
+        for (int co = 0; co < l.cout; co++) {
+            const bool dead = (li == u->layers.size() - 1) && co != 0;
+            h[l.boff + co] = 0.f;
+            if (dead) fr[l.boff + co] = 1.f;
+        }
+    }
+    hipMemcpy(u->w, h.data(), n * 4, hipMemcpyHostToDevice);
+    hipMemcpy(u->frozen, fr.data(), n * 4, hipMemcpyHostToDevice);
+    hipMemset(u->g, 0, n * 4);
+    hipMemset(u->m, 0, n * 4);
+    hipMemset(u->v, 0, n * 4);
+    u->wdirty = true;
+    *out = u;
+    return 0;
+}
+
+int efs_unet_destroy(efs_unet u) {
+    if (!u) return 0;
+    hipStreamSynchronize(u->stream);
+    for (void* p : u->bufs) hipFree(p);
+    hipFree(u->w); hipFree(u->g); hipFree(u->m); hipFree(u->v); hipFree(u->frozen); hipFree(u->wb); hipFree(u->wt);
+    if (u->wpart) hipFree(u->wpart);
+    hipStreamDestroy(u->stream);
+    delete u;
+    return 0;
+}
+
+int64_t efs_unet_num_params(efs_unet u) { return u ? (int64_t)u->nparams : -1; }
+
+int efs_unet_get_params(efs_unet u, float* host) {
+    if (!u || !host) return -1;
+    UHIP(hipMemcpyAsync(host, u->w, u->nparams * 4, hipMemcpyDeviceToHost, u->stream));
+    UHIP(hipStreamSynchronize(u->stream));
+    return 0;
+}
+int efs_unet_set_params(efs_unet u, const float* host) {
+    if (!u || !host) return -1;
+    UHIP(hipMemcpyAsync(u->w, host, u->nparams * 4, hipMemcpyHostToDevice, u->stream));
+    UHIP(hipStreamSynchronize(u->stream));
+    u->wdirty = true;
+    return 0;
+}
+int efs_unet_get_grads(efs_unet u, float* host) {
+    if (!u || !host) return -1;
+    UHIP(hipMemcpyAsync(host, u->g, u->nparams * 4, hipMemcpyDeviceToHost, u->stream));
+    UHIP(hipStreamSynchronize(u->stream));
+    return 0;
+}
+
+// Input: device FP32 channels-last (D, H, W, 16).  Output: device FP32 (D, H, W) = channel 0.
+int efs_unet_forward_device(efs_unet u, const float* in, int32_t D, int32_t H, int32_t W, float* out) {
+    if (!u || !in || !out) return -1;
+    int rc = unet_alloc_dims(u, D, H, W);
+    if (rc) return rc;
+    const long long n = (long long)D * H * W * 16;
+    hipLaunchKernelGGL(k_to_bf16, dim3(nb(n)), dim3(256), 0, u->stream, n, in, u->inb);
+    if ((rc = forward(u, true))) return rc;
+    UHIP(hipMemcpyAsync(out, u->out0, (size_t)D * H * W * 4, hipMemcpyDeviceToDevice, u->stream));
+    UHIP(hipStreamSynchronize(u->stream));
+    return 0;
+}
+
+// Loss (mean over voxels with mask > 0 of mask * (y - t)^2) and its parameter gradient.
+// in: (D,H,W,16), target: (D,H,W), mask: (D,H,W) or null; device FP32.  Gradients replace u->g.
+int efs_unet_loss_grad_device(efs_unet u, const float* in, const float* target, const float* mask, int32_t D, int32_t H,
+                              int32_t W, double* loss) {
+    if (!u || !in || !target) return -1;
+    int rc = unet_alloc_dims(u, D, H, W);
+    if (rc) return rc;
+    const long long nvox = (long long)D * H * W;
+    hipLaunchKernelGGL(k_to_bf16, dim3(nb(nvox * 16)), dim3(256), 0, u->stream, nvox * 16, in, u->inb);
+    if ((rc = forward(u))) return rc;
+    const float inv = 1.f / (float)nvox;
+    hipLaunchKernelGGL(k_mse, dim3(1024), dim3(256), 0, u->stream, nvox, 16, u->out, target, mask, inv, u->ga, u->lpart);
+    if ((rc = backward(u))) return rc;
+    std::vector<double> lp(1024);
+    UHIP(hipMemcpyAsync(lp.data(), u->lpart, 1024 * 8, hipMemcpyDeviceToHost, u->stream));
+    UHIP(hipStreamSynchronize(u->stream));
+    double s = 0;
+    for (double x : lp) s += x;
+    if (loss) *loss = s / (double)nvox;
+    return 0;
+}
+
+int efs_unet_adam_step(efs_unet u, float lr) {
+    if (!u) return -1;
+    u->adam_t++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)u->adam_t), c2 = 1.f - std::pow(b2, (float)u->adam_t);
+    hipLaunchKernelGGL(k_adam, dim3(nb(u->nparams)), dim3(256), 0, u->stream, (int)u->nparams, u->w, u->g, u->m, u->v, lr, b1,
+                       b2, 1e-8f, c1, c2, u->frozen);
+    UHIP(hipGetLastError());
+    u->wdirty = true;
+    return 0;
+}
+
+int efs_unet_synchronize(efs_unet u) {
+    if (!u) return -1;
+    UHIP(hipStreamSynchronize(u->stream));
+    return 0;
+}
+
+void* efs_unet_stream(efs_unet u) { return u ? (void*)u->stream : nullptr; }
+
+}  // extern "C"
+
+int efs_unet_forward_on(efs_unet u, hipStream_t stream, int D, int H, int W, const float** out16) {
+    int rc = unet_alloc_dims(u, D, H, W);
+    if (rc) return rc;
+    hipStream_t saved = u->stream;
+    u->stream = stream;
+    rc = forward(u, true);
+    u->stream = saved;
+    if (rc) return rc;
+    *out16 = u->out0;
+    return 0;
+}
+int efs_unet_input_on(efs_unet u, int D, int H, int W, uint16_t** inb) {
+    int rc = unet_alloc_dims(u, D, H, W);
+    if (rc) return rc;
+    *inb = u->inb;
+    return 0;
+}

@@ -61,6 +61,12 @@ enum efs_criterion {
 
 enum efs_precond { EFS_PRECOND_NONE = 0, EFS_PRECOND_LEARNED = 1 };
 
+/* Initial CG guess on the fluid rows. */
+enum efs_initial_guess {
+    EFS_GUESS_DIVERGENCE = 0, /* x0 = b: the CPU backend (project clears p, CG sets p = b, CGSolver.cpp:429) */
+    EFS_GUESS_PREVIOUS = 1    /* x0 = last step's pressure: the CUDA backend (CudaSolver.cu:427)           */
+};
+
 /* The step's parameter set (Properties.h:136-218).  gravity_force is GRAVITY_FORCE in
  * cells/s^2 = GRAVITY_FORCE_PHYSICAL / CELL_SIZE (src/Application.cpp:89). */
 typedef struct {
@@ -80,6 +86,7 @@ typedef struct {
     int32_t criterion;        /* efs_criterion                                             */
     int32_t precond;          /* efs_precond                                               */
     int32_t true_residual;    /* 1 = FP64 true residual ||b - A p|| / ||b|| after each solve */
+    int32_t initial_guess;    /* efs_initial_guess                                         */
 } efs_params;
 
 typedef struct {
@@ -122,6 +129,9 @@ int efs_stage(efs_handle h, int32_t stage, efs_stats* stats);
 /* Reference: ISolver3D::solve (include/simulation/solver/ISolver.h:40), on the current
  * divergence; pressure is cleared first as project() does (CPUSimulator3D.cpp:513). */
 int efs_solve_pressure(efs_handle h, efs_stats* stats);
+/* The first half of project() (CPUSimulator3D.cpp:513-539): wall enforcement and the
+ * divergence with density correction, without the solve. */
+int efs_project_divergence(efs_handle h);
 /* Reference: CGSolverCPU3D::impl_mat_mul (CGSolver.cpp:467-497) on device vectors of N floats. */
 int efs_apply_operator(efs_handle h, const float* x_device, float* y_device);
 
@@ -150,6 +160,49 @@ typedef struct {
 int efs_scene_rasterise(efs_handle h, const efs_prim* prims, int32_t nprims);
 /* N(0, sigma) noise on faces that touch no solid, counter-based from seed (stream 1). */
 int efs_scene_face_noise(efs_handle h, uint64_t seed, double sigma);
+
+/* ---- Learned preconditioner (no counterpart in the reference: it accelerates the solve that
+ * CGSolverCPU3D::solve performs, CGSolver.cpp:416-465) ------------------------------------
+ * A 4-level 3D U-Net (channel widths w0..w3, 3x3x3 convolutions, BF16 matrix cores with FP32
+ * accumulation) on the half-resolution residual, used inside flexible CG with a warm start and
+ * a fallback to plain CG.  Network tensors are channels-last FP32 on the device. */
+typedef struct efs_unet_s* efs_unet;
+int efs_unet_create(const int32_t widths[4], uint64_t seed, efs_unet* out);  /* widths NULL = 16,32,48,64 */
+int efs_unet_destroy(efs_unet u);
+int64_t efs_unet_num_params(efs_unet u);
+int efs_unet_get_params(efs_unet u, float* host);
+int efs_unet_set_params(efs_unet u, const float* host);
+int efs_unet_get_grads(efs_unet u, float* host);
+/* in: (D, H, W, 16), out: (D, H, W); D, H, W multiples of 8 */
+int efs_unet_forward_device(efs_unet u, const float* in, int32_t D, int32_t H, int32_t W, float* out);
+/* loss = mean over voxels of mask * (out - target)^2 (mask NULL = 1); gradients replace the
+ * gradient buffer (efs_unet_get_grads) */
+int efs_unet_loss_grad_device(efs_unet u, const float* in, const float* target, const float* mask, int32_t D, int32_t H,
+                              int32_t W, double* loss);
+int efs_unet_adam_step(efs_unet u, float lr);
+int efs_unet_synchronize(efs_unet u);
+void* efs_unet_stream(efs_unet u);
+
+typedef struct {
+    float out_scale;      /* network output unit: pressure = out_scale * sigma * prediction   */
+    float jacobi_weight;  /* weight of the diagonal term - r / nCount                         */
+    int32_t warm_start;   /* x0 = M(b) instead of b                                            */
+    int32_t max_iters;    /* flexible iterations before falling back to plain CG (0 = max_steps) */
+    int32_t every;        /* network applied every `every` flexible iterations (the others use the
+                             diagonal term only); 0 = warm start only, then plain CG     */
+    int32_t stall_window; /* fall back when r.r fails to drop below stall_ratio times its value */
+    float stall_ratio;    /*   stall_window iterations earlier (0 window = never)              */
+} efs_learned_opts;
+void efs_default_learned_opts(efs_learned_opts* o);
+/* Attaches a network (owned by the caller) as the preconditioner used when params.precond is
+ * EFS_PRECOND_LEARNED. */
+int efs_set_learned_preconditioner(efs_handle h, efs_unet net, const efs_learned_opts* opts);
+/* Half-resolution dims (interior / 2; the interior must be a multiple of 16 per axis). */
+int efs_learned_coarse_dims(efs_handle h, int32_t dims[3]);
+/* Training triple from the current divergence and the last CG solution: network input
+ * (nz, ny, nx, 16), target (nz, ny, nx) in units of out_scale * sigma_b, mask = fluid fraction;
+ * device FP32 buffers.  *sigma = RMS of the divergence. */
+int efs_learned_coarse_pair(efs_handle h, float out_scale, float* in16, float* target, float* mask, double* sigma);
 
 const char* efs_last_error(void);
 const char* efs_version(void);
