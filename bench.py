@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--cold-steps", type=int, default=3, help="extra timed steps with x0 = b, reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128, help="interior resolution of the CPU sample")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="before the run, one device-to-device copy of MIB MiB (a counter calibration kernel of "
+                         "known traffic for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -94,6 +97,16 @@ def main():
     p.precond = _abi.PRECOND_LEARNED if ARGS.precond == "learned" else _abi.PRECOND_NONE
     p.true_residual = 1
     p.initial_guess = _abi.GUESS_PREVIOUS if ARGS.guess == "previous" else _abi.GUESS_DIVERGENCE
+    if ARGS.copy_probe > 0:
+        import torch
+        n = ARGS.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: {ARGS.copy_probe} MiB read + {ARGS.copy_probe} MiB written", file=sys.stderr, flush=True)
+        del src, dst
     sim = HipSimulator3D(*sc.dims, params=p)
     t0 = time.perf_counter()
     sim.load_scene(sc)

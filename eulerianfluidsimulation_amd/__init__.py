@@ -1,10 +1,10 @@
 """MI355X (gfx950) HIP backend for the Eulerian liquid step of MarQMark/EulerianFluidSimulation.
 
-`HipSimulator3D` mirrors the reference's ISimulator3D backend interface
-(include/simulation/simulator/ISimulator.h:8-53): `update()` runs the seven stages in order
-with the DISABLE_* gating, the protected stage methods are exposed under the same names, and
-`grid()` gives the IGrid3D field accessors (include/simulation/data/IGrid.h:52-59) as host
-mirrors in the reference layout.  Every call goes through the C ABI of include/efs.h; the
+`HipSimulator3D` / `HipSimulator2D` mirror the reference's ISimulator3D / ISimulator2D backend
+interface (include/simulation/simulator/ISimulator.h:8-53): `update()` runs the seven stages in
+order with the DISABLE_* gating, the protected stage methods are exposed under the same names,
+and `grid()` gives the IGrid3D / IGrid2D field accessors (include/simulation/data/IGrid.h) as
+host mirrors in the reference layout.  Every call goes through the C ABI of include/efs.h; the
 library must be built (there is no CPU fallback).
 """
 import ctypes
@@ -15,7 +15,8 @@ from . import _abi
 from ._abi import (FIELD_BETA, FIELD_BOUNDARY, FIELD_DIVERGENCE, FIELD_FORCES, FIELD_GAMMA, FIELD_PRESSURE, FIELD_RHO,
                    FIELD_VEL, Params, Prim, Stats, EfsError, check)
 
-__all__ = ["HipSimulator3D", "HipGrid3D", "Params", "Stats", "EfsError", "default_params", "load_library"]
+__all__ = ["HipSimulator3D", "HipGrid3D", "HipSimulator2D", "HipGrid2D", "Params", "Stats", "EfsError", "default_params",
+           "load_library"]
 
 _FIELD_NAMES = {
     "vel": FIELD_VEL, "rho": FIELD_RHO, "gamma": FIELD_GAMMA, "beta": FIELD_BETA,
@@ -40,9 +41,12 @@ class HipGrid3D:
     def __init__(self, sim):
         self._sim = sim
 
-    def _get(self, f):
+    def _shape(self, f):
         W, H, D = self._sim.dims
-        shape = (D, H, W, 3) if f in (FIELD_VEL, FIELD_FORCES) else (D, H, W)
+        return (D, H, W, 3) if f in (FIELD_VEL, FIELD_FORCES) else (D, H, W)
+
+    def _get(self, f):
+        shape = self._shape(f)
         a = np.empty(shape, dtype=np.float32)
         check(self._sim.lib.efs_download(self._sim.h, f, a.ctypes.data, a.nbytes))
         return a
@@ -77,6 +81,14 @@ class HipGrid3D:
         return self._get(FIELD_BOUNDARY)
 
 
+class HipGrid2D(HipGrid3D):
+    """IGrid2D over the device fields: shape (H, W) or (H, W, 2) (Grid2D's y*W + x order)."""
+
+    def _shape(self, f):
+        W, H = self._sim.dims
+        return (H, W, 2) if f in (FIELD_VEL, FIELD_FORCES) else (H, W)
+
+
 class HipSimulator3D:
     """ISimulator3D(width, height, depth) on one GPU."""
 
@@ -87,6 +99,9 @@ class HipSimulator3D:
         h = ctypes.c_void_p()
         check(self.lib.efs_create_3d(width, height, depth, ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h
+        self._init_state()
+
+    def _init_state(self):
         self.last_stats = Stats()
         self.paused = False
         self.frames = 0
@@ -183,3 +198,23 @@ class HipSimulator3D:
 
     def synchronize(self):
         check(self.lib.efs_synchronize(self.h))
+
+
+class HipSimulator2D(HipSimulator3D):
+    """ISimulator2D(width, height) on one GPU with CPUSimulator2D's semantics
+    (src/simulation/simulator/CPUSimulator2D.cpp)."""
+
+    def __init__(self, width, height, params: Params = None):
+        self.lib = _abi.load()
+        self.dims = (int(width), int(height))
+        self.params = params if params is not None else default_params()
+        h = ctypes.c_void_p()
+        check(self.lib.efs_create_2d(width, height, ctypes.byref(self.params), ctypes.byref(h)))
+        self.h = h
+        self._init_state()
+
+    def grid(self) -> HipGrid2D:
+        return HipGrid2D(self)
+
+    def get_render_data(self):
+        return {"dim": 2, "grid": self.grid()}

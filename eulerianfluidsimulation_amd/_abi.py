@@ -37,6 +37,7 @@ class Params(ctypes.Structure):
         ("precond", ctypes.c_int32),
         ("true_residual", ctypes.c_int32),
         ("initial_guess", ctypes.c_int32),
+        ("re_calc_div", ctypes.c_int32),
     ]
 
 
@@ -79,6 +80,7 @@ class Prim(ctypes.Structure):
 SIGNATURES = {
     "efs_default_params": (None, [ctypes.POINTER(Params)]),
     "efs_create_3d": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
+    "efs_create_2d": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
     "efs_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "efs_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
     "efs_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),

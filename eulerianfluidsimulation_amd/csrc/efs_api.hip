@@ -37,6 +37,19 @@ __global__ void k_interleave(int n, const float* __restrict__ a, const float* __
     aos[3 * (size_t)i + 1] = b[i];
     aos[3 * (size_t)i + 2] = c[i];
 }
+// AoS float[2] (2D velocity / forces) <-> two SoA arrays
+__global__ void k_deinterleave2(int n, const float* __restrict__ aos, float* __restrict__ a, float* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    a[i] = aos[2 * (size_t)i];
+    b[i] = aos[2 * (size_t)i + 1];
+}
+__global__ void k_interleave2(int n, const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ aos) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    aos[2 * (size_t)i] = a[i];
+    aos[2 * (size_t)i + 1] = b[i];
+}
 __global__ void k_fill_f32(int n, float* a, float v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -53,7 +66,7 @@ bool valid_handle(efs_sim* s) {
 }
 
 size_t field_bytes(const efs_sim* s, int f) {
-    if (f == EFS_FIELD_VEL || f == EFS_FIELD_FORCES) return (size_t)s->N * 12;
+    if (f == EFS_FIELD_VEL || f == EFS_FIELD_FORCES) return (size_t)s->N * (s->dim == 2 ? 8 : 12);
     return (size_t)s->N * 4;
 }
 
@@ -74,10 +87,12 @@ int ensure_forces(efs_sim* s) {
     const size_t b = (size_t)s->N * 4;
     EFS_HIP(hipMalloc(&s->fu, b));
     EFS_HIP(hipMalloc(&s->fv, b));
-    EFS_HIP(hipMalloc(&s->fw, b));
     EFS_HIP(hipMemsetAsync(s->fu, 0, b, s->stream));
     EFS_HIP(hipMemsetAsync(s->fv, 0, b, s->stream));
-    EFS_HIP(hipMemsetAsync(s->fw, 0, b, s->stream));
+    if (s->dim == 3) {
+        EFS_HIP(hipMalloc(&s->fw, b));
+        EFS_HIP(hipMemsetAsync(s->fw, 0, b, s->stream));
+    }
     return 0;
 }
 
@@ -104,7 +119,17 @@ int transfer(efs_sim* s, int f, void* other, size_t bytes, bool upload, hipMemcp
         float* a = f == EFS_FIELD_VEL ? s->u : s->fu;
         float* b = f == EFS_FIELD_VEL ? s->v : s->fv;
         float* c = f == EFS_FIELD_VEL ? s->w : s->fw;
-        if (upload) {
+        if (s->dim == 2) {
+            if (upload) {
+                EFS_HIP(hipMemcpyAsync(s->stage_bytes, other, bytes, kind, s->stream));
+                hipLaunchKernelGGL(k_deinterleave2, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->stage_bytes, a, b);
+                EFS_HIP(hipGetLastError());
+            } else {
+                hipLaunchKernelGGL(k_interleave2, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, a, b, s->stage_bytes);
+                EFS_HIP(hipGetLastError());
+                EFS_HIP(hipMemcpyAsync(other, s->stage_bytes, bytes, kind, s->stream));
+            }
+        } else if (upload) {
             EFS_HIP(hipMemcpyAsync(s->stage_bytes, other, bytes, kind, s->stream));
             hipLaunchKernelGGL(k_deinterleave, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->stage_bytes, a, b, c);
             EFS_HIP(hipGetLastError());
@@ -179,14 +204,15 @@ int alloc_all(efs_sim* s) {
     return efs_solver_alloc(s);
 }
 
-// Grid3D ctor (Grid3D.cpp:10-48): gamma = 1, border solid with rho 0.
+// Grid3D / Grid2D ctors (Grid3D.cpp:10-48, Grid2D.cpp:11-60): gamma = 1, border solid with rho 0.
 int init_grid(efs_sim* s) {
     hipLaunchKernelGGL(k_fill_f32, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->gamma, 1.f);
     std::vector<float> bnd((size_t)s->N, 0.f);
+    const bool is3 = s->dim == 3;
     for (int z = 0; z < s->D; z++)
         for (int y = 0; y < s->H; y++)
             for (int x = 0; x < s->W; x++)
-                if (x == 0 || y == 0 || z == 0 || x == s->W - 1 || y == s->H - 1 || z == s->D - 1)
+                if (x == 0 || y == 0 || x == s->W - 1 || y == s->H - 1 || (is3 && (z == 0 || z == s->D - 1)))
                     bnd[((size_t)z * s->H + y) * s->W + x] = 1.f;
     EFS_HIP(hipMemcpyAsync(s->bndf, bnd.data(), bnd.size() * 4, hipMemcpyHostToDevice, s->stream));
     int rc = efs_launch_boundary_flags(s);
@@ -218,6 +244,7 @@ int run_project(efs_sim* s, efs_stats* st) {
 }
 
 int run_stage(efs_sim* s, int stage, efs_stats* st) {
+    if (s->dim == 2) return efs2_run_stage(s, stage, st);
     switch (stage) {
         case EFS_STAGE_VEL_DECAY: return efs_launch_decay(s);
         case EFS_STAGE_VEL_EXTRAPOLATE: return efs_launch_extrapolate(s);
@@ -277,13 +304,13 @@ void efs_default_params(efs_params* p) {
     p->initial_guess = EFS_GUESS_DIVERGENCE;
 }
 
-int efs_create_3d(int32_t width, int32_t height, int32_t depth, const efs_params* params, efs_handle* out) {
+static int create(int dim, int32_t width, int32_t height, int32_t depth, const efs_params* params, efs_handle* out) {
     if (!out) {
         efs_set_error("null output handle");
         return -1;
     }
     *out = nullptr;
-    if (width < 3 || height < 3 || depth < 3) {
+    if (width < 3 || height < 3 || (dim == 3 && depth < 3)) {
         efs_set_error("grid must be at least 3 cells per axis (one interior cell plus the border)");
         return -1;
     }
@@ -292,12 +319,21 @@ int efs_create_3d(int32_t width, int32_t height, int32_t depth, const efs_params
         efs_set_error("grid too large for 32-bit cell indexing of the 8-entry advection arrays");
         return -1;
     }
+    if (params && params->solver_type != EFS_SOLVER_SELF_CG && params->solver_type != EFS_SOLVER_JACOBI) {
+        efs_set_error("solver_type must be SELF_CG (2) or JACOBI (3)");
+        return -1;
+    }
+    if (params && dim == 2 && params->precond == EFS_PRECOND_LEARNED) {
+        efs_set_error("the learned preconditioner is 3D only");
+        return -1;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         efs_set_error("no HIP device available");
         return -3;
     }
     efs_sim* s = new efs_sim();
+    s->dim = dim;
     s->W = width; s->H = height; s->D = depth; s->N = (int)n;
     if (params) s->P = *params;
     else efs_default_params(&s->P);
@@ -307,7 +343,7 @@ int efs_create_3d(int32_t width, int32_t height, int32_t depth, const efs_params
         return efs_hip_fail(e, "hipStreamCreateWithFlags");
     }
     s->own_stream = true;
-    int rc = alloc_all(s);
+    int rc = dim == 2 ? efs2_alloc(s) : alloc_all(s);
     if (rc == 0) rc = init_grid(s);
     if (rc) {
         free_all(s);
@@ -316,6 +352,14 @@ int efs_create_3d(int32_t width, int32_t height, int32_t depth, const efs_params
     }
     *out = s;
     return 0;
+}
+
+int efs_create_3d(int32_t width, int32_t height, int32_t depth, const efs_params* params, efs_handle* out) {
+    return create(3, width, height, depth, params, out);
+}
+
+int efs_create_2d(int32_t width, int32_t height, const efs_params* params, efs_handle* out) {
+    return create(2, width, height, 1, params, out);
 }
 
 int efs_destroy(efs_handle h) {
@@ -329,7 +373,11 @@ int efs_destroy(efs_handle h) {
 int efs_set_params(efs_handle h, const efs_params* p) {
     if (!valid_handle(h) || !p) return -1;
     if (p->solver_type != EFS_SOLVER_SELF_CG && p->solver_type != EFS_SOLVER_JACOBI) {
-        efs_set_error("solver_type must be SELF_CG (2) or JACOBI (3) for the 3D backend");
+        efs_set_error("solver_type must be SELF_CG (2) or JACOBI (3)");
+        return -1;
+    }
+    if (h->dim == 2 && p->precond == EFS_PRECOND_LEARNED) {
+        efs_set_error("the learned preconditioner is 3D only");
         return -1;
     }
     h->P = *p;
@@ -408,7 +456,8 @@ int efs_solve_pressure(efs_handle h, efs_stats* st) {
     out->true_rel_residual = -1;
     EFS_HIP(hipMemsetAsync(h->p, 0, (size_t)h->N * 4, h->stream));
     int rc;
-    if (h->P.solver_type == EFS_SOLVER_JACOBI) rc = efs_jacobi_solve(h, out);
+    if (h->dim == 2) rc = efs2_solve(h, out);
+    else if (h->P.solver_type == EFS_SOLVER_JACOBI) rc = efs_jacobi_solve(h, out);
     else if (h->P.precond == EFS_PRECOND_LEARNED) rc = efs_learned_cg_solve(h, out);
     else rc = efs_cg_solve(h, out);
     if (rc < 0) return rc;
@@ -419,7 +468,7 @@ int efs_solve_pressure(efs_handle h, efs_stats* st) {
 
 int efs_project_divergence(efs_handle h) {
     if (!valid_handle(h)) return -1;
-    const int rc = efs_launch_walls_divergence(h);
+    const int rc = h->dim == 2 ? efs2_walls_divergence(h) : efs_launch_walls_divergence(h);
     if (rc) return rc;
     EFS_HIP(hipStreamSynchronize(h->stream));
     return 0;
@@ -427,7 +476,7 @@ int efs_project_divergence(efs_handle h) {
 
 int efs_apply_operator(efs_handle h, const float* x, float* y) {
     if (!valid_handle(h) || !x || !y) return -1;
-    const int rc = efs_launch_spmv(h, x, y);
+    const int rc = h->dim == 2 ? efs2_spmv(h, x, y) : efs_launch_spmv(h, x, y);
     if (rc) return rc;
     EFS_HIP(hipStreamSynchronize(h->stream));
     return 0;
@@ -491,24 +540,26 @@ int efs_scene_rasterise(efs_handle h, const efs_prim* prims, int32_t nprims) {
     EFS_HIP(hipMemcpyAsync(rho.data(), h->rho, rho.size() * 4, hipMemcpyDeviceToHost, h->stream));
     EFS_HIP(hipMemcpyAsync(bnd.data(), h->bndf, bnd.size() * 4, hipMemcpyDeviceToHost, h->stream));
     EFS_HIP(hipStreamSynchronize(h->stream));
-    const double hx = 1.0 / (W - 2), hy = 1.0 / (H - 2), hz = 1.0 / (D - 2);
+    const bool is3 = h->dim == 3;
+    const double hx = 1.0 / (W - 2), hy = 1.0 / (H - 2), hz = is3 ? 1.0 / (D - 2) : 1.0;
     for (int z = 0; z < D; z++)
         for (int y = 0; y < H; y++)
             for (int x = 0; x < W; x++) {
                 const size_t c = ((size_t)z * H + y) * W + x;
                 float r = rho[c], b = bnd[c];
-                if (x == 0 || y == 0 || z == 0 || x == W - 1 || y == H - 1 || z == D - 1) {
+                if (x == 0 || y == 0 || x == W - 1 || y == H - 1 || (is3 && (z == 0 || z == D - 1))) {
                     b = 1.f;
                     r = 0.f;
                 } else {
-                    const double cx = (x - 0.5) * hx, cy = (y - 0.5) * hy, cz = (z - 0.5) * hz;
+                    const double cx = (x - 0.5) * hx, cy = (y - 0.5) * hy, cz = is3 ? (z - 0.5) * hz : 0.0;
                     for (int k = 0; k < nprims; k++) {
                         const efs_prim& P = prims[k];
                         bool in;
                         if (P.kind == 0) {
-                            in = cx >= P.a[0] && cx < P.b[0] && cy >= P.a[1] && cy < P.b[1] && cz >= P.a[2] && cz < P.b[2];
+                            in = cx >= P.a[0] && cx < P.b[0] && cy >= P.a[1] && cy < P.b[1] &&
+                                 (!is3 || (cz >= P.a[2] && cz < P.b[2]));
                         } else {
-                            const double dx = cx - P.a[0], dy = cy - P.a[1], dz = cz - P.a[2];
+                            const double dx = cx - P.a[0], dy = cy - P.a[1], dz = is3 ? cz - P.a[2] : 0.0;
                             in = dx * dx + dy * dy + dz * dz <= P.b[0] * P.b[0];
                         }
                         if (!in) continue;
@@ -531,20 +582,22 @@ int efs_scene_rasterise(efs_handle h, const efs_prim* prims, int32_t nprims) {
 int efs_scene_face_noise(efs_handle h, uint64_t seed, double sigma) {
     if (!valid_handle(h)) return -1;
     const int W = h->W, H = h->H, D = h->D;
-    std::vector<float> bnd((size_t)h->N), vel((size_t)h->N * 3);
+    const int nc = h->dim == 3 ? 3 : 2;  // components per cell
+    std::vector<float> bnd((size_t)h->N), vel((size_t)h->N * nc);
     EFS_HIP(hipMemcpyAsync(bnd.data(), h->bndf, bnd.size() * 4, hipMemcpyDeviceToHost, h->stream));
     EFS_HIP(hipStreamSynchronize(h->stream));
     int rc = efs_download(h, EFS_FIELD_VEL, vel.data(), vel.size() * 4);
     if (rc) return rc;
-    for (int z = 1; z < D - 1; z++)
+    const int z0 = nc == 3 ? 1 : 0, z1 = nc == 3 ? D - 1 : 1;
+    for (int z = z0; z < z1; z++)
         for (int y = 1; y < H - 1; y++)
             for (int x = 1; x < W - 1; x++) {
                 const size_t c = ((size_t)z * H + y) * W + x;
                 if (bnd[c] == 1.f) continue;
-                const size_t nb[3] = {c - 1, c - (size_t)W, c - (size_t)W * H};
-                for (int k = 0; k < 3; k++) {
-                    if (bnd[nb[k]] == 1.f) continue;
-                    vel[c * 3 + k] += (float)(sigma * normal01(seed, 1, c * 3 + k));
+                for (int k = 0; k < nc; k++) {
+                    const size_t nb = k == 0 ? c - 1 : (k == 1 ? c - (size_t)W : c - (size_t)W * H);
+                    if (bnd[nb] == 1.f) continue;
+                    vel[c * nc + k] += (float)(sigma * normal01(seed, 1, c * nc + k));
                 }
             }
     return efs_upload(h, EFS_FIELD_VEL, vel.data(), vel.size() * 4);

@@ -234,6 +234,7 @@ struct CGState {
 };
 
 struct efs_sim {
+    int dim = 3;  // 2: CPUSimulator2D semantics, D = 1 (efs_2d.hip)
     int W, H, D, N;
     efs_params P;
     hipStream_t stream = nullptr;
@@ -312,6 +313,13 @@ int efs_cg_finalize(efs_sim* s);                    // x64 += xd, pressure = (fl
 int efs_jacobi_solve(efs_sim* s, efs_stats* st);
 int efs_launch_spmv(efs_sim* s, const float* x, float* y);
 int efs_true_residual(efs_sim* s, double* num, double* den);
+
+// 2D step (efs_2d.hip)
+int efs2_alloc(efs_sim* s);
+int efs2_run_stage(efs_sim* s, int stage, efs_stats* st);
+int efs2_solve(efs_sim* s, efs_stats* st);            // on the current divergence, p cleared by the caller
+int efs2_walls_divergence(efs_sim* s);
+int efs2_spmv(efs_sim* s, const float* x, float* y);
 
 // learned preconditioner (efs_learned.hip)
 int efs_learned_cg_solve(efs_sim* s, efs_stats* st);

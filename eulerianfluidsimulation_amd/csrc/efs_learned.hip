@@ -523,6 +523,10 @@ void efs_default_learned_opts(efs_learned_opts* o) {
 
 int efs_set_learned_preconditioner(efs_handle s, efs_unet net, const efs_learned_opts* opts) {
     if (!s) return -1;
+    if (s->dim != 3) {
+        efs_set_error("the learned preconditioner is 3D only");
+        return -1;
+    }
     Learned* L = get(s);
     if (!L) {
         L = new Learned();
@@ -536,7 +540,7 @@ int efs_set_learned_preconditioner(efs_handle s, efs_unet net, const efs_learned
 
 int efs_learned_coarse_dims(efs_handle s, int32_t dims[3]) {
     if (!s || !dims) return -1;
-    if (!coarse_ok(s)) {
+    if (s->dim != 3 || !coarse_ok(s)) {
         efs_set_error("the learned preconditioner needs an interior that is a multiple of 16 per axis");
         return -1;
     }
@@ -549,7 +553,7 @@ int efs_learned_coarse_dims(efs_handle s, int32_t dims[3]) {
 // buffers: in16 (nz, ny, nx, 16), target and mask (nz, ny, nx)); *sigma = RMS of b.
 int efs_learned_coarse_pair(efs_handle s, float out_scale, float* in16, float* target, float* mask, double* sigma) {
     if (!s || !in16 || !target || !mask) return -1;
-    if (!coarse_ok(s)) {
+    if (s->dim != 3 || !coarse_ok(s)) {
         efs_set_error("the learned preconditioner needs an interior that is a multiple of 16 per axis");
         return -1;
     }

@@ -12,7 +12,8 @@
 // Kernel shape.  A block is a 64 x 8 tile of an xy plane (8 wavefronts, one per row) marching
 // along z over a chunk of planes.  The plane's lateral neighbours are staged in LDS with a
 // one-cell halo (double-buffered, one barrier per plane); the z neighbours of the thread's own
-// column stay in registers; the loads for plane z+1 are issued before plane z is computed.
+// column stay in registers; the loads of the next K1_PF / K2_PF planes are in flight while plane z is
+// computed.
 // Blocks are renumbered so each XCD owns a contiguous run of tiles (workgroups are dealt to
 // the 8 XCDs round-robin), keeping the halo rows shared by neighbouring tiles in one L2.
 //
@@ -149,7 +150,44 @@ __global__ __launch_bounds__(CG_NT) void k_cg_init(Dom d, const uint16_t* __rest
     write_partial(vb, p_bb, red);
 }
 
-// K1 (iteration it): convergence test, d_it = r_it + beta d_{it-1}, t = A d_it, partial d.t
+// Loads of one plane zz, issued K1_PF / K2_PF planes ahead of the plane being computed so that
+// each wavefront keeps that many planes of loads in flight (one plane per wavefront leaves the
+// kernel latency-bound at about a third of HBM bandwidth).  The ring slots are indexed with
+// compile-time constants (the plane loop is unrolled by the ring depth), so no slot is ever copied
+// while its loads are outstanding.
+#ifndef EFS_K1_PF
+#define EFS_K1_PF 2
+#endif
+#ifndef EFS_K2_PF
+#define EFS_K2_PF 2
+#endif
+constexpr int K1_PF = EFS_K1_PF, K2_PF = EFS_K2_PF;
+
+struct K1Plane {
+    uint16_t rc;                        // row code of the own cell (widened where it is used)
+    float od, orr, hxd, hxr, hyd, hyr;  // d_{i-1} and r of the own cell and of the halo cells
+};
+
+// Every load is unconditional: the address is clamped into the array and a value that is not
+// needed (a plane past the chunk, a halo cell outside the grid, a lane outside the domain) is
+// loaded and never used -- a row code bit never points at such a cell.  A load under a branch,
+// or a select right after a load, would make the compiler wait for all outstanding loads.
+__device__ inline int clampi(int c, int hi) { return min(max(c, 0), hi); }
+
+__device__ inline void k1_load(K1Plane& q, int c, int last, const Stager<CG_TY>& S, const float* __restrict__ dsrc,
+                               const float* __restrict__ r, const uint16_t* __restrict__ rcode) {
+    const int co = clampi(c, last), cx = clampi(c + S.hx_off, last), cy = clampi(c + S.hy_off, last);
+    q.rc = rcode[co];
+    q.od = dsrc[co];
+    q.orr = r[co];
+    q.hxd = dsrc[cx];
+    q.hxr = r[cx];
+    q.hyd = dsrc[cy];
+    q.hyr = r[cy];
+}
+
+// K1 (iteration it): convergence test, d_it = r_it + beta d_{it-1}, t = A d_it, partial d.t.
+// Iteration 0 has beta = 0 and stages d_0 = r_0 (k_cg_init stored it in d0 as well).
 __global__ __launch_bounds__(CG_NT) void k_cg_dir_spmv(Dom d, int it, const uint16_t* __restrict__ rcode, const float* __restrict__ r,
                                                        float* d0, float* d1, CGState* st, const double* p_rr,
                                                        const double* p_bb, double* p_dad) {
@@ -160,13 +198,10 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir_spmv(Dom d, int it, const uint
     const double beta = (it == 0) ? 0.0 : g_it / st->gam[(it + 1) & 1];
     const bool first = (it == 0);
     float* dnew = (it & 1) ? d1 : d0;
-    const float* dold = (it & 1) ? d0 : d1;
+    const float* dsrc = first ? dnew : ((it & 1) ? d0 : d1);
     __shared__ float sh[2][CG_TY + 2][TX + 2];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    auto Dv = [&](int c) -> float {
-        if (first) return dnew[c];
-        return (float)((double)dold[c] * beta + (double)r[c]);
-    };
+    auto Dv = [&](float dold, float rr) -> float { return (float)((double)dold * beta + (double)rr); };
     int bx, by, bz;
     double v = 0.0;
     if (block_coords(d, bx, by, bz)) {
@@ -175,40 +210,73 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir_spmv(Dom d, int it, const uint
         Stager<CG_TY> S;
         S.init(d, x, y);
         const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
-        const int sz = d.W * d.H;
+        const int sz = d.W * d.H, last = sz * d.D - 1;
         int c = (zb * d.H + y) * d.W + x;
-        float dm = (zb > 0 && S.own_ok) ? Dv(c - sz) : 0.f;
-        float dc = S.own_ok ? Dv(c) : 0.f;
+        const int cm = clampi(c - sz, last);
+        float dm = Dv(dsrc[cm], r[cm]);
+        K1Plane q0;
+        k1_load(q0, c, last, S, dsrc, r, rcode);
+        K1Plane ring[K1_PF];
+#pragma unroll
+        for (int k = 0; k < K1_PF; k++) k1_load(ring[k], c + (k + 1) * sz, last, S, dsrc, r, rcode);
+        float dc = Dv(q0.od, q0.orr);
+        unsigned rc_cur = q0.rc;
         sh[0][ty + 1][tx + 1] = dc;
-        if (S.hx_slot >= 0) sh[0][ty + 1][S.hx_slot] = S.hx_ok ? Dv(c + S.hx_off) : 0.f;
-        if (S.hy_slot >= 0) sh[0][S.hy_slot][tx + 1] = S.hy_ok ? Dv(c + S.hy_off) : 0.f;
-        __syncthreads();
+        if (S.hx_slot >= 0) sh[0][ty + 1][S.hx_slot] = Dv(q0.hxd, q0.hxr);
+        if (S.hy_slot >= 0) sh[0][S.hy_slot][tx + 1] = Dv(q0.hyd, q0.hyr);
+        lds_barrier();
         int cur = 0;
-        for (int z = zb; z < ze; z++, c += sz) {
-            const bool has_next = z + 1 < d.D;
-            const float np = (has_next && S.own_ok) ? Dv(c + sz) : 0.f;
-            const float nhx = (has_next && S.hx_ok) ? Dv(c + sz + S.hx_off) : 0.f;
-            const float nhy = (has_next && S.hy_ok) ? Dv(c + sz + S.hy_off) : 0.f;
-            if (in_dom) {
-                const unsigned rc = rcode[c];
-                float t = dc;
-                if (rc & RC_CG_ROW)
-                    t = apply_row(rc, dc, sh[cur][ty + 1][tx], sh[cur][ty + 1][tx + 2], sh[cur][ty][tx + 1],
-                                  sh[cur][ty + 2][tx + 1], np, dm);
-                if (!first) dnew[c] = dc;
-                v += (double)dc * (double)t;
+        int z = zb;
+        while (z < ze) {
+#pragma unroll
+            for (int k = 0; k < K1_PF; k++) {
+                if (z < ze) {
+                    K1Plane& q = ring[k];  // plane z + 1
+                    const float np = Dv(q.od, q.orr);
+                    const float nhx = Dv(q.hxd, q.hxr), nhy = Dv(q.hyd, q.hyr);
+                    const unsigned rc_next = q.rc;
+                    k1_load(q, c + (1 + K1_PF) * sz, last, S, dsrc, r, rcode);
+                    if (in_dom) {
+                        float t = dc;
+                        if (rc_cur & RC_CG_ROW)
+                            t = apply_row(rc_cur, dc, sh[cur][ty + 1][tx], sh[cur][ty + 1][tx + 2], sh[cur][ty][tx + 1],
+                                          sh[cur][ty + 2][tx + 1], np, dm);
+                        if (!first) dnew[c] = dc;
+                        v += (double)dc * (double)t;
+                    }
+                    sh[cur ^ 1][ty + 1][tx + 1] = np;
+                    if (S.hx_slot >= 0) sh[cur ^ 1][ty + 1][S.hx_slot] = nhx;
+                    if (S.hy_slot >= 0) sh[cur ^ 1][S.hy_slot][tx + 1] = nhy;
+                    lds_barrier();
+                    cur ^= 1;
+                    dm = dc;
+                    dc = np;
+                    rc_cur = rc_next;
+                    z++;
+                    c += sz;
+                }
             }
-            sh[cur ^ 1][ty + 1][tx + 1] = np;
-            if (S.hx_slot >= 0) sh[cur ^ 1][ty + 1][S.hx_slot] = nhx;
-            if (S.hy_slot >= 0) sh[cur ^ 1][S.hy_slot][tx + 1] = nhy;
-            __syncthreads();
-            cur ^= 1;
-            dm = dc;
-            dc = np;
         }
     }
     write_partial(v, p_dad, red);
     if (blockIdx.x == 0 && tx == 0 && ty == 0) st->beta = beta;
+}
+
+struct K2Plane {
+    uint16_t rc;
+    float o, hx, hy;  // d_i of the own cell and of the halo cells
+    float xd, r;      // increment and residual of the own cell
+};
+
+__device__ inline void k2_load(K2Plane& q, int c, int last, const Stager<CG_TY>& S, const float* __restrict__ dd,
+                               const float* xd, const float* r, const uint16_t* __restrict__ rcode) {
+    const int co = clampi(c, last);
+    q.rc = rcode[co];
+    q.xd = xd[co];
+    q.r = r[co];
+    q.o = dd[co];
+    q.hx = dd[clampi(c + S.hx_off, last)];
+    q.hy = dd[clampi(c + S.hy_off, last)];
 }
 
 // K2 (iteration it): alpha = gamma / d.t, xd += alpha d, r -= alpha A d, partial r.r
@@ -229,39 +297,56 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(Dom d, int it, const uint16
         Stager<CG_TY> S;
         S.init(d, x, y);
         const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
-        const int sz = d.W * d.H;
+        const int sz = d.W * d.H, last = sz * d.D - 1;
         int c = (zb * d.H + y) * d.W + x;
-        float dm = (zb > 0 && S.own_ok) ? dd[c - sz] : 0.f;
-        float dc = S.own_ok ? dd[c] : 0.f;
+        float dm = dd[clampi(c - sz, last)];
+        K2Plane q0;
+        k2_load(q0, c, last, S, dd, xd, r, rcode);
+        K2Plane ring[K2_PF];
+#pragma unroll
+        for (int k = 0; k < K2_PF; k++) k2_load(ring[k], c + (k + 1) * sz, last, S, dd, xd, r, rcode);
+        float dc = q0.o;
+        unsigned rc_cur = q0.rc;
+        float xd_cur = q0.xd, r_cur = q0.r;
         sh[0][ty + 1][tx + 1] = dc;
-        if (S.hx_slot >= 0) sh[0][ty + 1][S.hx_slot] = S.hx_ok ? dd[c + S.hx_off] : 0.f;
-        if (S.hy_slot >= 0) sh[0][S.hy_slot][tx + 1] = S.hy_ok ? dd[c + S.hy_off] : 0.f;
-        __syncthreads();
+        if (S.hx_slot >= 0) sh[0][ty + 1][S.hx_slot] = q0.hx;
+        if (S.hy_slot >= 0) sh[0][S.hy_slot][tx + 1] = q0.hy;
+        lds_barrier();
         int cur = 0;
-        for (int z = zb; z < ze; z++, c += sz) {
-            const bool has_next = z + 1 < d.D;
-            const float np = (has_next && S.own_ok) ? dd[c + sz] : 0.f;
-            const float nhx = (has_next && S.hx_ok) ? dd[c + sz + S.hx_off] : 0.f;
-            const float nhy = (has_next && S.hy_ok) ? dd[c + sz + S.hy_off] : 0.f;
-            if (in_dom) {
-                const unsigned rc = rcode[c];
-                const float xdo = xd[c], ro = r[c];
-                float t = dc;
-                if (rc & RC_CG_ROW)
-                    t = apply_row(rc, dc, sh[cur][ty + 1][tx], sh[cur][ty + 1][tx + 2], sh[cur][ty][tx + 1],
-                                  sh[cur][ty + 2][tx + 1], np, dm);
-                xd[c] = (float)((double)xdo + alpha * (double)dc);
-                const float rn = (float)((double)ro - alpha * (double)t);
-                r[c] = rn;
-                v += (double)rn * (double)rn;
+        int z = zb;
+        while (z < ze) {
+#pragma unroll
+            for (int k = 0; k < K2_PF; k++) {
+                if (z < ze) {
+                    K2Plane& q = ring[k];  // plane z + 1
+                    const float np = q.o, nhx = q.hx, nhy = q.hy;
+                    const unsigned rc_next = q.rc;
+                    const float xd_next = q.xd, r_next = q.r;
+                    k2_load(q, c + (1 + K2_PF) * sz, last, S, dd, xd, r, rcode);
+                    if (in_dom) {
+                        float t = dc;
+                        if (rc_cur & RC_CG_ROW)
+                            t = apply_row(rc_cur, dc, sh[cur][ty + 1][tx], sh[cur][ty + 1][tx + 2], sh[cur][ty][tx + 1],
+                                          sh[cur][ty + 2][tx + 1], np, dm);
+                        xd[c] = (float)((double)xd_cur + alpha * (double)dc);
+                        const float rn = (float)((double)r_cur - alpha * (double)t);
+                        r[c] = rn;
+                        v += (double)rn * (double)rn;
+                    }
+                    sh[cur ^ 1][ty + 1][tx + 1] = np;
+                    if (S.hx_slot >= 0) sh[cur ^ 1][ty + 1][S.hx_slot] = nhx;
+                    if (S.hy_slot >= 0) sh[cur ^ 1][S.hy_slot][tx + 1] = nhy;
+                    lds_barrier();
+                    cur ^= 1;
+                    dm = dc;
+                    dc = np;
+                    rc_cur = rc_next;
+                    xd_cur = xd_next;
+                    r_cur = r_next;
+                    z++;
+                    c += sz;
+                }
             }
-            sh[cur ^ 1][ty + 1][tx + 1] = np;
-            if (S.hx_slot >= 0) sh[cur ^ 1][ty + 1][S.hx_slot] = nhx;
-            if (S.hy_slot >= 0) sh[cur ^ 1][S.hy_slot][tx + 1] = nhy;
-            __syncthreads();
-            cur ^= 1;
-            dm = dc;
-            dc = np;
         }
     }
     write_partial(v, p_rr, red);

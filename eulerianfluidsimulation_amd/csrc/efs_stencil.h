@@ -152,6 +152,16 @@ struct Stager {
 
 constexpr int CG_TY = 8, CG_NT = TX * CG_TY;
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also fences global memory at
+// workgroup scope, which waits for every outstanding global load (s_waitcnt vmcnt(0)) and so
+// drains the plane prefetch at each plane; the stencil kernels share data between threads
+// through LDS only.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // sum of the n partials of the previous kernel, identical in every block
 __device__ inline double sum_partials(const double* __restrict__ p, int n, double* red) {
     const int lane = threadIdx.x + threadIdx.y * blockDim.x;

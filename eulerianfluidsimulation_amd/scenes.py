@@ -89,6 +89,20 @@ def dam2d(n=128, solver="cg"):
                  max_steps=400 if solver == "jacobi" else 2000)
 
 
+def pool2d(n=64, seed=0, nboxes=4, solver="cg"):
+    """2D counterpart of the sphere-over-pool scene: pool of depth 0.3, a fluid disc r = 0.15 at
+    height 0.7 and `nboxes` seeded solid boxes (side U(0.05, 0.2), centre U(0, 1)^2, stream 5)."""
+    prims = [_box((-BIG, -BIG, -BIG), (BIG, 0.3, BIG)), Prim(SPHERE, 0, (0.5, 0.7, 0.0), (0.15, 0.0, 0.0))]
+    for k in range(nboxes):
+        side = [uniform(seed, 5, 4 * k + a, 0.05, 0.2) for a in range(2)]
+        cen = [uniform01(seed, 5, 4 * k + 2 + a) for a in range(2)]
+        prims.append(_box([cen[0] - side[0] / 2, cen[1] - side[1] / 2, -BIG],
+                          [cen[0] + side[0] / 2, cen[1] + side[1] / 2, BIG], solid=1))
+    return Scene(name=f"pool2d_{n}", dims=(n + 2, n + 2, 0), prims=prims, dt=1.0 / 120.0, gravity_force=9.81 * n,
+                 steps=100, seed=seed, solver_type=3 if solver == "jacobi" else 2,
+                 max_steps=400 if solver == "jacobi" else 2000)
+
+
 def dam3d(n=128):
     """Config 2: 3D dam break, fluid x < 0.4 and (vertical) y < 0.5 over the full depth,
     dt = 1/60, CG tolerance 1e-5, at most 500 iterations, 200 steps."""

@@ -87,6 +87,8 @@ typedef struct {
     int32_t precond;          /* efs_precond                                               */
     int32_t true_residual;    /* 1 = FP64 true residual ||b - A p|| / ||b|| after each solve */
     int32_t initial_guess;    /* efs_initial_guess                                         */
+    int32_t re_calc_div;      /* RE_CALC_DIV: 2D only, divergence recomputed after the projection
+                                 (CPUSimulator2D.cpp:503-521)                              */
 } efs_params;
 
 typedef struct {
@@ -107,6 +109,12 @@ void efs_default_params(efs_params* p);
 /* Reference: CPUSimulator3D(w,h,d) / CudaSimulator3D ctor + Grid3D ctor (Grid3D.cpp:10-48):
  * allocates the grid, gamma = 1, border cells solid with rho 0. */
 int efs_create_3d(int32_t width, int32_t height, int32_t depth, const efs_params* params, efs_handle* out);
+/* Reference: CPUSimulator2D(w,h) + Grid2D ctor (Grid2D.cpp:11-60), the 2D backend with
+ * CPUSimulator2D's semantics (face-wise forces and pressure gradient, two-sided wall
+ * enforcement, 7 diffusion iterations).  Fields cross the boundary as `float` per cell and
+ * `float[2]` (glm::vec2) per cell for velocity and forces, cell (x,y) at y*W + x.  efs_dims
+ * reports depth 1.  The learned preconditioner is 3D only. */
+int efs_create_2d(int32_t width, int32_t height, const efs_params* params, efs_handle* out);
 /* Reference: CudaSimulator3D::~CudaSimulator3D (CudaSimulator3D.cu:51-80). */
 int efs_destroy(efs_handle h);
 /* Reference: the PROPERTY_* reads each stage performs (Properties.h:16-32). */
@@ -115,7 +123,7 @@ int efs_get_params(efs_handle h, efs_params* params);
 int efs_dims(efs_handle h, int32_t dims[3]);
 
 /* Reference: CudaGrid3D::upload / download (CudaGrid3D.cu:247-269), one field at a time, in the
- * reference layout; bytes must equal the field size (N*4, or N*12 for VEL/FORCES). */
+ * reference layout; bytes must equal the field size (N*4, or N*12 for VEL/FORCES; N*8 in 2D). */
 int efs_upload(efs_handle h, int32_t field, const void* host, size_t bytes);
 int efs_download(efs_handle h, int32_t field, void* host, size_t bytes);
 /* Device-to-device variants for callers that keep fields in HBM (same layout). */

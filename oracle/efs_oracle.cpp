@@ -1145,16 +1145,39 @@ template <class T> struct Sim2D {
                 if (rho[c] + del_rho[c] < 0 || rho[c] < 1e-5) del_rho[c] = -rho[c];
                 else if (rho[c] > T(0.5f)) del_rho[c] = 0;
             }
-        for (int y = 1; y < H - 1; y++)
-            for (int x = 1; x < W - 1; x++) {
-                const size_t c = id(x, y);
-                const T dr = del_rho[c];
-                if (std::abs(dr) < T(1e-6f)) continue;
-                rho[c] += dr;
-                rho[c] = std::max(rho[c], T(0.f));
-                const V2<T> p = trace_along_field(V2<T>((T)x, (T)y), (T)P.rho_d, 10);
-                scatter_values(p, -dr);
-            }
+        if (P.sharpen_mode == 0) {
+            for (int y = 1; y < H - 1; y++)
+                for (int x = 1; x < W - 1; x++) {
+                    const size_t c = id(x, y);
+                    const T dr = del_rho[c];
+                    if (std::abs(dr) < T(1e-6f)) continue;
+                    rho[c] += dr;
+                    rho[c] = std::max(rho[c], T(0.f));
+                    const V2<T> p = trace_along_field(V2<T>((T)x, (T)y), (T)P.rho_d, 10);
+                    scatter_values(p, -dr);
+                }
+        } else {
+            // apply-all-then-trace (see the 3D restatement): scatters into a double buffer
+            for (int y = 1; y < H - 1; y++)
+                for (int x = 1; x < W - 1; x++) {
+                    const size_t c = id(x, y);
+                    const T dr = del_rho[c];
+                    if (std::abs(dr) < T(1e-6f)) continue;
+                    rho[c] += dr;
+                    rho[c] = std::max(rho[c], T(0.f));
+                }
+            std::vector<double> acc(N, 0.0);
+            for (int y = 1; y < H - 1; y++)
+                for (int x = 1; x < W - 1; x++) {
+                    const size_t c = id(x, y);
+                    const T dr = del_rho[c];
+                    if (std::abs(dr) < T(1e-6f)) continue;
+                    const V2<T> p = trace_along_field(V2<T>((T)x, (T)y), (T)P.rho_d, 10);
+                    scatter_acc(acc, p, -dr);
+                }
+            for (size_t c = 0; c < N; c++)
+                if (acc[c] != 0.0) rho[c] = (T)((double)rho[c] + acc[c]);
+        }
     }
     V2<T> trace_along_field(V2<T> start, T Dm, int substeps) const {  // CPUSimulator2D.cpp:526-562
         const T stepSize = Dm / (T)substeps;
@@ -1173,6 +1196,24 @@ template <class T> struct Sim2D {
             pos = next;
         }
         return pos;
+    }
+    // scatter_values' weights, accumulated in double (the parallel restatement's order)
+    void scatter_acc(std::vector<double>& acc, V2<T> pos, T value) const {
+        const int x0 = clampv((int)pos.x, 0, W - 1), y0 = clampv((int)pos.y, 0, H - 1);
+        const int x1 = clampv((int)pos.x + 1, 0, W - 1), y1 = clampv((int)pos.y + 1, 0, H - 1);
+        const T fx = pos.x - (T)(int)pos.x, fy = pos.y - (T)(int)pos.y;
+        T w00 = (T(1.f) - fx) * (T(1.f) - fy), w10 = fx * (T(1.f) - fy), w01 = (T(1.f) - fx) * fy, w11 = fx * fy;
+        if (boundary[id(x0, y0)] > T(0.5f)) w00 = 0;
+        if (boundary[id(x1, y0)] > T(0.5f)) w10 = 0;
+        if (boundary[id(x0, y1)] > T(0.5f)) w01 = 0;
+        if (boundary[id(x1, y1)] > T(0.5f)) w11 = 0;
+        const T total = w00 + w10 + w01 + w11;
+        if (total < T(1e-6f)) return;
+        const T inv = T(1.f) / total;
+        acc[id(x0, y0)] += (double)(value * w00 * inv);
+        acc[id(x1, y0)] += (double)(value * w10 * inv);
+        acc[id(x0, y1)] += (double)(value * w01 * inv);
+        acc[id(x1, y1)] += (double)(value * w11 * inv);
     }
     void scatter_values(V2<T> pos, T value) {  // CPUSimulator2D.cpp:564-595
         const int x0 = clampv((int)pos.x, 0, W - 1), y0 = clampv((int)pos.y, 0, H - 1);

@@ -31,8 +31,8 @@ def test_binding_covers_header():
 
 
 def test_struct_layouts_match_header():
-    # efs_params: 8 floats, 3 int32, float, uint32, 4 int32 = 68 bytes; efs_stats padded to 8
-    assert ctypes.sizeof(_abi.Params) == 68
+    # efs_params: 8 floats, 3 int32, float, uint32, 5 int32 = 72 bytes; efs_stats padded to 8
+    assert ctypes.sizeof(_abi.Params) == 72
     assert ctypes.sizeof(_abi.Stats) == 4 + 4 + 8 * 4 + 4 + 4 + 28 + 4
     assert ctypes.sizeof(_abi.Prim) == 56
 
