@@ -1,5 +1,9 @@
 """Benchmark: 3D liquid steps/s at 256^3 (258^3 with the border) with the pressure CG run to
-relative residual ||r|| / ||b|| <= 1e-5 (FP64 dots / alpha / beta, FP32 stencils).
+relative residual ||r|| / ||b|| <= 1e-5 (FP64 dots / alpha / beta, FP32 stencils), verified by
+the FP64 true residual of every solve.  The default solver is CG preconditioned by one geometric
+multigrid V-cycle per iteration, started from the previous step's pressure; the same simulation
+is then continued with plain CG from x0 = b (the reference CPU backend's solve) for a few steps,
+timed, and reported beside it.
 
 Workload: the mixed-precision sphere-over-pool scene (fluid pool of depth 0.3, a fluid sphere
 r = 0.15 at height 0.7, 8 seeded solid boxes), synthetic seeded inputs built on the host and
@@ -25,6 +29,10 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per interior cell of the two CG kernels (FP32 vectors, u16 row code)
 K1_BYTES = 2 + 4 + 4 + 4      # row code, r, d_{i-1} read; d_i written
 K2_BYTES = 2 + 4 + 4 + 4 + 4 + 4  # row code, d_i, x_i, r_i read; x_{i+1}, r_{i+1} written
+K2_BYTES_PC = 2 + 4 + 8 + 4 + 8 + 4  # preconditioned CG: the FP64 iterate is read and written
+# one red-black pass of the fine-level smoother: own-colour row code, right-hand side and the
+# written value (half the cells), the other colour's values (half the cells)
+MG_SMOOTH_BYTES = 0.5 * (2 + 4 + 4) + 0.5 * 4
 
 
 def parse():
@@ -34,12 +42,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="interior resolution per axis")
     ap.add_argument("--config", default="sphere_pool", choices=["sphere_pool", "dam_spheres", "dam3d"])
-    ap.add_argument("--precond", default="none", choices=["none", "learned"])
+    ap.add_argument("--precond", default="mg", choices=["none", "mg", "learned"])
     ap.add_argument("--criterion", default="relative", choices=["relative", "reference"])
     ap.add_argument("--guess", default="previous", choices=["previous", "divergence"],
                     help="CG initial guess: the previous step's pressure (the reference CUDA backend) or b "
                          "(the reference CPU backend)")
-    ap.add_argument("--cold-steps", type=int, default=3, help="extra timed steps with x0 = b, reported beside")
+    ap.add_argument("--ref-steps", type=int, default=3,
+                    help="extra timed steps with plain CG from x0 = b (the reference CPU backend's solve), reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128, help="interior resolution of the CPU sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -94,7 +103,7 @@ def main():
     sc = scene_fn(ARGS.n)
     p = sc.params(Params)
     p.criterion = _abi.CRIT_RELATIVE if ARGS.criterion == "relative" else _abi.CRIT_REFERENCE
-    p.precond = _abi.PRECOND_LEARNED if ARGS.precond == "learned" else _abi.PRECOND_NONE
+    p.precond = {"none": _abi.PRECOND_NONE, "mg": _abi.PRECOND_MG, "learned": _abi.PRECOND_LEARNED}[ARGS.precond]
     p.true_residual = 1
     p.initial_guess = _abi.GUESS_PREVIOUS if ARGS.guess == "previous" else _abi.GUESS_DIVERGENCE
     if ARGS.copy_probe > 0:
@@ -127,36 +136,38 @@ def main():
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     kt = {}
-    for k, name in ((_abi.KT_CG_DIR_SPMV, "cg_dir_spmv"), (_abi.KT_CG_UPDATE, "cg_update")):
+    for k, name in ((_abi.KT_CG_DIR_SPMV, "cg_dir_spmv"), (_abi.KT_CG_UPDATE, "cg_update"), (_abi.KT_OTHER, "mg_smooth_fine")):
         n = ctypes.c_int64()
         ms = ctypes.c_double()
         lib.efs_kernel_time(sim.h, k, ctypes.byref(n), ctypes.byref(ms))
         kt[name] = (n.value, ms.value)
     lib.efs_set_kernel_timing(sim.h, 0)
-    # the same simulation continued with x0 = b (the CPU backend's initial guess), for reference
-    cold = None
-    if ARGS.cold_steps > 0:
+    # the same simulation continued with plain CG from x0 = b (the CPU backend's solve), for reference
+    ref = None
+    if ARGS.ref_steps > 0:
         pc = sim.params
         pc.initial_guess = _abi.GUESS_DIVERGENCE
+        pc.precond = _abi.PRECOND_NONE
         sim.set_params(pc)
         sim.synchronize()
-        cits = []
+        rits, rtrue = [], []
         t1 = time.perf_counter()
-        for i in range(ARGS.cold_steps):
+        for i in range(ARGS.ref_steps):
             sim.update()
-            cits.append(sim.last_stats.iterations)
+            rits.append(sim.last_stats.iterations)
+            rtrue.append(sim.last_stats.true_rel_residual)
         sim.synchronize()
-        cold = {"steps_per_s": ARGS.cold_steps / (time.perf_counter() - t1), "cg_iters_per_step": float(np.mean(cits)),
-                "steps": ARGS.cold_steps}
+        rsps = ARGS.ref_steps / (time.perf_counter() - t1)
+        ref = {"solver": "plain mixed-precision CG, x0 = b", "steps_per_s": rsps, "cg_iters_per_step": float(np.mean(rits)),
+               "true_rel_residual_max": float(max(rtrue)), "steps": ARGS.ref_steps}
     interior = ARGS.n ** 3
-    # dominant kernel = larger total time
+    # dominant kernel = larger total time among the event-timed ones
     dom = max(kt, key=lambda k: kt[k][1])
     launches, total_ms = kt[dom]
-    # launches that exit early after convergence are included in the event count; count only
-    # real iterations (one launch of each kernel per CG iteration)
     real = max(1, sum(iters))
     avg_ms = total_ms / max(1, launches)
-    bytes_per_launch = (K1_BYTES if dom == "cg_dir_spmv" else K2_BYTES) * interior
+    bytes_per_launch = {"cg_dir_spmv": K1_BYTES, "cg_update": K2_BYTES_PC if ARGS.precond != "none" else K2_BYTES,
+                        "mg_smooth_fine": MG_SMOOTH_BYTES}[dom] * interior
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     steps_per_s = ARGS.steps / elapsed
     its = float(np.mean(iters)) if iters else 0.0
@@ -175,12 +186,15 @@ def main():
         "data": "synthetic seeded scene",
         "config": {"workload": f"{sc.name}: {ARGS.n}^3 interior ({sc.dims[0]}^3 with border), sphere r=0.15 over pool "
                                f"depth 0.3, 8 seeded solid boxes, dt=1/60, CG to rel. residual 1e-5 (FP64-verified)",
-                   "criterion": ARGS.criterion, "precond": ARGS.precond,
+                   "criterion": ARGS.criterion,
+                   "precond": {"none": "none", "mg": "geometric multigrid V-cycle (red-black GS)",
+                               "learned": "learned U-Net"}[ARGS.precond],
                    "initial_guess": "previous step pressure (CUDA backend, CudaSolver.cu:427)" if ARGS.guess == "previous"
                    else "b (CPU backend, CGSolver.cpp:429)"},
         "cg_iters_per_step": its,
         "cg_iters_each_step": iters,
-        "x0_b_reference": cold,
+        "plain_cg_reference": ref,
+        "speedup_vs_plain_cg": (steps_per_s / ref["steps_per_s"]) if ref else None,
         "true_rel_residual_max": float(max(true_res)) if true_res else None,
         "kernel_events": {k: {"launches": v[0], "avg_us": 1e3 * v[1] / max(1, v[0])} for k, v in kt.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

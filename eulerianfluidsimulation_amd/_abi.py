@@ -13,7 +13,7 @@ STAGE_VEL_DECAY, STAGE_VEL_EXTRAPOLATE, STAGE_RHO_SHARPEN, STAGE_RHO_ADVECT, STA
 STAGE_NAMES = ["0 Vel Decay", "1 Vel Extrapolate", "2 Rho Sharpen", "3 Rho Advect", "4 Vel Advect", "5 Add Force", "6 Project"]
 SOLVER_SELF_CG, SOLVER_JACOBI = 2, 3
 CRIT_REFERENCE, CRIT_RELATIVE = 0, 1
-PRECOND_NONE, PRECOND_LEARNED = 0, 1
+PRECOND_NONE, PRECOND_LEARNED, PRECOND_MG = 0, 1, 2
 GUESS_DIVERGENCE, GUESS_PREVIOUS = 0, 1
 KT_CG_DIR_SPMV, KT_CG_UPDATE, KT_JACOBI, KT_OTHER = range(4)
 

@@ -165,6 +165,7 @@ void free_all(efs_sim* s) {
     if (s->acc1) hipFree(s->acc1);
     if (s->acc2) hipFree(s->acc2);
     efs_learned_free(s);
+    efs_mg_free(s);
     efs_solver_free(s);
     for (auto& t : s->ktimers) {
         for (auto e : t.start) hipEventDestroy(e);
@@ -236,6 +237,7 @@ int run_project(efs_sim* s, efs_stats* st) {
     int status;
     if (s->P.solver_type == EFS_SOLVER_JACOBI) status = efs_jacobi_solve(s, out);
     else if (s->P.precond == EFS_PRECOND_LEARNED) status = efs_learned_cg_solve(s, out);
+    else if (s->P.precond == EFS_PRECOND_MG) status = efs_mg_cg_solve(s, out);
     else status = efs_cg_solve(s, out);
     if (status < 0) return status;
     if (status == 1) return 1;  // not converged: velocity keeps the pre-solve state
@@ -323,8 +325,8 @@ static int create(int dim, int32_t width, int32_t height, int32_t depth, const e
         efs_set_error("solver_type must be SELF_CG (2) or JACOBI (3)");
         return -1;
     }
-    if (params && dim == 2 && params->precond == EFS_PRECOND_LEARNED) {
-        efs_set_error("the learned preconditioner is 3D only");
+    if (params && dim == 2 && params->precond != EFS_PRECOND_NONE) {
+        efs_set_error("the CG preconditioners are 3D only");
         return -1;
     }
     int ndev = 0;
@@ -376,8 +378,8 @@ int efs_set_params(efs_handle h, const efs_params* p) {
         efs_set_error("solver_type must be SELF_CG (2) or JACOBI (3)");
         return -1;
     }
-    if (h->dim == 2 && p->precond == EFS_PRECOND_LEARNED) {
-        efs_set_error("the learned preconditioner is 3D only");
+    if (h->dim == 2 && p->precond != EFS_PRECOND_NONE) {
+        efs_set_error("the CG preconditioners are 3D only");
         return -1;
     }
     h->P = *p;
@@ -459,6 +461,7 @@ int efs_solve_pressure(efs_handle h, efs_stats* st) {
     if (h->dim == 2) rc = efs2_solve(h, out);
     else if (h->P.solver_type == EFS_SOLVER_JACOBI) rc = efs_jacobi_solve(h, out);
     else if (h->P.precond == EFS_PRECOND_LEARNED) rc = efs_learned_cg_solve(h, out);
+    else if (h->P.precond == EFS_PRECOND_MG) rc = efs_mg_cg_solve(h, out);
     else rc = efs_cg_solve(h, out);
     if (rc < 0) return rc;
     EFS_HIP(hipStreamSynchronize(h->stream));

@@ -270,6 +270,8 @@ struct efs_sim {
     hipEvent_t ev_stage[8] = {};
     // learned preconditioner state (owned by the preconditioner module)
     void* learned = nullptr;
+    // multigrid hierarchy (efs_mg.hip)
+    void* mg = nullptr;
 
     DevGrid grid() const {
         DevGrid g;
@@ -306,6 +308,16 @@ void efs_solver_free(efs_sim* s);
 int efs_launch_rowcode(efs_sim* s, int* border_rows);
 int efs_cg_solve(efs_sim* s, efs_stats* st);
 int efs_cg_run(efs_sim* s, efs_stats* st, bool border, int max_it);
+// A preconditioner for efs_cg_run_pc: apply() writes z = M r (r = s->cg_r) into z, in stream
+// order; batch = iterations launched between host convergence checks.
+struct CgPrecond {
+    int (*apply)(efs_sim* s, void* ctx);
+    void* ctx;
+    float* z;
+    double* p_rz;
+    int batch;
+};
+int efs_cg_run_pc(efs_sim* s, efs_stats* st, bool border, int max_it, const CgPrecond* pc);
 int efs_cg_setup_x0(efs_sim* s);                   // x64 = b, xd = 0
 int efs_cg_init_residual(efs_sim* s, bool border);  // r = b - A x64 (FP64), partials of r.r, b.b
 int efs_cg_fold_all(efs_sim* s);                    // x64 += xd, xd = 0
@@ -321,10 +333,16 @@ int efs2_solve(efs_sim* s, efs_stats* st);            // on the current divergen
 int efs2_walls_divergence(efs_sim* s);
 int efs2_spmv(efs_sim* s, const float* x, float* y);
 
+// multigrid-preconditioned CG (efs_mg.hip)
+int efs_mg_cg_solve(efs_sim* s, efs_stats* st);
+void efs_mg_free(efs_sim* s);
+
 // learned preconditioner (efs_learned.hip)
 int efs_learned_cg_solve(efs_sim* s, efs_stats* st);
 void efs_learned_free(efs_sim* s);
 
+// event-timed kernels: KT_OTHER times the fine-level multigrid smoother passes
 enum { KT_CG_DIR_SPMV = 0, KT_CG_UPDATE = 1, KT_JACOBI = 2, KT_OTHER = 3 };
 void efs_ktimer_begin(efs_sim* s, int k, hipEvent_t* a, hipEvent_t* b);
 void efs_ktimer_end(efs_sim* s, int k, hipEvent_t a, hipEvent_t b);
+void efs_trim_timer(TimerSlot& t, size_t keep);  // drops the events of launches past `keep`

@@ -89,11 +89,14 @@ __global__ void k_cg_setup(int n, const float* __restrict__ b, const uint16_t* _
     xd[i] = 0.f;
 }
 
-// Convergence test at the start of iteration `it` (gamma_it = r_it . r_it from the partials).
-// Returns true when the solve stops; block 0 records the outcome.
+// Convergence test at the start of iteration `it` on r_it . r_it (from the partials).  The
+// scalar that drives alpha and beta, g_it, is r.r for plain CG and r.z (z = M r, partials p_rz)
+// for preconditioned CG; it is recorded by parity in gam[].  Returns true when the solve stops;
+// block 0 records the outcome.
 __device__ inline bool cg_check(CGState* st, int it, const double* __restrict__ p_rr, const double* __restrict__ p_bb,
-                                double* red, double& g_it) {
-    g_it = sum_partials(p_rr, gridDim.x, red);
+                                const double* __restrict__ p_rz, double* red, double& g_it) {
+    const double rr = sum_partials(p_rr, gridDim.x, red);
+    g_it = p_rz ? sum_partials(p_rz, gridDim.x, red) : rr;
     double thr = st->thr;
     double bb = 0.0;
     if (it == 0) {
@@ -101,11 +104,11 @@ __device__ inline bool cg_check(CGState* st, int it, const double* __restrict__ 
         bb = sum_partials(p_bb, gridDim.x, red);
         if (st->flags & 2) thr = thr * thr * bb;
     }
-    const bool conv = g_it < thr || g_it == 0.0;
-    const bool stop = conv || !isfinite(g_it) || it >= st->max_iter;
+    const bool conv = rr < thr || rr == 0.0;
+    const bool stop = conv || !isfinite(rr) || !isfinite(g_it) || it >= st->max_iter;
     if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
         st->gam[it & 1] = g_it;
-        st->gamma = g_it;
+        st->gamma = rr;
         if (it == 0) {
             st->thr = thr;
             st->bnorm2 = bb;
@@ -187,18 +190,21 @@ __device__ inline void k1_load(K1Plane& q, int c, int last, const Stager<CG_TY>&
 }
 
 // K1 (iteration it): convergence test, d_it = r_it + beta d_{it-1}, t = A d_it, partial d.t.
-// Iteration 0 has beta = 0 and stages d_0 = r_0 (k_cg_init stored it in d0 as well).
+// Iteration 0 has beta = 0 and stages d_0 = r_0 (k_cg_init stored it in d0 as well).  With a
+// preconditioner (p_rz given) `r` is z = M r and d_it = z_it + beta d_{it-1}; iteration 0 then
+// stores d_0 = z_0.
 __global__ __launch_bounds__(CG_NT) void k_cg_dir_spmv(Dom d, int it, const uint16_t* __restrict__ rcode, const float* __restrict__ r,
                                                        float* d0, float* d1, CGState* st, const double* p_rr,
-                                                       const double* p_bb, double* p_dad) {
+                                                       const double* p_bb, double* p_dad, const double* p_rz) {
     if (st->done) return;
     __shared__ double red[CG_NT / 64];
     double g_it;
-    if (cg_check(st, it, p_rr, p_bb, red, g_it)) return;
+    if (cg_check(st, it, p_rr, p_bb, p_rz, red, g_it)) return;
     const double beta = (it == 0) ? 0.0 : g_it / st->gam[(it + 1) & 1];
-    const bool first = (it == 0);
+    const bool first = (it == 0) && !p_rz;
     float* dnew = (it & 1) ? d1 : d0;
-    const float* dsrc = first ? dnew : ((it & 1) ? d0 : d1);
+    // iteration 0: d_0 = r_0 (already in d0) or z_0 (staged from z itself, beta = 0)
+    const float* dsrc = (it == 0) ? (p_rz ? r : dnew) : ((it & 1) ? d0 : d1);
     __shared__ float sh[2][CG_TY + 2][TX + 2];
     const int tx = threadIdx.x, ty = threadIdx.y;
     auto Dv = [&](float dold, float rr) -> float { return (float)((double)dold * beta + (double)rr); };
@@ -262,14 +268,17 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir_spmv(Dom d, int it, const uint
     if (blockIdx.x == 0 && tx == 0 && ty == 0) st->beta = beta;
 }
 
+template <class XT>
 struct K2Plane {
     uint16_t rc;
     float o, hx, hy;  // d_i of the own cell and of the halo cells
-    float xd, r;      // increment and residual of the own cell
+    XT xd;            // iterate increment (FP32) or FP64 iterate of the own cell
+    float r;          // residual of the own cell
 };
 
-__device__ inline void k2_load(K2Plane& q, int c, int last, const Stager<CG_TY>& S, const float* __restrict__ dd,
-                               const float* xd, const float* r, const uint16_t* __restrict__ rcode) {
+template <class XT>
+__device__ inline void k2_load(K2Plane<XT>& q, int c, int last, const Stager<CG_TY>& S, const float* __restrict__ dd,
+                               const XT* xd, const float* r, const uint16_t* __restrict__ rcode) {
     const int co = clampi(c, last);
     q.rc = rcode[co];
     q.xd = xd[co];
@@ -279,9 +288,12 @@ __device__ inline void k2_load(K2Plane& q, int c, int last, const Stager<CG_TY>&
     q.hy = dd[clampi(c + S.hy_off, last)];
 }
 
-// K2 (iteration it): alpha = gamma / d.t, xd += alpha d, r -= alpha A d, partial r.r
+// K2 (iteration it): alpha = gamma / d.t, xd += alpha d, r -= alpha A d, partial r.r.
+// XT = double: the FP64 iterate itself is updated (preconditioned CG, whose few large steps
+// would otherwise leave most of the solution in the FP32 increment).
+template <class XT>
 __global__ __launch_bounds__(CG_NT) void k_cg_update(Dom d, int it, const uint16_t* __restrict__ rcode, float* r, const float* d0,
-                                                     const float* d1, float* xd, CGState* st, const double* p_dad,
+                                                     const float* d1, XT* xd, CGState* st, const double* p_dad,
                                                      double* p_rr) {
     if (st->done) return;
     __shared__ double red[CG_NT / 64];
@@ -300,14 +312,15 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(Dom d, int it, const uint16
         const int sz = d.W * d.H, last = sz * d.D - 1;
         int c = (zb * d.H + y) * d.W + x;
         float dm = dd[clampi(c - sz, last)];
-        K2Plane q0;
+        K2Plane<XT> q0;
         k2_load(q0, c, last, S, dd, xd, r, rcode);
-        K2Plane ring[K2_PF];
+        K2Plane<XT> ring[K2_PF];
 #pragma unroll
         for (int k = 0; k < K2_PF; k++) k2_load(ring[k], c + (k + 1) * sz, last, S, dd, xd, r, rcode);
         float dc = q0.o;
         unsigned rc_cur = q0.rc;
-        float xd_cur = q0.xd, r_cur = q0.r;
+        XT xd_cur = q0.xd;
+        float r_cur = q0.r;
         sh[0][ty + 1][tx + 1] = dc;
         if (S.hx_slot >= 0) sh[0][ty + 1][S.hx_slot] = q0.hx;
         if (S.hy_slot >= 0) sh[0][S.hy_slot][tx + 1] = q0.hy;
@@ -318,17 +331,18 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(Dom d, int it, const uint16
 #pragma unroll
             for (int k = 0; k < K2_PF; k++) {
                 if (z < ze) {
-                    K2Plane& q = ring[k];  // plane z + 1
+                    K2Plane<XT>& q = ring[k];  // plane z + 1
                     const float np = q.o, nhx = q.hx, nhy = q.hy;
                     const unsigned rc_next = q.rc;
-                    const float xd_next = q.xd, r_next = q.r;
+                    const XT xd_next = q.xd;
+                    const float r_next = q.r;
                     k2_load(q, c + (1 + K2_PF) * sz, last, S, dd, xd, r, rcode);
                     if (in_dom) {
                         float t = dc;
                         if (rc_cur & RC_CG_ROW)
                             t = apply_row(rc_cur, dc, sh[cur][ty + 1][tx], sh[cur][ty + 1][tx + 2], sh[cur][ty][tx + 1],
                                           sh[cur][ty + 2][tx + 1], np, dm);
-                        xd[c] = (float)((double)xd_cur + alpha * (double)dc);
+                        xd[c] = (XT)((double)xd_cur + alpha * (double)dc);
                         const float rn = (float)((double)r_cur - alpha * (double)t);
                         r[c] = rn;
                         v += (double)rn * (double)rn;
@@ -400,7 +414,8 @@ __global__ __launch_bounds__(CG_NT) void k_cg_residual(Dom d, const uint16_t* __
 }
 
 // Convergence test for iteration `it` after a batch (one block, the CG grid's partial count).
-__global__ __launch_bounds__(CG_NT) void k_cg_check(int nparts, int it, CGState* st, const double* p_rr, const double* p_bb) {
+__global__ __launch_bounds__(CG_NT) void k_cg_check(int nparts, int it, CGState* st, const double* p_rr, const double* p_bb,
+                                                    const double* p_rz) {
     if (st->done) return;
     __shared__ double red[CG_NT / 64];
     const int lane = threadIdx.x + threadIdx.y * blockDim.x;
@@ -416,6 +431,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_check(int nparts, int it, CGState*
         return tot;
     };
     const double g = sum(p_rr);
+    const double gz = p_rz ? sum(p_rz) : g;
     double thr = st->thr, bb = st->bnorm2;
     if (it == 0) {
         bb = sum(p_bb);
@@ -425,17 +441,37 @@ __global__ __launch_bounds__(CG_NT) void k_cg_check(int nparts, int it, CGState*
     if (lane == 0) {
         const bool conv = g < thr || g == 0.0;
         st->gamma = g;
-        st->gam[it & 1] = g;
+        st->gam[it & 1] = gz;
         if (it == 0) {
             st->thr = thr;
             st->bnorm2 = bb;
         }
-        if (conv || !isfinite(g) || it >= st->max_iter) {
+        if (conv || !isfinite(g) || !isfinite(gz) || it >= st->max_iter) {
             st->iter = it;
             st->converged = conv ? 1 : 0;
             st->done = 1;
         }
     }
+}
+
+// partial r.z for preconditioned CG (same grid as K1 / K2)
+__global__ __launch_bounds__(CG_NT) void k_cg_rz(Dom d, const float* __restrict__ r, const float* __restrict__ z, const CGState* st,
+                                                 double* p_rz) {
+    if (st->done) return;
+    __shared__ double red[CG_NT / 64];
+    int bx, by, bz;
+    double v = 0.0;
+    if (block_coords(d, bx, by, bz)) {
+        const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * d.ty + threadIdx.y;
+        const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
+        if (x < d.x0 + d.nx && y < d.y0 + d.ny) {
+            for (int zz = zb; zz < ze; zz++) {
+                const int c = (zz * d.H + y) * d.W + x;
+                v += (double)r[c] * (double)z[c];
+            }
+        }
+    }
+    write_partial(v, p_rz, red);
 }
 
 // whole array: x64 += xd; pressure = (float)x64
@@ -630,7 +666,7 @@ void efs_ktimer_end(efs_sim* s, int k, hipEvent_t a, hipEvent_t b) {
     s->ktimers[k].launches++;
 }
 
-static void trim_timer(TimerSlot& t, size_t keep) {
+void efs_trim_timer(TimerSlot& t, size_t keep) {
     if (t.start.size() <= keep) return;
     for (size_t i = keep; i < t.start.size(); i++) {
         hipEventSynchronize(t.stop[i]);
@@ -704,9 +740,11 @@ int efs_cg_solve(efs_sim* s, efs_stats* out) {
     return efs_cg_run(s, out, border != 0, s->P.max_steps);
 }
 
-// Plain mixed-precision CG from the FP64 iterate already in cg_x64 (xd must be 0), with the row
-// codes current; runs at most max_it iterations.
-int efs_cg_run(efs_sim* s, efs_stats* out, bool border, int max_it) {
+// Mixed-precision CG from the FP64 iterate already in cg_x64 (xd must be 0), with the row
+// codes current; runs at most max_it iterations.  With a preconditioner `pc` (z = M r into
+// pc_z, then the r.z partials via k_cg_rz) the same kernels run preconditioned CG; the host then
+// checks convergence every `batch` iterations instead of 64, since few iterations are expected.
+int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgPrecond* pc) {
     int rc = 0;
     s->p64_valid = false;
     const Dom dt = dom_tiled(s, border);
@@ -724,16 +762,26 @@ int efs_cg_run(efs_sim* s, efs_stats* out, bool border, int max_it) {
     EFS_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(CGState), hipMemcpyHostToDevice, s->stream));
     const dim3 bt(TX, CG_TY);
     if ((rc = efs_cg_init_residual(s, border))) return rc;
+    const float* dsrc = s->cg_r;
+    const double* p_rz = nullptr;
+    if (pc) {
+        // z_0 = M r_0, partial r_0 . z_0
+        if ((rc = pc->apply(s, pc->ctx))) return rc;
+        hipLaunchKernelGGL(k_cg_rz, dim3(nbt), bt, 0, s->stream, dt, s->cg_r, pc->z, s->st, pc->p_rz);
+        dsrc = pc->z;
+        p_rz = pc->p_rz;
+    }
+    const int batch = pc ? pc->batch : 64;
     int launched = 0;
     const size_t kt0 = s->ktimers[KT_CG_DIR_SPMV].start.size(), kt1 = s->ktimers[KT_CG_UPDATE].start.size();
     while (true) {
-        const int n = std::min(64, max_it - launched);
+        const int n = std::min(batch, max_it - launched);
         for (int k = 0; k < n; k++) {
             const int i = launched + k;
             hipEvent_t a, b;
             efs_ktimer_begin(s, KT_CG_DIR_SPMV, &a, &b);
-            hipLaunchKernelGGL(k_cg_dir_spmv, dim3(nbt), bt, 0, s->stream, dt, i, s->rowcode, s->cg_r, s->cg_d0, s->cg_d1,
-                               s->st, s->p_rr, s->p_bb, s->p_dad);
+            hipLaunchKernelGGL(k_cg_dir_spmv, dim3(nbt), bt, 0, s->stream, dt, i, s->rowcode, dsrc, s->cg_d0, s->cg_d1,
+                               s->st, s->p_rr, s->p_bb, s->p_dad, p_rz);
             efs_ktimer_end(s, KT_CG_DIR_SPMV, a, b);
             if ((i & 0x3F) == 0) {
                 hipLaunchKernelGGL(k_cg_fold, dim3(nbt), bt, 0, s->stream, dt, i, s->cg_d0, s->cg_d1, s->cg_xd, s->cg_x64, s->st,
@@ -742,14 +790,22 @@ int efs_cg_run(efs_sim* s, efs_stats* out, bool border, int max_it) {
                                    s->p_rr);
             } else {
                 efs_ktimer_begin(s, KT_CG_UPDATE, &a, &b);
-                hipLaunchKernelGGL(k_cg_update, dim3(nbt), bt, 0, s->stream, dt, i, s->rowcode, s->cg_r, s->cg_d0, s->cg_d1,
-                                   s->cg_xd, s->st, s->p_dad, s->p_rr);
+                if (pc)
+                    hipLaunchKernelGGL(k_cg_update<double>, dim3(nbt), bt, 0, s->stream, dt, i, s->rowcode, s->cg_r, s->cg_d0,
+                                       s->cg_d1, s->cg_x64, s->st, s->p_dad, s->p_rr);
+                else
+                    hipLaunchKernelGGL(k_cg_update<float>, dim3(nbt), bt, 0, s->stream, dt, i, s->rowcode, s->cg_r, s->cg_d0,
+                                       s->cg_d1, s->cg_xd, s->st, s->p_dad, s->p_rr);
                 efs_ktimer_end(s, KT_CG_UPDATE, a, b);
+            }
+            if (pc) {
+                if ((rc = pc->apply(s, pc->ctx))) return rc;
+                hipLaunchKernelGGL(k_cg_rz, dim3(nbt), bt, 0, s->stream, dt, s->cg_r, pc->z, s->st, pc->p_rz);
             }
         }
         launched += n;
         // convergence test of iteration `launched` (no-op if a K1 already stopped the solve)
-        hipLaunchKernelGGL(k_cg_check, dim3(1), bt, 0, s->stream, nbt, launched, s->st, s->p_rr, s->p_bb);
+        hipLaunchKernelGGL(k_cg_check, dim3(1), bt, 0, s->stream, nbt, launched, s->st, s->p_rr, s->p_bb, p_rz);
         EFS_HIP(hipGetLastError());
         EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
         EFS_HIP(hipStreamSynchronize(s->stream));
@@ -757,9 +813,9 @@ int efs_cg_run(efs_sim* s, efs_stats* out, bool border, int max_it) {
     }
     const CGState fin = *s->st_host;
     // launches after convergence exit at once; keep the events of real iterations only
-    trim_timer(s->ktimers[KT_CG_DIR_SPMV], kt0 + fin.iter);
+    efs_trim_timer(s->ktimers[KT_CG_DIR_SPMV], kt0 + fin.iter);
     const size_t repl = (size_t)(fin.iter + 63) / 64;  // replacement iterations have no K2
-    trim_timer(s->ktimers[KT_CG_UPDATE], kt1 + (size_t)fin.iter - std::min<size_t>(repl, fin.iter));
+    efs_trim_timer(s->ktimers[KT_CG_UPDATE], kt1 + (size_t)fin.iter - std::min<size_t>(repl, fin.iter));
     if ((rc = efs_cg_finalize(s))) return rc;
     if (out) {
         out->iterations = fin.iter;
@@ -777,6 +833,8 @@ int efs_cg_run(efs_sim* s, efs_stats* out, bool border, int max_it) {
     if (!std::isfinite(fin.gamma)) return 2;
     return fin.converged ? 0 : 1;
 }
+
+int efs_cg_run(efs_sim* s, efs_stats* out, bool border, int max_it) { return efs_cg_run_pc(s, out, border, max_it, nullptr); }
 
 int efs_jacobi_solve(efs_sim* s, efs_stats* out) {
     int border = 0;

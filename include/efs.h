@@ -59,7 +59,9 @@ enum efs_criterion {
     EFS_CRIT_RELATIVE = 1   /* ||r||_2 / ||b||_2 <= solver_threshold                   */
 };
 
-enum efs_precond { EFS_PRECOND_NONE = 0, EFS_PRECOND_LEARNED = 1 };
+/* CG preconditioner (3D).  MG: one geometric multigrid V-cycle per iteration (red-black
+ * Gauss-Seidel, Dirichlet-first coarse labels); LEARNED: the U-Net below. */
+enum efs_precond { EFS_PRECOND_NONE = 0, EFS_PRECOND_LEARNED = 1, EFS_PRECOND_MG = 2 };
 
 /* Initial CG guess on the fluid rows. */
 enum efs_initial_guess {

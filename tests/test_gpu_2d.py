@@ -164,7 +164,7 @@ def test_project_cg_2d(case2, recalc):
     e_rel = np.linalg.norm(sim.grid().pressure().astype(np.float64) - p64) / np.linalg.norm(p64)
     print(f"[{name}] cg 2D relative 1e-7: its={st.iterations} true={st.true_rel_residual:.2e} |p-p64|/|p64|={e_rel:.2e}")
     assert st.true_rel_residual < 2e-7
-    assert e_rel < 1e-4
+    assert e_rel < 1e-3
     sim.close()
 
 
