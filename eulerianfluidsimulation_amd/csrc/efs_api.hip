@@ -99,6 +99,7 @@ int ensure_forces(efs_sim* s) {
 // Moves `bytes` of field f between the library and a caller buffer of the given memcpy kind.
 int transfer(efs_sim* s, int f, void* other, size_t bytes, bool upload, hipMemcpyKind kind) {
     if (!valid_handle(s)) return -1;
+    if (upload) s->traces_valid = false;
     if (f < 0 || f > EFS_FIELD_BOUNDARY) {
         efs_set_error("unknown field id");
         return -1;
@@ -247,6 +248,9 @@ int run_project(efs_sim* s, efs_stats* st) {
 
 int run_stage(efs_sim* s, int stage, efs_stats* st) {
     if (s->dim == 2) return efs2_run_stage(s, stage, st);
+    // every stage but the velocity advection that consumes them invalidates the stored traces
+    // (rho_advect stores fresh ones)
+    if (stage != EFS_STAGE_VEL_ADVECT) s->traces_valid = false;
     switch (stage) {
         case EFS_STAGE_VEL_DECAY: return efs_launch_decay(s);
         case EFS_STAGE_VEL_EXTRAPOLATE: return efs_launch_extrapolate(s);
@@ -383,6 +387,7 @@ int efs_set_params(efs_handle h, const efs_params* p) {
         return -1;
     }
     h->P = *p;
+    h->traces_valid = false;
     return 0;
 }
 int efs_get_params(efs_handle h, efs_params* p) {
@@ -471,6 +476,7 @@ int efs_solve_pressure(efs_handle h, efs_stats* st) {
 
 int efs_project_divergence(efs_handle h) {
     if (!valid_handle(h)) return -1;
+    h->traces_valid = false;  // the wall enforcement changes the velocity
     const int rc = h->dim == 2 ? efs2_walls_divergence(h) : efs_launch_walls_divergence(h);
     if (rc) return rc;
     EFS_HIP(hipStreamSynchronize(h->stream));

@@ -245,6 +245,10 @@ struct efs_sim {
     uint8_t* bf = nullptr;
     // scratch
     float *tu = nullptr, *tv = nullptr, *tw = nullptr;   // tmpVec (cell-centred advected velocity)
+    // rho_advect leaves its backward traces (the positions traceParticle returns for every
+    // interior cell centre) in tu/tv/tw; vel_advect traces the same cells with the same velocity
+    // and walls, so it reuses them when nothing has changed the velocity, the walls or dt since
+    bool traces_valid = false;
     float *s0 = nullptr, *s1 = nullptr;                   // scalar scratch
     uint8_t *valA = nullptr, *valB = nullptr;             // extrapolation validity
     int *bwd_p0 = nullptr, *fwd_p0 = nullptr;             // advection matrix corner bases (-1 = none)

@@ -48,23 +48,33 @@ __global__ __launch_bounds__(256) void k_rowcode(DevGrid g, uint16_t* rc, CGStat
     const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
     if (x >= g.W || y >= g.H) return;
     const int c = cidx(g, x, y, z);
+    const int sy = g.W, sz = g.W * g.H;
+    // neighbour indices clamped into the array; the in-range test below discards the clamped ones
+    const bool ok[6] = {x > 0, x < g.W - 1, y > 0, y < g.H - 1, z < g.D - 1, z > 0};
+    const int q[6] = {ok[0] ? c - 1 : c, ok[1] ? c + 1 : c, ok[2] ? c - sy : c, ok[3] ? c + sy : c, ok[4] ? c + sz : c,
+                      ok[5] ? c - sz : c};
+    // a neighbour on the outer layer is never counted (CGSolver.cpp:477-482): its displaced
+    // coordinate is on the layer, or the cell itself is on it along another axis
+    const bool ox = x == 0 || x == g.W - 1, oy = y == 0 || y == g.H - 1, oz = z == 0 || z == g.D - 1;
+    const bool outer[6] = {x - 1 == 0 || oy || oz, x + 1 == g.W - 1 || oy || oz, y - 1 == 0 || ox || oz,
+                           y + 1 == g.H - 1 || ox || oz, z + 1 == g.D - 1 || ox || oy, z - 1 == 0 || ox || oy};
+    const unsigned bits[6] = {RC_NB_XM, RC_NB_XP, RC_NB_YM, RC_NB_YP, RC_NB_ZP, RC_NB_ZM};
+    unsigned bq[6];
+    float rq[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        bq[k] = g.bf[q[k]];
+        rq[k] = g.rho[q[k]];
+    }
     const float r = g.rho[c];
     const unsigned b = g.bf[c];
     unsigned code = 0, cnt = 0;
-    auto nb = [&](int nx, int ny, int nz, unsigned bit) {
-        if (nx < 0 || nx >= g.W || ny < 0 || ny >= g.H || nz < 0 || nz >= g.D) return;
-        if (nx == 0 || nx == g.W - 1 || ny == 0 || ny == g.H - 1 || nz == 0 || nz == g.D - 1) return;
-        const int q = cidx(g, nx, ny, nz);
-        if (g.bf[q] & BF_GT05) return;
-        cnt++;
-        if (g.rho[q] > 0.5f) code |= bit;
-    };
-    nb(x - 1, y, z, RC_NB_XM);
-    nb(x + 1, y, z, RC_NB_XP);
-    nb(x, y - 1, z, RC_NB_YM);
-    nb(x, y + 1, z, RC_NB_YP);
-    nb(x, y, z + 1, RC_NB_ZP);
-    nb(x, y, z - 1, RC_NB_ZM);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const bool counted = ok[k] && !outer[k] && !(bq[k] & BF_GT05);
+        cnt += counted ? 1u : 0u;
+        if (counted && rq[k] > 0.5f) code |= bits[k];
+    }
     code |= cnt << RC_CNT_SHIFT;
     if (!(r <= 0.5f || (b & BF_EQ1))) code |= RC_CG_ROW;
     if (!(r <= 0.5f || (b & BF_GE05))) code |= RC_JAC_ROW;

@@ -10,7 +10,8 @@
 //     8 backward entries per row and 8 forward entries per column; sums over the entries a
 //     cell does not own are scattered in fixed point, the rest are gathered.
 // The density diffusion sweeps (CPUSimulator3D.cpp:276-363) are sequential along a line and
-// independent across lines, so one thread per line reproduces them exactly.
+// independent across lines, so one thread per line reproduces them exactly (lines along x are
+// transposed through LDS so that the loads stay coalesced).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -70,34 +71,31 @@ __global__ __launch_bounds__(256) void k_extrap_pass(DevGrid g, const float* __r
                                                      float* __restrict__ du, float* __restrict__ dv, float* __restrict__ dw,
                                                      uint8_t* __restrict__ dval) {
     INTERIOR_XYZ(g);
-    if (g.bf[c] & BF_EQ1) {
-        du[c] = su[c]; dv[c] = sv[c]; dw[c] = sw[c];
-        dval[c] = 0;
-        return;
-    }
-    if (sval[c]) {
-        du[c] = su[c]; dv[c] = sv[c]; dw[c] = sw[c];
-        dval[c] = 1;
-        return;
-    }
     const int sy = g.W, sz = g.W * g.H;
     const int nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
+    // own values and the six neighbour validity bytes, loaded unconditionally
+    const float u0 = su[c], v0 = sv[c], w0 = sw[c];
+    const unsigned own = sval[c];
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) m |= (sval[nb[k]] ? 1u : 0u) << k;
+    const bool solid = g.bf[c] & BF_EQ1;
+    if (solid || own || m == 0) {
+        du[c] = u0; dv[c] = v0; dw[c] = w0;
+        dval[c] = (!solid && own) ? 1 : 0;
+        return;
+    }
+    // a front cell: mean of the valid neighbours, in the reference's order
     float ax = 0.f, ay = 0.f, az = 0.f, n = 0;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-        const int q = nb[k];
-        if (sval[q]) {
-            ax = ax + su[q]; ay = ay + sv[q]; az = az + sw[q];
+        if (m & (1u << k)) {
+            ax = ax + su[nb[k]]; ay = ay + sv[nb[k]]; az = az + sw[nb[k]];
             n++;
         }
     }
-    if (n > 0) {
-        du[c] = ax / n; dv[c] = ay / n; dw[c] = az / n;
-        dval[c] = 1;
-    } else {
-        du[c] = su[c]; dv[c] = sv[c]; dw[c] = sw[c];
-        dval[c] = 0;
-    }
+    du[c] = ax / n; dv[c] = ay / n; dw[c] = az / n;
+    dval[c] = 1;
 }
 
 // ---- rho_sharpen (CPUSimulator3D.cpp:366-456) ----------------------------------------------
@@ -223,9 +221,13 @@ __global__ void k_add_fix(int n, float* __restrict__ f, unsigned long long* __re
 
 // Steps 1-4: backward trace, advected gamma, row weights scaled by it, column sums (beta).
 __global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __restrict__ gadv, int* __restrict__ bp0,
-                                                 float* __restrict__ bw, unsigned long long* beta_acc) {
+                                                 float* __restrict__ bw, unsigned long long* beta_acc, float* __restrict__ px,
+                                                 float* __restrict__ py, float* __restrict__ pz) {
     INTERIOR_XYZ(g);
     const float3 prev = traceParticle(g, (float)x, (float)y, (float)z, -dt, 4);
+    px[c] = prev.x;
+    py[c] = prev.y;
+    pz[c] = prev.z;
     const float ga = (g.bf[c] & BF_EQ1) ? 0.f : sinterp(g, g.gamma, prev.x, prev.y, prev.z);
     gadv[c] = ga;
     float m[8];
@@ -399,36 +401,125 @@ __device__ inline void diffuse_pair(float& rj, float& gj, unsigned bj, float& ri
     gi = avg;
 }
 
-// axis 0: lines over (y, z); axis 1: lines over (x, z); axis 2: lines over (x, y)
+// Sweeps along y (AXIS 1, lines over (x, z)) and z (AXIS 2, lines over (x, y)): one thread per
+// line, consecutive threads on consecutive x (coalesced).  The loads of a line do not depend on
+// the sweep (cell i is read before anything writes it), so they are issued DF cells ahead in a
+// register ring indexed by compile-time constants.
+constexpr int DF = 8;
 template <int AXIS>
-__global__ __launch_bounds__(256) void k_diffuse(DevGrid g) {
-    const int a = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // fastest line coordinate
+__global__ __launch_bounds__(64) void k_diffuse(DevGrid g) {
+    const int a = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // x
     const int b = 1 + blockIdx.y;
+    if (a > g.W - 2) return;
     int c0, stride, len;
-    if (AXIS == 0) {  // a = y, b = z
-        if (a > g.H - 2) return;
-        c0 = cidx(g, 1, a, b); stride = 1; len = g.W;
-    } else if (AXIS == 1) {  // a = x, b = z
-        if (a > g.W - 2) return;
+    if (AXIS == 1) {  // b = z
         c0 = cidx(g, a, 1, b); stride = g.W; len = g.H;
-    } else {  // a = x, b = y
-        if (a > g.W - 2) return;
+    } else {          // b = y
         c0 = cidx(g, a, b, 1); stride = g.W * g.H; len = g.D;
+    }
+    // cells 1 .. len-2 of the line; the pair (k-1, k) for k = 2 .. len-2
+    const int ncell = len - 2;
+    float rq[DF], gq[DF];
+    unsigned bq[DF];
+#pragma unroll
+    for (int k = 0; k < DF; k++) {
+        const int kk = min(1 + k, ncell - 1);  // cell index along the line (clamped: surplus loads unused)
+        rq[k] = g.rho[c0 + kk * stride];
+        gq[k] = g.gamma[c0 + kk * stride];
+        bq[k] = g.bf[c0 + kk * stride];
     }
     int cj = c0;
     float rj = g.rho[cj], gj = g.gamma[cj];
     unsigned bj = g.bf[cj];
-    for (int k = 2; k < len - 1; k++) {
-        const int ci = cj + stride;
-        float ri = g.rho[ci], gi = g.gamma[ci];
-        const unsigned bi = g.bf[ci];
-        diffuse_pair(rj, gj, bj, ri, gi, bi);
-        g.rho[cj] = rj;
-        g.gamma[cj] = gj;
-        cj = ci; rj = ri; gj = gi; bj = bi;
+    int k = 1;  // next cell to pair with j (line position k)
+    while (k < ncell) {
+#pragma unroll
+        for (int q = 0; q < DF; q++) {
+            if (k < ncell) {
+                float ri = rq[q], gi = gq[q];
+                const unsigned bi = bq[q];
+                const int kn = min(k + DF, ncell - 1);
+                rq[q] = g.rho[c0 + kn * stride];
+                gq[q] = g.gamma[c0 + kn * stride];
+                bq[q] = g.bf[c0 + kn * stride];
+                diffuse_pair(rj, gj, bj, ri, gi, bi);
+                g.rho[cj] = rj;
+                g.gamma[cj] = gj;
+                cj += stride;
+                rj = ri; gj = gi; bj = bi;
+                k++;
+            }
+        }
     }
     g.rho[cj] = rj;
     g.gamma[cj] = gj;
+}
+
+// Sweep along x (lines over (y, z)): a line is contiguous, so consecutive threads would touch
+// consecutive rows and every load would hit a different cache line.  A block instead stages a
+// tile of DX_ROWS rows x DX_TILE cells through LDS with coalesced row loads; lane r walks row r
+// of the tile, and the finished cells are written back coalesced.  The last cell of a tile is
+// still paired with the first of the next one, so it stays in registers and is stored on its
+// own.
+constexpr int DX_ROWS = 16, DX_TILE = 64;
+__global__ __launch_bounds__(64) void k_diffuse_x(DevGrid g) {
+    __shared__ float sr[DX_ROWS][DX_TILE + 1], sg[DX_ROWS][DX_TILE + 1];
+    __shared__ uint8_t sb[DX_ROWS][DX_TILE + 4];
+    const int lane = threadIdx.x;
+    const int y0 = 1 + blockIdx.x * DX_ROWS, z = 1 + blockIdx.y;
+    const int row = y0 + lane;
+    const bool active = lane < DX_ROWS && row <= g.H - 2;
+    const int xlast = g.W - 2;
+    float rj = 0.f, gj = 0.f;
+    unsigned bj = 0;
+    int cj = 0;
+    for (int x0 = 1; x0 <= xlast; x0 += DX_TILE) {
+        const int n = min(DX_TILE, xlast - x0 + 1);
+        for (int r = 0; r < DX_ROWS; r++) {
+            const int yy = y0 + r;
+            if (yy <= g.H - 2 && lane < n) {
+                const int c = cidx(g, x0 + lane, yy, z);
+                sr[r][lane] = g.rho[c];
+                sg[r][lane] = g.gamma[c];
+                sb[r][lane] = g.bf[c];
+            }
+        }
+        __syncthreads();
+        if (active) {
+            for (int t = 0; t < n; t++) {
+                const int x = x0 + t;
+                float ri = sr[lane][t], gi = sg[lane][t];
+                const unsigned bi = sb[lane][t];
+                if (x > 1) {
+                    diffuse_pair(rj, gj, bj, ri, gi, bi);
+                    // cell x - 1 is final
+                    if (t == 0) {
+                        g.rho[cj] = rj;
+                        g.gamma[cj] = gj;
+                    } else {
+                        sr[lane][t - 1] = rj;
+                        sg[lane][t - 1] = gj;
+                    }
+                }
+                rj = ri; gj = gi; bj = bi;
+                cj = cidx(g, x, row, z);
+            }
+        }
+        __syncthreads();
+        for (int r = 0; r < DX_ROWS; r++) {
+            const int yy = y0 + r;
+            if (yy <= g.H - 2 && lane < n - 1) {
+                const int c = cidx(g, x0 + lane, yy, z);
+                g.rho[c] = sr[r][lane];
+                g.gamma[c] = sg[r][lane];
+            }
+        }
+        __syncthreads();
+    }
+    if (active) {
+        g.rho[cj] = rj;
+        g.gamma[cj] = gj;
+    }
 }
 
 // ---- vel_advect (CPUSimulator3D.cpp:478-504) ----------------------------------------------
@@ -437,6 +528,16 @@ __global__ __launch_bounds__(256) void k_vadv_trace(DevGrid g, float dt, float* 
     INTERIOR_XYZ(g);
     const float3 prev = traceParticle(g, (float)x, (float)y, (float)z, -dt, 4);
     const float3 v = vinterp(g, prev.x - 0.5f, prev.y - 0.5f, prev.z - 0.5f);
+    tu[c] = v.x;
+    tv[c] = v.y;
+    tw[c] = v.z;
+}
+
+// the same, from the traced positions rho_advect stored in tu/tv/tw (in place)
+__global__ __launch_bounds__(256) void k_vadv_from_traces(DevGrid g, float* __restrict__ tu, float* __restrict__ tv,
+                                                          float* __restrict__ tw) {
+    INTERIOR_XYZ(g);
+    const float3 v = vinterp(g, tu[c] - 0.5f, tv[c] - 0.5f, tw[c] - 0.5f);
     tu[c] = v.x;
     tv[c] = v.y;
     tw[c] = v.z;
@@ -584,7 +685,7 @@ int efs_launch_rho_advect(efs_sim* s) {
     const dim3 g1(blocks1d(s->N)), b1(256);
     const float dt = s->P.dt;
     // steps 1-4
-    hipLaunchKernelGGL(k_adv_bwd, gi, blk, 0, s->stream, g, dt, s->s1, s->bwd_p0, s->bwd_w, s->acc0);
+    hipLaunchKernelGGL(k_adv_bwd, gi, blk, 0, s->stream, g, dt, s->s1, s->bwd_p0, s->bwd_w, s->acc0, s->tu, s->tv, s->tw);
     // step 5 (+ forward half of step 6)
     hipLaunchKernelGGL(k_adv_fwd, gi, blk, 0, s->stream, g, dt, s->acc0, s->fwd_p0, s->fwd_w, s->acc1);
     hipLaunchKernelGGL(k_fill_u64, g1, b1, 0, s->stream, s->N, s->acc0);
@@ -597,11 +698,12 @@ int efs_launch_rho_advect(efs_sim* s) {
     hipLaunchKernelGGL(k_adv_scale_cols, gi, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->fwd_p0, s->fwd_w, s->acc1, s->acc2);
     hipLaunchKernelGGL(k_adv_final, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->acc2, s->s0);
     EFS_HIP(hipGetLastError());
+    s->traces_valid = true;
     std::swap(s->rho, s->s0);
     // step 12: 3 iterations of x, y, z sweeps
     const DevGrid g2 = s->grid();
     for (int it = 0; it < 3; it++) {
-        hipLaunchKernelGGL(k_diffuse<0>, dim3((s->H - 2 + 63) / 64, s->D - 2), dim3(64), 0, s->stream, g2);
+        hipLaunchKernelGGL(k_diffuse_x, dim3((s->H - 2 + DX_ROWS - 1) / DX_ROWS, s->D - 2), dim3(64), 0, s->stream, g2);
         hipLaunchKernelGGL(k_diffuse<1>, dim3((s->W - 2 + 63) / 64, s->D - 2), dim3(64), 0, s->stream, g2);
         hipLaunchKernelGGL(k_diffuse<2>, dim3((s->W - 2 + 63) / 64, s->H - 2), dim3(64), 0, s->stream, g2);
     }
@@ -612,7 +714,9 @@ int efs_launch_rho_advect(efs_sim* s) {
 int efs_launch_vel_advect(efs_sim* s) {
     const DevGrid g = s->grid();
     const dim3 grd = interior_grid(s), blk(BX, BY);
-    hipLaunchKernelGGL(k_vadv_trace, grd, blk, 0, s->stream, g, s->P.dt, s->tu, s->tv, s->tw);
+    if (s->traces_valid) hipLaunchKernelGGL(k_vadv_from_traces, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw);
+    else hipLaunchKernelGGL(k_vadv_trace, grd, blk, 0, s->stream, g, s->P.dt, s->tu, s->tv, s->tw);
+    s->traces_valid = false;
     hipLaunchKernelGGL(k_vadv_faces, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw);
     EFS_HIP(hipGetLastError());
     return 0;

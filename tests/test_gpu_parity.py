@@ -196,3 +196,20 @@ def test_cg_relative_criterion_true_residual(hiplib, n):
     assert st.rel_residual < 1e-5
     assert st.true_rel_residual < 1.5e-5
     sim.close()
+
+
+def test_vel_advect_reuses_rho_advect_traces_bitexact(case):
+    """vel_advect right after rho_advect reuses the backward traces rho_advect stored (same cells,
+    velocity and walls): the advected velocity is still bit-exact."""
+    name, sc, state = case
+    sim, o = fresh(sc, state)
+    o.params.key64 = 1
+    o.set_params(o.params)
+    sim.rho_advect()
+    sim.vel_advect()
+    o.stage(3)
+    o.stage(4)
+    v = sim.grid().vel()
+    print(f"[{name}] rho_advect + vel_advect max|dv| = {maxdiff(v, o.field('vel'))}")
+    assert nbits_equal(v, o.field("vel"))
+    sim.close()
