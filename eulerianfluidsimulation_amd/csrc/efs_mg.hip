@@ -12,8 +12,9 @@
 //     coarse correction is twice as large and, measured on the sphere-over-pool scene, halves the
 //     iteration count at 256^3: 35 -> 16.)
 //   Smoother.  Red-black Gauss-Seidel, one sweep down (red, black) and one sweep up (black, red)
-//     so that M is symmetric; the first down pass starts from x = 0 and the first up pass reads
-//     the coarse correction of its neighbours on the fly (the prolongation is never stored).
+//     so that M is symmetric.  The down sweep starts from x = 0, so both of its passes are one
+//     kernel (k_mg_presmooth); the first up pass reads the coarse correction of its neighbours on
+//     the fly (the prolongation is never stored).
 //   Coarsest level (<= 8 cells per axis): 16 symmetric sweeps in one workgroup, in LDS.
 // Non-fluid rows of the fine level return z = 0 (their residual is 0: identity rows).
 #include <hip/hip_runtime.h>
@@ -147,7 +148,6 @@ __device__ inline float nb_sum(const Lvl& l, unsigned rc, int i, int x, int y, i
 }
 
 // One red-black Gauss-Seidel pass over the cells of `color` ((x + y + z) & 1).  mode 0: normal;
-// mode 1: first pass from x = 0 (cells of the other colour are zeroed, own cells get -b/(s n));
 // mode 2: neighbours read x + the coarse correction of their parent (the first pass after the
 // coarse-grid correction).
 __global__ __launch_bounds__(256) void k_mg_smooth(Lvl l, int color, int mode, Lvl cl, const CGState* st) {
@@ -157,16 +157,52 @@ __global__ __launch_bounds__(256) void k_mg_smooth(Lvl l, int color, int mode, L
     // thread k owns the pair of cells (1 + 2k, 2 + 2k) of its row: one of each colour
     const int xa = 1 + 2 * (int)(blockIdx.x * MB + threadIdx.x);
     if (xa > l.nx) return;
-    const bool first_own = ((xa + y + z) & 1) == color;
-    const int x = first_own ? xa : xa + 1, xo = first_own ? xa + 1 : xa;
-    if (mode == 1 && xo <= l.nx) l.x[lidx(l, xo, y, z)] = 0.f;  // the other colour starts at 0
+    const int x = (((xa + y + z) & 1) == color) ? xa : xa + 1;
     if (x > l.nx) return;
     const int i = lidx(l, x, y, z);
     const unsigned rc = l.rc[i];
     const float bi = l.b[i];
-    const float sum = (mode == 1) ? 0.f : nb_sum(l, rc, i, x, y, z, mode == 2 ? &cl : nullptr);
+    const float sum = nb_sum(l, rc, i, x, y, z, mode == 2 ? &cl : nullptr);
     const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
     l.x[i] = ((rc & RC_CG_ROW) && cnt > 0) ? (sum - bi / l.s) / (float)cnt : 0.f;
+}
+
+// The first down sweep from x = 0 (red pass, then black pass) in one kernel: a red cell gets
+// -b / (s n); a black cell gets (sum of its fluid red neighbours' -b_j / (s n_j) - b / s) / n,
+// the values the two passes would compute, in the same operations.
+__global__ __launch_bounds__(256) void k_mg_presmooth(Lvl l, const CGState* st) {
+    if (st->done) return;
+    const int y = 1 + blockIdx.y * MBY + threadIdx.y, z = 1 + blockIdx.z;
+    if (y > l.ny) return;
+    const int xa = 1 + 2 * (int)(blockIdx.x * MB + threadIdx.x);
+    if (xa > l.nx) return;
+    const int sy = l.W, sz = l.W * l.H;
+    auto red_value = [&](unsigned rc, float b) -> float {
+        const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
+        return ((rc & RC_CG_ROW) && cnt > 0) ? (0.f - b / l.s) / (float)cnt : 0.f;
+    };
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int x = xa + h;
+        if (x > l.nx) break;
+        const int i = lidx(l, x, y, z);
+        const unsigned rc = l.rc[i];
+        const float bi = l.b[i];
+        if (((x + y + z) & 1) == 0) {
+            l.x[i] = red_value(rc, bi);
+        } else {
+            const int nb[6] = {i - 1, i + 1, i - sy, i + sy, i + sz, i - sz};
+            const unsigned bits[6] = {RC_NB_XM, RC_NB_XP, RC_NB_YM, RC_NB_YP, RC_NB_ZP, RC_NB_ZM};
+            float v[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) v[k] = red_value(l.rc[nb[k]], l.b[nb[k]]);
+            float sum = 0.f;
+#pragma unroll
+            for (int k = 0; k < 6; k++) sum += (rc & bits[k]) ? v[k] : 0.f;
+            const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
+            l.x[i] = ((rc & RC_CG_ROW) && cnt > 0) ? (sum - bi / l.s) / (float)cnt : 0.f;
+        }
+    }
 }
 
 // residual of level f restricted to level c: b_c(I) = 1/8 sum over the fluid children of
@@ -313,7 +349,7 @@ int vcycle(efs_sim* s, void*) {
     const int nl = (int)m->L.size();
     const dim3 blk(MB, MBY);
     const CGState* st = s->st;
-    // fine-level passes are event-timed when kernel timing is on
+    // the fine-level passes of the up sweep are event-timed when kernel timing is on
     auto smooth = [&](int k, int color, int mode, const Lvl& cl) {
         const Lvl& l = m->L[k];
         hipEvent_t a = nullptr, b = nullptr;
@@ -323,8 +359,7 @@ int vcycle(efs_sim* s, void*) {
     };
     for (int k = 0; k < nl - 1; k++) {
         const Lvl& l = m->L[k];
-        smooth(k, 0, 1, l);
-        smooth(k, 1, 0, l);
+        hipLaunchKernelGGL(k_mg_presmooth, color_grid(l), blk, 0, s->stream, l, st);
         const Lvl& c = m->L[k + 1];
         hipLaunchKernelGGL(k_mg_restrict, dim3((c.nx + MB - 1) / MB, (c.ny + MBY - 1) / MBY, c.nz), blk, 0, s->stream, l, c, st);
     }
@@ -370,8 +405,8 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
     for (int attempt = 0; attempt < 3; attempt++) {
         const size_t kt0 = s->ktimers[KT_OTHER].start.size();
         rc = efs_cg_run_pc(s, o, false, std::max(1, s->P.max_steps - total), &pc);
-        // launches after convergence return at once: keep the 4 fine passes of each real V-cycle
-        efs_trim_timer(s->ktimers[KT_OTHER], kt0 + 4 * ((size_t)s->st_host->iter + 1));
+        // launches after convergence return at once: keep the 2 timed fine passes of each real V-cycle
+        efs_trim_timer(s->ktimers[KT_OTHER], kt0 + 2 * ((size_t)s->st_host->iter + 1));
         total += o->iterations;
         const bool verify = s->P.true_residual && s->P.criterion == EFS_CRIT_RELATIVE;
         if (rc != 0 || !verify || o->true_rel_residual <= (double)s->P.solver_threshold) break;

@@ -1,6 +1,7 @@
-// C++ host side of the HIP backend: HipSimulator3D / HipGrid3D mirror the reference's backend
-// interface (ISimulator3D, include/simulation/simulator/ISimulator.h:8-53 and ISimulator3D.h;
-// IGrid3D, include/simulation/data/IGrid.h:44-68) over the C ABI of include/efs.h.
+// C++ host side of the HIP backend: HipSimulator3D / HipGrid3D and HipSimulator2D / HipGrid2D
+// mirror the reference's backend interface (ISimulator3D / ISimulator2D,
+// include/simulation/simulator/ISimulator.h:8-53; IGrid3D / IGrid2D,
+// include/simulation/data/IGrid.h) over the C ABI of include/efs.h.
 //
 // The device is authoritative: fields stay resident in HBM across steps.  Host mirrors are
 // downloaded on first access after a step (the lazy download CudaGrid3D does with setDirty(),
@@ -30,9 +31,10 @@ inline int check(int rc, const char* what, bool allow_status = false) {
     return rc;
 }
 
-class HipGrid3D {
+// Host mirrors of the eight fields; vcomp = floats per cell of vel / forces (3 in 3D, 2 in 2D).
+class HipGridBase {
 public:
-    HipGrid3D(efs_handle h, unsigned w, unsigned hgt, unsigned d) : _h(h), _w(w), _hgt(hgt), _d(d) {
+    HipGridBase(efs_handle h, unsigned w, unsigned hgt, unsigned d, int vcomp) : _h(h), _w(w), _hgt(hgt), _d(d), _vcomp(vcomp) {
         for (int f = 0; f < 8; f++) {
             _mirror[f].resize((size_t)w * hgt * d * ncomp(f));
             _host_stale[f] = true;
@@ -44,7 +46,7 @@ public:
     unsigned depth() const { return _d; }
     unsigned size() const { return _w * _hgt * _d; }
 
-    // IGrid3D accessors (reference layout: float per cell, float[3] per cell for vel/forces)
+    // IGrid accessors (reference layout: float per cell, float[vcomp] per cell for vel/forces)
     std::vector<float>& vel() { return field(EFS_FIELD_VEL); }
     std::vector<float>& rho() { return field(EFS_FIELD_RHO); }
     std::vector<float>& gamma() { return field(EFS_FIELD_GAMMA); }
@@ -72,7 +74,7 @@ public:
     }
 
 private:
-    static int ncomp(int f) { return (f == EFS_FIELD_VEL || f == EFS_FIELD_FORCES) ? 3 : 1; }
+    int ncomp(int f) const { return (f == EFS_FIELD_VEL || f == EFS_FIELD_FORCES) ? _vcomp : 1; }
     std::vector<float>& field(int f) {
         if (_host_stale[f] && !_host_modified[f]) {
             check(efs_download(_h, f, _mirror[f].data(), _mirror[f].size() * sizeof(float)), "efs_download");
@@ -82,34 +84,37 @@ private:
     }
     efs_handle _h;
     unsigned _w, _hgt, _d;
+    int _vcomp;
     std::array<std::vector<float>, 8> _mirror;
     std::array<bool, 8> _host_stale, _host_modified;
 };
 
-// ISimulator3D on one GPU.  update() runs the seven stages of ISimulator::update in order with
-// the DISABLE_* gating (efs_params::disable_mask); the stage methods run one stage each.
-class HipSimulator3D {
+class HipGrid3D : public HipGridBase {
 public:
-    struct RenderData3D {  // SimulationDisplay3D::RenderData3D (dim, grid)
-        int dim = 3;
-        HipGrid3D* grid = nullptr;
+    HipGrid3D(efs_handle h, unsigned w, unsigned hgt, unsigned d) : HipGridBase(h, w, hgt, d, 3) {}
+};
+
+class HipGrid2D : public HipGridBase {
+public:
+    HipGrid2D(efs_handle h, unsigned w, unsigned hgt) : HipGridBase(h, w, hgt, 1, 2) {}
+};
+
+// ISimulator on one GPU.  update() runs the seven stages of ISimulator::update in order with
+// the DISABLE_* gating (efs_params::disable_mask); the stage methods run one stage each.
+template <class Grid, int Dim>
+class HipSimulatorT {
+public:
+    struct RenderData {  // SimulationDisplay{2D,3D}::RenderData{2D,3D} (dim, grid)
+        int dim = Dim;
+        Grid* grid = nullptr;
     };
 
-    HipSimulator3D(unsigned width, unsigned height, unsigned depth, const efs_params* params = nullptr)
-        : _width(width), _height(height), _depth(depth) {
-        efs_params p;
-        if (params) p = *params;
-        else efs_default_params(&p);
-        check(efs_create_3d((int32_t)width, (int32_t)height, (int32_t)depth, &p, &_h), "efs_create_3d");
-        _grid = new HipGrid3D(_h, width, height, depth);
-        _data.grid = _grid;
-    }
-    ~HipSimulator3D() {
+    HipSimulatorT(const HipSimulatorT&) = delete;
+    HipSimulatorT& operator=(const HipSimulatorT&) = delete;
+    virtual ~HipSimulatorT() {
         delete _grid;
         efs_destroy(_h);
     }
-    HipSimulator3D(const HipSimulator3D&) = delete;
-    HipSimulator3D& operator=(const HipSimulator3D&) = delete;
 
     void setParams(const efs_params& p) { check(efs_set_params(_h, &p), "efs_set_params"); }
 
@@ -124,7 +129,7 @@ public:
         return rc;
     }
     void* getRenderData() { return &_data; }
-    HipGrid3D& grid() { return *_grid; }
+    Grid& grid() { return *_grid; }
     const efs_stats& lastStats() const { return _stats; }
     bool paused() const { return _paused; }
     long frames() const { return _frames; }
@@ -139,6 +144,14 @@ public:
     void add_force() { stage(EFS_STAGE_ADD_FORCE); }
     int project() { return stage(EFS_STAGE_PROJECT); }
 
+protected:
+    HipSimulatorT() = default;
+    void attach(efs_handle h, Grid* g) {
+        _h = h;
+        _grid = g;
+        _data.grid = g;
+    }
+
 private:
     int stage(int s) {
         _grid->uploadModified();
@@ -146,13 +159,40 @@ private:
         _grid->setDirty();
         return rc;
     }
-    unsigned _width, _height, _depth;
     efs_handle _h = nullptr;
-    HipGrid3D* _grid = nullptr;
-    RenderData3D _data;
+    Grid* _grid = nullptr;
+    RenderData _data;
     efs_stats _stats{};
     bool _paused = false;
     long _frames = 0;
+};
+
+// ISimulator3D(width, height, depth): CPUSimulator3D's semantics.
+class HipSimulator3D : public HipSimulatorT<HipGrid3D, 3> {
+public:
+    using RenderData3D = RenderData;
+    HipSimulator3D(unsigned width, unsigned height, unsigned depth, const efs_params* params = nullptr) {
+        efs_params p;
+        if (params) p = *params;
+        else efs_default_params(&p);
+        efs_handle h = nullptr;
+        check(efs_create_3d((int32_t)width, (int32_t)height, (int32_t)depth, &p, &h), "efs_create_3d");
+        attach(h, new HipGrid3D(h, width, height, depth));
+    }
+};
+
+// ISimulator2D(width, height): CPUSimulator2D's semantics.
+class HipSimulator2D : public HipSimulatorT<HipGrid2D, 2> {
+public:
+    using RenderData2D = RenderData;
+    HipSimulator2D(unsigned width, unsigned height, const efs_params* params = nullptr) {
+        efs_params p;
+        if (params) p = *params;
+        else efs_default_params(&p);
+        efs_handle h = nullptr;
+        check(efs_create_2d((int32_t)width, (int32_t)height, &p, &h), "efs_create_2d");
+        attach(h, new HipGrid2D(h, width, height));
+    }
 };
 
 }  // namespace efs

@@ -1,5 +1,6 @@
-"""The C++ host adapter (HipSimulator3D.h) drives the backend: mass is conserved and every
-step converges on a small dam break."""
+"""The C++ host adapter (HipSimulator3D.h: HipSimulator3D / HipSimulator2D) drives the backend:
+mass is conserved and every step converges on a small dam break (3D plain CG, 3D multigrid
+preconditioned CG, 2D)."""
 import os
 import re
 import subprocess
@@ -11,9 +12,10 @@ from eulerianfluidsimulation_amd import build
 pytestmark = pytest.mark.gpu
 
 
-def test_cpp_driver_runs(hiplib):
+@pytest.mark.parametrize("extra", [[], ["mg"], ["2d"]])
+def test_cpp_driver_runs(hiplib, extra):
     assert os.path.exists(build.HOST_BIN), "host driver not built"
-    out = subprocess.run([build.HOST_BIN, "24", "4"], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([build.HOST_BIN, "24", "4"] + extra, capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     lines = [l for l in out.stdout.splitlines() if l.startswith("step")]
