@@ -35,6 +35,26 @@ K2_BYTES_PC = 2 + 4 + 8 + 4 + 8 + 4  # preconditioned CG: the FP64 iterate is re
 MG_SMOOTH_BYTES = 0.5 * (2 + 4 + 4) + 0.5 * 4
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_sphere_pool_256_mg.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (calibrated against a copy of known size), for the default workload only."""
+    if not (ARGS.n == 256 and ARGS.config == "sphere_pool" and ARGS.precond == "mg"):
+        return None
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_json:"):
+                    t = json.loads(line.split(":", 1)[1])
+                    key = {"cg_update": "cg_update_pcg"}.get(kernel, kernel)
+                    return t.get(key)
+    except OSError:
+        return None
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,7 +218,7 @@ def main():
         "true_rel_residual_max": float(max(true_res)) if true_res else None,
         "kernel_events": {k: {"launches": v[0], "avg_us": 1e3 * v[1] / max(1, v[0])} for k, v in kt.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
                      "algorithmic_bytes_per_launch": bytes_per_launch, "real_iterations": real},
     }
     sim.close()

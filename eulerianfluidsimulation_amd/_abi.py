@@ -102,6 +102,8 @@ SIGNATURES = {
     "efs_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]),
     "efs_scene_rasterise": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prim), ctypes.c_int32]),
     "efs_scene_face_noise": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double]),
+    "efs_scene_voxelise": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
+                                          ctypes.POINTER(ctypes.c_float), ctypes.c_int32, ctypes.c_float, ctypes.c_float]),
     "efs_unet_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     "efs_unet_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "efs_unet_num_params": (ctypes.c_int64, [ctypes.c_void_p]),

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libefs_hip.so")
-SOURCES = ["efs_api.hip", "efs_step.hip", "efs_solver.hip", "efs_learned.hip", "efs_unet.hip", "efs_2d.hip", "efs_mg.hip"]
+SOURCES = ["efs_api.hip", "efs_step.hip", "efs_solver.hip", "efs_learned.hip", "efs_unet.hip", "efs_2d.hip", "efs_mg.hip", "efs_scene.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("EFS_OFFLOAD_ARCH", "gfx950")
 

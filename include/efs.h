@@ -170,6 +170,14 @@ typedef struct {
 int efs_scene_rasterise(efs_handle h, const efs_prim* prims, int32_t nprims);
 /* N(0, sigma) noise on faces that touch no solid, counter-based from seed (stream 1). */
 int efs_scene_face_noise(efs_handle h, uint64_t seed, double sigma);
+/* Reference: the editor's mesh voxelisation (assets/shaders/sdf.comp, dispatched by
+ * Editor::initFluid / initBoundaries, src/core/Editor.cpp:90-258).  triangles: ntris x 9 floats
+ * (v0, v1, v2) in world space; the domain box (SimulationDomainNode position = centre, scale =
+ * size) maps to the W x H x D cells.  field (EFS_FIELD_RHO or EFS_FIELD_BOUNDARY) is overwritten:
+ * value_inside where the cell centre is inside the mesh (|winding number| > pi, or on a
+ * triangle), value_outside elsewhere and on the last layer of each axis. */
+int efs_scene_voxelise(efs_handle h, const float* triangles, int32_t ntris, const float domain_center[3],
+                       const float domain_size[3], int32_t field, float value_inside, float value_outside);
 
 /* ---- Learned preconditioner (no counterpart in the reference: it accelerates the solve that
  * CGSolverCPU3D::solve performs, CGSolver.cpp:416-465) ------------------------------------
