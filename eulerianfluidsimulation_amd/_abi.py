@@ -67,6 +67,17 @@ class LearnedOpts(ctypes.Structure):
     ]
 
 
+class QueryResult(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_int32),
+        ("max_pos", ctypes.c_float * 3),
+        ("min_pos", ctypes.c_float * 3),
+        ("avg_pos", ctypes.c_float * 3),
+        ("vel", ctypes.c_float * 3),
+        ("pressure", ctypes.c_float),
+    ]
+
+
 class Prim(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32),
@@ -104,6 +115,11 @@ SIGNATURES = {
     "efs_scene_face_noise": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double]),
     "efs_scene_voxelise": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
                                           ctypes.POINTER(ctypes.c_float), ctypes.c_int32, ctypes.c_float, ctypes.c_float]),
+    "efs_scene_force_region": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
+                                              ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int32,
+                                              ctypes.POINTER(ctypes.c_int32)]),
+    "efs_query_box": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+                                     ctypes.POINTER(QueryResult)]),
     "efs_unet_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     "efs_unet_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "efs_unet_num_params": (ctypes.c_int64, [ctypes.c_void_p]),

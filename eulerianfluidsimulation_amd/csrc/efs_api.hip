@@ -18,6 +18,20 @@ int efs_hip_fail(hipError_t e, const char* what) {
     return -(1000 + (int)e);
 }
 
+int ensure_forces(efs_sim* s) {
+    if (s->fu) return 0;
+    const size_t b = (size_t)s->N * 4;
+    EFS_HIP(hipMalloc(&s->fu, b));
+    EFS_HIP(hipMalloc(&s->fv, b));
+    EFS_HIP(hipMemsetAsync(s->fu, 0, b, s->stream));
+    EFS_HIP(hipMemsetAsync(s->fv, 0, b, s->stream));
+    if (s->dim == 3) {
+        EFS_HIP(hipMalloc(&s->fw, b));
+        EFS_HIP(hipMemsetAsync(s->fw, 0, b, s->stream));
+    }
+    return 0;
+}
+
 namespace {
 
 // AoS float[3] <-> three SoA arrays
@@ -80,20 +94,6 @@ float* scalar_ptr(efs_sim* s, int f) {
         case EFS_FIELD_BOUNDARY: return s->bndf;
     }
     return nullptr;
-}
-
-int ensure_forces(efs_sim* s) {
-    if (s->fu) return 0;
-    const size_t b = (size_t)s->N * 4;
-    EFS_HIP(hipMalloc(&s->fu, b));
-    EFS_HIP(hipMalloc(&s->fv, b));
-    EFS_HIP(hipMemsetAsync(s->fu, 0, b, s->stream));
-    EFS_HIP(hipMemsetAsync(s->fv, 0, b, s->stream));
-    if (s->dim == 3) {
-        EFS_HIP(hipMalloc(&s->fw, b));
-        EFS_HIP(hipMemsetAsync(s->fw, 0, b, s->stream));
-    }
-    return 0;
 }
 
 // Moves `bytes` of field f between the library and a caller buffer of the given memcpy kind.

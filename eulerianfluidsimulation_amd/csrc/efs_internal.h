@@ -305,6 +305,7 @@ int efs_launch_add_force(efs_sim* s);
 int efs_launch_walls_divergence(efs_sim* s);
 int efs_launch_gradient_walls(efs_sim* s);
 int efs_launch_boundary_flags(efs_sim* s);
+int ensure_forces(efs_sim* s);  // allocates the forces field (zero) on first use
 
 // solver launchers (efs_solver.hip)
 int efs_solver_alloc(efs_sim* s);

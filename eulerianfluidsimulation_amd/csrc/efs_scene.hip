@@ -8,7 +8,16 @@
 // centre lying on a triangle counts as inside).
 //
 // One thread per cell; the triangles stream through LDS in chunks shared by the workgroup.
+//
+// The same inside test gives the cells of a force field (Editor::updateForceFields with
+// assets/shaders/forcefield_cells.comp), and a box reduction gives QueryNode's per-frame
+// statistics without downloading the fields.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <vector>
 
 #include "efs_internal.h"
 
@@ -59,7 +68,8 @@ __device__ inline bool on_triangle(float3 p, float3 v1, float3 v2, float3 v3) {
 // leaves the last layer of each axis to the cleared buffer, i.e. v_out); cell centres at
 // domain_min + cell_size * (i + 0.5) with cell_size = domain_size / (W, H, D).
 __global__ __launch_bounds__(VX_NT) void k_voxelise(int W, int H, int D, const float* __restrict__ tris, int ntris, float3 dmin,
-                                                    float3 csize, float v_in, float v_out, float* __restrict__ field) {
+                                                    float3 csize, float v_in, float v_out, float* __restrict__ field,
+                                                    unsigned int* __restrict__ count) {
     __shared__ float st[VX_CHUNK * 9];
     const int n = W * H * D;
     const int c = blockIdx.x * VX_NT + threadIdx.x;
@@ -83,10 +93,83 @@ __global__ __launch_bounds__(VX_NT) void k_voxelise(int W, int H, int D, const f
             }
         }
     }
-    if (c < n) field[c] = (active && (fabsf(w) > 3.1415926f || on)) ? v_in : v_out;
+    const bool inside = active && (fabsf(w) > 3.1415926f || on);
+    if (c < n) field[c] = inside ? v_in : v_out;
+    if (count && inside) atomicAdd(count, 1u);
+}
+
+// forces = f on the cells whose mask is set (the force-field cells, Editor::updateForceFields)
+__global__ void k_apply_force(int n, const float* __restrict__ mask, float fx, float fy, float fz, float* __restrict__ fu,
+                              float* __restrict__ fv, float* __restrict__ fw) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || mask[c] != 1.f) return;
+    fu[c] = fx;
+    fv[c] = fy;
+    fw[c] = fz;
+}
+
+// QueryNode::update's box statistics (QueryNode.cpp:58-76): per block, over the cells of the
+// box [lo, hi) (clipped to [0, n-1)) with rho >= 0.5 when only_fluid: count, min / max cell
+// coordinates, and FP64 sums of the coordinates, the face velocities and the pressure.
+constexpr int QY_NT = 256, QY_MAXB = 512;
+struct QueryPart {
+    double sum[7];  // x, y, z, u, v, w, p
+    int cnt, mn[3], mx[3], pad;
+};
+__global__ __launch_bounds__(QY_NT) void k_query_box(DevGrid g, int3 lo, int3 hi, int only_fluid, QueryPart* parts) {
+    const int ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
+    const long long n = (long long)ex * ey * ez;
+    double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+    int cnt = 0, mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (long long t = (long long)blockIdx.x * QY_NT + threadIdx.x; t < n; t += (long long)gridDim.x * QY_NT) {
+        const int x = lo.x + (int)(t % ex), y = lo.y + (int)((t / ex) % ey), z = lo.z + (int)(t / ((long long)ex * ey));
+        const int c = cidx(g, x, y, z);
+        if (only_fluid && g.rho[c] < 0.5f) continue;
+        cnt++;
+        const int xyz[3] = {x, y, z};
+        for (int k = 0; k < 3; k++) {
+            mn[k] = min(mn[k], xyz[k]);
+            mx[k] = max(mx[k], xyz[k]);
+            sum[k] += (double)xyz[k];
+        }
+        sum[3] += (double)g.u[c];
+        sum[4] += (double)g.v[c];
+        sum[5] += g.w ? (double)g.w[c] : 0.0;
+        sum[6] += (double)g.p[c];
+    }
+    __shared__ QueryPart sh[QY_NT / 64];
+    // wave reduction, then the waves in order
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int k = 0; k < 7; k++) sum[k] += __shfl_down(sum[k], o, 64);
+        cnt += __shfl_down(cnt, o, 64);
+        for (int k = 0; k < 3; k++) {
+            mn[k] = min(mn[k], __shfl_down(mn[k], o, 64));
+            mx[k] = max(mx[k], __shfl_down(mx[k], o, 64));
+        }
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        for (int k = 0; k < 7; k++) sh[wv].sum[k] = sum[k];
+        sh[wv].cnt = cnt;
+        for (int k = 0; k < 3; k++) { sh[wv].mn[k] = mn[k]; sh[wv].mx[k] = mx[k]; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        QueryPart q = sh[0];
+        for (int w = 1; w < QY_NT / 64; w++) {
+            for (int k = 0; k < 7; k++) q.sum[k] += sh[w].sum[k];
+            q.cnt += sh[w].cnt;
+            for (int k = 0; k < 3; k++) { q.mn[k] = min(q.mn[k], sh[w].mn[k]); q.mx[k] = max(q.mx[k], sh[w].mx[k]); }
+        }
+        parts[blockIdx.x] = q;
+    }
 }
 
 }  // namespace
+
+namespace {
+inline float3 f3(const float* v) { return make_float3(v[0], v[1], v[2]); }
+}
 
 extern "C" int efs_scene_voxelise(efs_handle h, const float* triangles, int32_t ntris, const float domain_center[3],
                                   const float domain_size[3], int32_t field, float value_inside, float value_outside) {
@@ -110,7 +193,7 @@ extern "C" int efs_scene_voxelise(efs_handle h, const float* triangles, int32_t 
                                     domain_center[2] - 0.5f * domain_size[2]);
     float* out = field == EFS_FIELD_RHO ? h->rho : h->bndf;
     hipLaunchKernelGGL(k_voxelise, dim3((h->N + VX_NT - 1) / VX_NT), dim3(VX_NT), 0, h->stream, W, H, D, dtris, ntris, dmin, csize,
-                       value_inside, value_outside, out);
+                       value_inside, value_outside, out, (unsigned int*)nullptr);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (dtris) hipFree(dtris);
@@ -120,5 +203,104 @@ extern "C" int efs_scene_voxelise(efs_handle h, const float* triangles, int32_t 
         if (rc) return rc;
         EFS_HIP(hipStreamSynchronize(h->stream));
     }
+    return 0;
+}
+
+// Editor::updateForceFields for one force field (src/core/Editor.cpp:260-330 and
+// assets/shaders/forcefield_cells.comp): the cells inside the field's mesh get force.  A field
+// covering more than max_cells cells is skipped, as the reference does (max_cells <= 0: no
+// limit); *ncells receives the count either way.
+extern "C" int efs_scene_force_region(efs_handle h, const float* triangles, int32_t ntris, const float domain_center[3],
+                                      const float domain_size[3], const float force[3], int32_t max_cells, int32_t* ncells) {
+    if (!h || (ntris > 0 && !triangles) || !domain_center || !domain_size || !force || ntris < 0) {
+        efs_set_error("efs_scene_force_region: invalid argument");
+        return -1;
+    }
+    if (h->dim != 3) {
+        efs_set_error("efs_scene_force_region: 3D only");
+        return -1;
+    }
+    if (int rc = ensure_forces(h)) return rc;
+    float* dtris = nullptr;
+    unsigned int* dcount = nullptr;
+    EFS_HIP(hipMalloc(&dcount, sizeof(unsigned int)));
+    EFS_HIP(hipMemsetAsync(dcount, 0, sizeof(unsigned int), h->stream));
+    if (ntris > 0) {
+        EFS_HIP(hipMalloc(&dtris, (size_t)ntris * 9 * sizeof(float)));
+        EFS_HIP(hipMemcpyAsync(dtris, triangles, (size_t)ntris * 9 * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    }
+    const int W = h->W, H = h->H, D = h->D;
+    const float3 csize = make_float3(domain_size[0] / (float)W, domain_size[1] / (float)H, domain_size[2] / (float)D);
+    const float3 dmin = make_float3(domain_center[0] - 0.5f * domain_size[0], domain_center[1] - 0.5f * domain_size[1],
+                                    domain_center[2] - 0.5f * domain_size[2]);
+    // the mask goes to the stage scratch s0 (1 inside, 0 outside)
+    hipLaunchKernelGGL(k_voxelise, dim3((h->N + VX_NT - 1) / VX_NT), dim3(VX_NT), 0, h->stream, W, H, D, dtris, ntris, dmin, csize,
+                       1.f, 0.f, h->s0, dcount);
+    unsigned int cnt = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, dcount, sizeof(cnt), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && (max_cells <= 0 || (int64_t)cnt <= max_cells)) {
+        hipLaunchKernelGGL(k_apply_force, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->N, h->s0, force[0], force[1], force[2],
+                           h->fu, h->fv, h->fw);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    if (dtris) hipFree(dtris);
+    hipFree(dcount);
+    if (e != hipSuccess) return efs_hip_fail(e, "efs_scene_force_region");
+    if (ncells) *ncells = (int32_t)cnt;
+    h->traces_valid = false;
+    return 0;
+}
+
+// QueryNode::update's statistics of one box (src/simulation/scene/nodes/QueryNode.cpp:45-84),
+// reduced on the device (FP64 sums; the reference sums in FP32 in loop order).
+extern "C" int efs_query_box(efs_handle h, const int32_t lo_in[3], const int32_t hi_in[3], int32_t only_fluid, efs_query_result* out) {
+    if (!h || !lo_in || !hi_in || !out) {
+        efs_set_error("efs_query_box: invalid argument");
+        return -1;
+    }
+    const int dims[3] = {h->W, h->H, h->D};
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {  // [max(0, lo), min(n - 1, hi)) as the reference's loops
+        lo[k] = std::max(0, lo_in[k]);
+        hi[k] = std::max(lo[k], std::min(dims[k] - 1, hi_in[k]));
+    }
+    if (h->dim == 2) {  // the 2D grid is one plane
+        lo[2] = 0;
+        hi[2] = 1;
+    }
+    std::memset(out, 0, sizeof(*out));
+    const long long n = (long long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+    if (n == 0) return 0;
+    const int nb = (int)std::min<long long>(QY_MAXB, (n + QY_NT - 1) / QY_NT);
+    QueryPart* dparts = nullptr;
+    EFS_HIP(hipMalloc(&dparts, (size_t)nb * sizeof(QueryPart)));
+    hipLaunchKernelGGL(k_query_box, dim3(nb), dim3(QY_NT), 0, h->stream, h->grid(), make_int3(lo[0], lo[1], lo[2]),
+                       make_int3(hi[0], hi[1], hi[2]), only_fluid, dparts);
+    std::vector<QueryPart> parts((size_t)nb);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(parts.data(), dparts, (size_t)nb * sizeof(QueryPart), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(dparts);
+    if (e != hipSuccess) return efs_hip_fail(e, "efs_query_box");
+    double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+    long long cnt = 0;
+    int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (const QueryPart& q : parts) {
+        for (int k = 0; k < 7; k++) sum[k] += q.sum[k];
+        cnt += q.cnt;
+        for (int k = 0; k < 3; k++) { mn[k] = std::min(mn[k], q.mn[k]); mx[k] = std::max(mx[k], q.mx[k]); }
+    }
+    out->count = (int32_t)cnt;
+    if (cnt == 0) return 0;  // positions stay 0 (QueryNode.cpp:82-84)
+    for (int k = 0; k < 3; k++) {
+        out->max_pos[k] = (float)mx[k];
+        out->min_pos[k] = (float)mn[k];
+        out->avg_pos[k] = (float)(sum[k] / (double)cnt);
+        out->vel[k] = (float)(sum[3 + k] / (double)cnt);
+    }
+    out->pressure = (float)(sum[6] / (double)cnt);
     return 0;
 }

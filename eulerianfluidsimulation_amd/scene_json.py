@@ -14,8 +14,15 @@ backend's entry points:
     the domain border is made solid and rho is cleared under every solid (Editor.cpp:191-257).
 Meshes are Wavefront OBJ files ("v" and "f" records, polygons fanned into triangles), placed by
 the node's translate * rotate * scale transform (Transform.cpp:70-76; "rotation" is stored as
-x, y, z, w, ModelNode.cpp:29-30).  QueryNode, DeletionNode and force fields are not part of the
-grid setup and are returned as plain records.
+x, y, z, w, ModelNode.cpp:29-30).
+
+ForceFieldNodes (ForceFieldNode.cpp:24-45) become the forces field as
+Editor::updateForceFields sets it (Editor.cpp:260-330): each field's cells are the cell centres
+inside its transformed unit cube [-0.5, 0.5]^3 (the voxeliser's inside test,
+assets/shaders/forcefield_cells.comp), a field over "Max ForceField cells" (default 1000) is
+skipped, and the active fields write direction * strength on their cells in scene order, the
+direction being the normalised first column of the node's transform.  QueryNodes are returned
+as query.QueryNode objects; other node types are kept as plain records.
 """
 import ctypes
 import json
@@ -51,6 +58,32 @@ class MeshNode:
 
 
 @dataclass
+class ForceFieldSpec:
+    name: str
+    position: Tuple[float, float, float]
+    rotation_xyzw: Tuple[float, float, float, float]
+    scale: Tuple[float, float, float]
+    active: bool = True
+    strength: float = 5.0
+
+    def matrix(self) -> np.ndarray:
+        """The 3x3 linear part rotate * scale of the node's transform."""
+        return (quat_matrix(*self.rotation_xyzw) * np.asarray(self.scale, np.float32)[None, :]).astype(np.float32)
+
+    def direction(self) -> np.ndarray:
+        """normalize(vec3(transform[0])) (ForceFieldNode.cpp:89-93)."""
+        c = self.matrix()[:, 0].astype(np.float32)
+        return (c / np.float32(np.sqrt(np.float32(c @ c)))).astype(np.float32)
+
+    def triangles(self) -> np.ndarray:
+        """The unit cube's 12 triangles (include/assets/Primitives.h BoxGeometry) in world space,
+        (12, 9) float32."""
+        from .meshes import box
+        cube = np.asarray(box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)), np.float32)
+        return np.ascontiguousarray((cube @ self.matrix().T + np.asarray(self.position, np.float32)).reshape(-1, 9), np.float32)
+
+
+@dataclass
 class SceneSpec:
     resolution: Tuple[int, int, int]
     domain_center: Tuple[float, float, float]
@@ -58,6 +91,8 @@ class SceneSpec:
     properties: Dict[str, object]
     box_spawns: List[Tuple[Tuple[float, float, float], Tuple[float, float, float]]] = field(default_factory=list)
     meshes: List[MeshNode] = field(default_factory=list)
+    force_fields: List[ForceFieldSpec] = field(default_factory=list)
+    queries: list = field(default_factory=list)
     other_nodes: List[dict] = field(default_factory=list)
     base_dir: str = "."
 
@@ -118,6 +153,14 @@ def parse(path_or_dict, base_dir=None) -> SceneSpec:
                                         scale=_vec(n, "scale", 3, [1, 1, 1]),
                                         boundary=bool(n.get("boundary", False)) and t == "ModelNode",
                                         fluid_spawn=(t == "ModelFluidSpawnNode")))
+        elif t == "ForceFieldNode":
+            spec.force_fields.append(ForceFieldSpec(name=n.get("name", ""), position=_vec(n, "position", 3, [0, 0, 0]),
+                                                    rotation_xyzw=_vec(n, "rotation", 4, [0, 0, 0, 1]),
+                                                    scale=_vec(n, "scale", 3, [0.25, 0.25, 0.25]),
+                                                    active=bool(n.get("active", True)), strength=float(n.get("strength", 5.0))))
+        elif t == "QueryNode":
+            from .query import QueryNode
+            spec.queries.append(QueryNode.from_json(n))
         else:
             spec.other_nodes.append(n)
     if res is None:
@@ -234,6 +277,28 @@ def apply(sim, spec: SceneSpec, loader=load_obj):
     return rho, bnd
 
 
+def apply_force_fields(sim, spec: SceneSpec, max_cells=None):
+    """Sets the forces field of a 3D simulator from the scene's force fields (cleared first).
+    Returns the cell count of each field; a field over max_cells (default: the scene's
+    "Max ForceField cells", else 1000) is skipped, as the reference skips it."""
+    if max_cells is None:
+        max_cells = int(spec.properties.get("Max ForceField cells", 1000))
+    W, H, D = spec.resolution
+    sim.grid().set("forces", np.zeros((D, H, W, 3), np.float32))
+    c = (ctypes.c_float * 3)(*spec.domain_center)
+    s = (ctypes.c_float * 3)(*spec.domain_size)
+    counts = []
+    for ff in spec.force_fields:
+        tris = ff.triangles()
+        f = ff.direction() * np.float32(ff.strength) if ff.active else np.zeros(3, np.float32)
+        n = ctypes.c_int32(0)
+        if ff.active:
+            check(sim.lib.efs_scene_force_region(sim.h, tris.ctypes.data, len(tris), c, s, (ctypes.c_float * 3)(*f.tolist()),
+                                                 int(max_cells), ctypes.byref(n)))
+        counts.append(int(n.value))
+    return counts
+
+
 def simulator_from_scene(path, loader=load_obj, **param_over):
     """A HipSimulator3D set up from a scene file, with the scene's parameters (overridable)."""
     from . import HipSimulator3D, Params
@@ -243,4 +308,6 @@ def simulator_from_scene(path, loader=load_obj, **param_over):
         setattr(p, k, v)
     sim = HipSimulator3D(*spec.resolution, params=p)
     apply(sim, spec, loader)
+    if spec.force_fields:
+        apply_force_fields(sim, spec)
     return sim, spec

@@ -178,6 +178,29 @@ int efs_scene_face_noise(efs_handle h, uint64_t seed, double sigma);
  * triangle), value_outside elsewhere and on the last layer of each axis. */
 int efs_scene_voxelise(efs_handle h, const float* triangles, int32_t ntris, const float domain_center[3],
                        const float domain_size[3], int32_t field, float value_inside, float value_outside);
+/* Reference: Editor::updateForceFields for one active ForceFieldNode (src/core/Editor.cpp:260-330,
+ * assets/shaders/forcefield_cells.comp): forces = force on the cells inside the field's mesh
+ * (the same inside test as efs_scene_voxelise; the node's force is direction * strength).  A
+ * field covering more than max_cells cells is skipped, as MAX_FORCEFIELD_CELLS does
+ * (max_cells <= 0: no limit); *ncells receives the count.  3D only.  Clear the forces first
+ * (efs_upload of zeros) when the reference would (a field moved). */
+int efs_scene_force_region(efs_handle h, const float* triangles, int32_t ntris, const float domain_center[3],
+                           const float domain_size[3], const float force[3], int32_t max_cells, int32_t* ncells);
+
+/* Reference: QueryNode::update's per-frame statistics (src/simulation/scene/nodes/QueryNode.cpp:
+ * 45-84) over the cells [max(0, lo), min(n - 1, hi)) of each axis (the node's position and
+ * position + scale, truncated), skipping rho < 0.5 when only_fluid: cell count, max / min / mean
+ * cell coordinates, mean face velocity (u, v, w stored at the cell) and mean pressure (sums in
+ * FP64).  With no cell, every value is 0. */
+typedef struct {
+    int32_t count;
+    float max_pos[3];
+    float min_pos[3];
+    float avg_pos[3];
+    float vel[3];
+    float pressure;
+} efs_query_result;
+int efs_query_box(efs_handle h, const int32_t lo[3], const int32_t hi[3], int32_t only_fluid, efs_query_result* out);
 
 /* ---- Learned preconditioner (no counterpart in the reference: it accelerates the solve that
  * CGSolverCPU3D::solve performs, CGSolver.cpp:416-465) ------------------------------------

@@ -39,7 +39,8 @@ def test_parse_synthetic_scene(tmp_path):
     assert spec.domain_size == (1.0625, 0.9375, 0.8125)
     assert spec.box_spawns == [((20.0, 1.0, 1.0), (12.0, 14.0, 24.0))]
     assert len(spec.meshes) == 1 and spec.meshes[0].boundary
-    assert [n["type"] for n in spec.other_nodes] == ["QueryNode"]
+    assert spec.other_nodes == [] and len(spec.queries) == 1
+    assert list(spec.queries[0].position) == [1.0, 1.0, 1.0] and list(spec.queries[0].scale) == [4.0, 4.0, 4.0]
     from eulerianfluidsimulation_amd import Params
     p = spec.params(Params)
     assert p.dt == pytest.approx(0.01)
