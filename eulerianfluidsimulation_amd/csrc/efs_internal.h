@@ -31,6 +31,9 @@
 #define RC_CNT_SHIFT 8
 #define RC_CG_ROW 0x8000u
 #define RC_JAC_ROW 0x4000u
+// the cell is inside the counted region (not on the outer layer, boundary <= 0.5) and fluid:
+// exactly the cells whose neighbour bit is set in their neighbours' codes
+#define RC_CONTRIB 0x40u
 
 // fixed-point accumulation of scattered float contributions: deterministic regardless of the
 // order the atomics land in.  2^36 gives 1.5e-11 resolution and a range of +-1.3e8.
@@ -321,6 +324,7 @@ struct CgPrecond {
     float* z;
     double* p_rz;
     int batch;
+    bool writes_rz = false;  // apply also writes the r . z partials (no separate k_cg_rz)
 };
 int efs_cg_run_pc(efs_sim* s, efs_stats* st, bool border, int max_it, const CgPrecond* pc);
 int efs_cg_setup_x0(efs_sim* s);                   // x64 = b, xd = 0
@@ -346,7 +350,7 @@ void efs_mg_free(efs_sim* s);
 int efs_learned_cg_solve(efs_sim* s, efs_stats* st);
 void efs_learned_free(efs_sim* s);
 
-// event-timed kernels: KT_OTHER times the fine-level multigrid smoother passes
+// event-timed kernels: KT_OTHER times the fine-level multigrid up sweep
 enum { KT_CG_DIR_SPMV = 0, KT_CG_UPDATE = 1, KT_JACOBI = 2, KT_OTHER = 3 };
 void efs_ktimer_begin(efs_sim* s, int k, hipEvent_t* a, hipEvent_t* b);
 void efs_ktimer_end(efs_sim* s, int k, hipEvent_t a, hipEvent_t b);

@@ -12,9 +12,14 @@
 //     coarse correction is twice as large and, measured on the sphere-over-pool scene, halves the
 //     iteration count at 256^3: 35 -> 16.)
 //   Smoother.  Red-black Gauss-Seidel, one sweep down (red, black) and one sweep up (black, red)
-//     so that M is symmetric.  The down sweep starts from x = 0, so both of its passes are one
-//     kernel (k_mg_presmooth); the first up pass reads the coarse correction of its neighbours on
-//     the fly (the prolongation is never stored).
+//     so that M is symmetric.  Each level costs two kernels (k_mg_sweep): the down sweep from
+//     x = 0 fused with the residual and its restriction to the next level, and the up sweep,
+//     whose black pass reads the coarse correction of its neighbours' parents on the fly (the
+//     prolongation is never stored).  x is never read: a red cell's value before the pass that
+//     updates it is always the down sweep's, recomputed from b and the row code.
+//   Row arithmetic.  A row's update is (sum - b / s) * (1 / n): s is a power of two, so b / s is
+//     the exact product b * (1 / s); 1 / n is the hardware reciprocal (within an ulp of the
+//     IEEE quotient; the same in every pass, so the V-cycle stays symmetric).
 //   Coarsest level (<= 8 cells per axis): 16 symmetric sweeps in one workgroup, in LDS.
 // Non-fluid rows of the fine level return z = 0 (their residual is 0: identity rows).
 #include <hip/hip_runtime.h>
@@ -38,6 +43,12 @@
 #ifndef EFS_MG_BATCH
 #define EFS_MG_BATCH 8
 #endif
+#ifndef EFS_MG_IEEE_DIV
+#define EFS_MG_IEEE_DIV 0
+#endif
+#ifndef EFS_MG_FUSED_RZ
+#define EFS_MG_FUSED_RZ 1
+#endif
 
 namespace {
 
@@ -58,6 +69,7 @@ struct MG {
     std::vector<Lvl> L;
     float* z = nullptr;
     double* p_rz = nullptr;
+    bool fused_rz = false;  // the fine up sweep writes the r . z partials
 };
 
 MG* get(efs_sim* s) { return (MG*)s->mg; }
@@ -116,30 +128,23 @@ __global__ __launch_bounds__(256) void k_mg_rowcode(Lvl c) {
         nb(x, y + 1, z, RC_NB_YP);
         nb(x, y, z + 1, RC_NB_ZP);
         nb(x, y, z - 1, RC_NB_ZM);
-        code |= (cnt << RC_CNT_SHIFT) | RC_CG_ROW;
+        code |= (cnt << RC_CNT_SHIFT) | RC_CG_ROW | RC_CONTRIB;
     }
     c.rc[ci] = (uint16_t)code;
 }
 
-// sum of the contributing neighbours of interior cell i (x values, plus the coarse correction
-// of each neighbour's parent when cl is given).  Every load is issued unconditionally (the six
-// neighbours of an interior cell are inside the array) and masked afterwards: loads under a
-// branch would serialise on their latency.
-__device__ inline float nb_sum(const Lvl& l, unsigned rc, int i, int x, int y, int z, const Lvl* cl) {
-    const int sy = l.W, sz = l.W * l.H;
-    float v[6] = {l.x[i - 1], l.x[i + 1], l.x[i - sy], l.x[i + sy], l.x[i + sz], l.x[i - sz]};
-    if (cl) {
-        const int px0 = x >> 1, px1 = (x + 2) >> 1, px = (x + 1) >> 1;
-        const int py0 = y >> 1, py1 = (y + 2) >> 1, py = (y + 1) >> 1;
-        const int pz0 = z >> 1, pz1 = (z + 2) >> 1, pz = (z + 1) >> 1;
-        v[0] += cl->x[lidx(*cl, px0, py, pz)];
-        v[1] += cl->x[lidx(*cl, px1, py, pz)];
-        v[2] += cl->x[lidx(*cl, px, py0, pz)];
-        v[3] += cl->x[lidx(*cl, px, py1, pz)];
-        v[4] += cl->x[lidx(*cl, px, py, pz1)];
-        v[5] += cl->x[lidx(*cl, px, py, pz0)];
-    }
-    // the reference order x-, x+, y-, y+, z+, z-
+// Gauss-Seidel row update: (sum - b / s) / n on fluid rows with neighbours, else 0.
+__device__ inline float row_update(unsigned rc, float sum, float b, float inv_s) {
+    const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
+    const bool row = (rc & RC_CG_ROW) && cnt > 0;
+#if EFS_MG_IEEE_DIV
+    return row ? (sum - b * inv_s) / (float)cnt : 0.f;
+#else
+    return row ? (sum - b * inv_s) * __builtin_amdgcn_rcpf((float)cnt) : 0.f;
+#endif
+}
+// the masked neighbour sum in the reference order x-, x+, y-, y+, z+, z-
+__device__ inline float masked_sum6(unsigned rc, const float (&v)[6]) {
     const unsigned bits[6] = {RC_NB_XM, RC_NB_XP, RC_NB_YM, RC_NB_YP, RC_NB_ZP, RC_NB_ZM};
     float sum = 0.f;
 #pragma unroll
@@ -147,93 +152,208 @@ __device__ inline float nb_sum(const Lvl& l, unsigned rc, int i, int x, int y, i
     return sum;
 }
 
-// One red-black Gauss-Seidel pass over the cells of `color` ((x + y + z) & 1).  mode 0: normal;
-// mode 2: neighbours read x + the coarse correction of their parent (the first pass after the
-// coarse-grid correction).
-__global__ __launch_bounds__(256) void k_mg_smooth(Lvl l, int color, int mode, Lvl cl, const CGState* st) {
+// One smoothing sweep of a level in one kernel, in two modes.
+//   SW_UP (and SW_UP_RZ): the up sweep.  The black pass, whose neighbours (all red) read their value
+//     plus the coarse correction of their parent, then the red pass from the new black values;
+//     writes x.  With SW_UP_RZ (level 0: x = z, b = r) the block also writes its partial of r . z.
+//   SW_DOWN: the down sweep from x = 0 (red, then black), then the residual b - s A x of every cell
+//     and its restriction to the coarse level: b_c = 1/8 of the sum of the fluid rows' residuals
+//     over the 8 children (in the child order x, y, z), 0 on non-fluid coarse cells.  x itself is
+//     not stored: nothing reads it before the up sweep overwrites it.
+// In both modes a red cell's value before the pass that updates it is the down sweep's
+// (0 - b / s) / n, recomputed from b and the row code, so x is never read.
+//
+// A block owns a TX x PTY column tile and marches up a chunk of planes.  Per plane it loads b, the
+// row codes (and for the up sweep the red cells' parent correction) once for the tile and a halo
+// of 2, and keeps in LDS three planes of red values (halo 2), three planes of black values
+// (halo 1) and four planes of b and row codes.  Each lane owns two x-adjacent cells, one of each
+// colour, so every phase does one row update per lane.  The LDS values are stored masked by the
+// cell's RC_CONTRIB bit, which is the neighbour bit of every row that reads them, so the neighbour
+// sums need no masks (the same additions in the same order as masked sums).
+#ifndef EFS_MG_PTY
+#define EFS_MG_PTY 8
+#endif
+constexpr int PTY = EFS_MG_PTY;
+constexpr int PX = TX + 4, PY = PTY + 4, PP = PX * PY;  // a plane of the tile with its halo of 2
+constexpr int POST_NT = TX * PTY;
+constexpr int NF = (PX / 2) * PY, NB = ((PX - 2) / 2) * (PY - 2), NT2 = (TX / 2) * PTY;  // pair lanes per phase
+constexpr int NC = (TX / 2) * (PTY / 2);  // coarse cells of a tile plane pair
+static_assert(NF <= POST_NT, "one fetch pair per lane");
+static_assert(PTY % 2 == 0, "tiles cover whole coarse cells");
+enum { SW_UP = 0, SW_UP_RZ = 1, SW_DOWN = 2 };
+template <int MODE>
+__global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s, Dom d, const CGState* st, double* p_rz) {
+    constexpr bool UP = MODE != SW_DOWN, RZ = MODE == SW_UP_RZ;
     if (st->done) return;
-    const int y = 1 + blockIdx.y * MBY + threadIdx.y, z = 1 + blockIdx.z;
-    if (y > l.ny) return;
-    // thread k owns the pair of cells (1 + 2k, 2 + 2k) of its row: one of each colour
-    const int xa = 1 + 2 * (int)(blockIdx.x * MB + threadIdx.x);
-    if (xa > l.nx) return;
-    const int x = (((xa + y + z) & 1) == color) ? xa : xa + 1;
-    if (x > l.nx) return;
-    const int i = lidx(l, x, y, z);
-    const unsigned rc = l.rc[i];
-    const float bi = l.b[i];
-    const float sum = nb_sum(l, rc, i, x, y, z, mode == 2 ? &cl : nullptr);
-    const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
-    l.x[i] = ((rc & RC_CG_ROW) && cnt > 0) ? (sum - bi / l.s) / (float)cnt : 0.f;
-}
+    __shared__ float ro[3][PP], bn[3][PP], bs[4][PP];
+    __shared__ uint16_t rs[4][PP];
+    __shared__ float res[UP ? 1 : 4][UP ? 1 : TX * PTY];  // residuals of the tile (down sweep)
+    __shared__ double red[POST_NT / 64];
+    const int lane = threadIdx.x + threadIdx.y * TX;
+    double acc = 0.0;
+    int bx, by, bz;
+    if (block_coords(d, bx, by, bz)) {
+        const int x0 = 1 + bx * TX, y0 = 1 + by * PTY;
+        const int zb = 1 + bz * d.zc, ze = min(zb + d.zc, 1 + l.nz);  // planes [zb, ze)
+        const int plane = l.W * l.H, cplane = cl.W * cl.H;
+        // fetch lanes: pair (hx, hx + 1), hx = 2 f, in row hy of the halo-2 plane
+        const bool fl = lane < NF;
+        const int fj = lane % (PX / 2), fhy = min(lane / (PX / 2), PY - 1);
+        const int fk = fhy * PX + 2 * fj;
+        const int fxu = x0 - 2 + 2 * fj, fyu = y0 - 2 + fhy;  // unclamped (colour)
+        const int fx0 = min(max(fxu, 0), l.W - 1), fx1 = min(max(fxu + 1, 0), l.W - 1), fy = min(max(fyu, 0), l.H - 1);
+        const int fo0 = fy * l.W + fx0, fo1 = fy * l.W + fx1;
+        const int fc0 = ((fy + 1) >> 1) * cl.W + ((fx0 + 1) >> 1), fc1 = ((fy + 1) >> 1) * cl.W + ((fx1 + 1) >> 1);
+        // black lanes: pair (hx, hx + 1), hx = 1 + 2 j, in row hy = 1 .. PY - 2
+        const bool blk = lane < NB;
+        const int bj = lane % ((PX - 2) / 2), bhy = 1 + min(lane / ((PX - 2) / 2), PY - 3);
+        const int bk = bhy * PX + 1 + 2 * bj;
+        const int bxu = x0 - 1 + 2 * bj, byu = y0 - 2 + bhy;
+        const bool brow_tile = bhy >= 2 && bhy < 2 + PTY;
+        // finish lanes: pair (hx, hx + 1), hx = 2 + 2 j, in row hy = 2 .. PY - 3
+        const bool fin = lane < NT2;
+        const int tj = lane % (TX / 2), thy = 2 + min(lane / (TX / 2), PTY - 1);
+        const int tk = thy * PX + 2 + 2 * tj;
+        const int txu = x0 + 2 * tj, tyu = y0 - 2 + thy;
 
-// The first down sweep from x = 0 (red pass, then black pass) in one kernel: a red cell gets
-// -b / (s n); a black cell gets (sum of its fluid red neighbours' -b_j / (s n_j) - b / s) / n,
-// the values the two passes would compute, in the same operations.
-__global__ __launch_bounds__(256) void k_mg_presmooth(Lvl l, const CGState* st) {
-    if (st->done) return;
-    const int y = 1 + blockIdx.y * MBY + threadIdx.y, z = 1 + blockIdx.z;
-    if (y > l.ny) return;
-    const int xa = 1 + 2 * (int)(blockIdx.x * MB + threadIdx.x);
-    if (xa > l.nx) return;
-    const int sy = l.W, sz = l.W * l.H;
-    auto red_value = [&](unsigned rc, float b) -> float {
-        const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
-        return ((rc & RC_CG_ROW) && cnt > 0) ? (0.f - b / l.s) / (float)cnt : 0.f;
-    };
+        struct Fetched {
+            unsigned rc0, rc1;
+            float b0, b1, e;
+        };
+        auto fetch = [&](int p, Fetched& f) {  // loads of plane p (clamped into the array)
+            const int pz = min(max(p, 0), l.D - 1);
+            const int zo = pz * plane;
+            f.rc0 = l.rc[zo + fo0];
+            f.rc1 = l.rc[zo + fo1];
+            f.b0 = l.b[zo + fo0];
+            f.b1 = l.b[zo + fo1];
+            if (UP) {
+                const bool first_red = ((fxu + fyu + p) & 1) == 0;
+                f.e = cl.x[((pz + 1) >> 1) * cplane + (first_red ? fc0 : fc1)];
+            }
+        };
+        auto commit = [&](int p, const Fetched& f) {  // into the ring slots of plane p
+            if (!fl) return;
+            const bool first_red = ((fxu + fyu + p) & 1) == 0;
+            const unsigned rc = first_red ? f.rc0 : f.rc1;
+            float v = row_update(rc, 0.f, first_red ? f.b0 : f.b1, inv_s);
+            if (UP) v = v + f.e;
+            ro[(p + 3) % 3][fk + (first_red ? 0 : 1)] = (rc & RC_CONTRIB) ? v : 0.f;
+            bs[(p + 4) & 3][fk] = f.b0;
+            bs[(p + 4) & 3][fk + 1] = f.b1;
+            rs[(p + 4) & 3][fk] = (uint16_t)f.rc0;
+            rs[(p + 4) & 3][fk + 1] = (uint16_t)f.rc1;
+        };
+        // residual of a fluid row after the down sweep (b - s (sum - n x)), else 0
+        auto residual = [&](unsigned rc, float sum, float x, float b) -> float {
+            const float ax = l.s * (sum - (float)((rc >> RC_CNT_SHIFT) & 7u) * x);
+            return (rc & RC_CG_ROW) ? b - ax : 0.f;
+        };
+        auto black = [&](int p) {  // black value of the lane's pair in plane p
+            if (!blk) return;
+            const int q = ((bxu + byu + p) & 1) == 1 ? 0 : 1;
+            const int k = bk + q, x = bxu + q;
+            const float *r0 = ro[(p + 2) % 3], *r1 = ro[(p + 3) % 3], *r2 = ro[(p + 4) % 3];
+            const unsigned rc = rs[(p + 4) & 3][k];
+            const float b = bs[(p + 4) & 3][k];
+            const float sum = 0.f + r1[k - 1] + r1[k + 1] + r1[k - PX] + r1[k + PX] + r2[k] + r0[k];
+            const bool inside = x >= 1 && x <= l.nx && byu >= 1 && byu <= l.ny && p >= 1 && p <= l.nz;
+            const float val = inside ? row_update(rc, sum, b, inv_s) : 0.f;
+            bn[(p + 3) % 3][k] = (rc & RC_CONTRIB) ? val : 0.f;
+            const bool tile = brow_tile && x >= x0 && x < x0 + TX && p >= zb && p < ze;
+            if (UP) {
+                if (tile && inside) {
+                    l.x[p * plane + byu * l.W + x] = val;
+                    if (RZ) acc += (double)b * (double)val;
+                }
+            } else if (tile) {
+                res[p & 3][(byu - y0) * TX + (x - x0)] = inside ? residual(rc, sum, val, b) : 0.f;
+            }
+        };
+        auto finish = [&](int q) {  // red value (up) or red residual (down) of the lane's pair in plane q
+            if (!fin) return;
+            const int h = ((txu + tyu + q) & 1) == 0 ? 0 : 1;
+            const int k = tk + h, x = txu + h;
+            const bool inside = x <= l.nx && tyu <= l.ny;
+            const float *n0 = bn[(q + 2) % 3], *n1 = bn[(q + 3) % 3], *n2 = bn[(q + 4) % 3];
+            const unsigned rc = rs[(q + 4) & 3][k];
+            const float b = bs[(q + 4) & 3][k];
+            const float sum = 0.f + n1[k - 1] + n1[k + 1] + n1[k - PX] + n1[k + PX] + n2[k] + n0[k];
+            if (UP) {
+                if (!inside) return;
+                const float val = row_update(rc, sum, b, inv_s);
+                l.x[q * plane + tyu * l.W + x] = val;
+                if (RZ) acc += (double)b * (double)val;
+            } else {
+                res[q & 3][(tyu - y0) * TX + (x - x0)] = inside ? residual(rc, sum, row_update(rc, 0.f, b, inv_s), b) : 0.f;
+            }
+        };
+        // coarse right-hand sides of the pair of planes (q0, q0 + 1), q0 - zb even
+        auto restrict_pair = [&](int q0) {
+            if (lane >= NC) return;
+            const int cx = lane % (TX / 2), cy = lane / (TX / 2);
+            const int X = (x0 + 1) / 2 + cx, Y = (y0 + 1) / 2 + cy, Z = (q0 + 1) / 2;
+            if (X > cl.nx || Y > cl.ny || Z > cl.nz) return;
+            const int ci = lidx(cl, X, Y, Z);
+            float a = 0.f;
+            if (cl.cls[ci] == CLS_FLUID) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const int x = xa + h;
-        if (x > l.nx) break;
-        const int i = lidx(l, x, y, z);
-        const unsigned rc = l.rc[i];
-        const float bi = l.b[i];
-        if (((x + y + z) & 1) == 0) {
-            l.x[i] = red_value(rc, bi);
-        } else {
-            const int nb[6] = {i - 1, i + 1, i - sy, i + sy, i + sz, i - sz};
-            const unsigned bits[6] = {RC_NB_XM, RC_NB_XP, RC_NB_YM, RC_NB_YP, RC_NB_ZP, RC_NB_ZM};
-            float v[6];
-#pragma unroll
-            for (int k = 0; k < 6; k++) v[k] = red_value(l.rc[nb[k]], l.b[nb[k]]);
-            float sum = 0.f;
-#pragma unroll
-            for (int k = 0; k < 6; k++) sum += (rc & bits[k]) ? v[k] : 0.f;
-            const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
-            l.x[i] = ((rc & RC_CG_ROW) && cnt > 0) ? (sum - bi / l.s) / (float)cnt : 0.f;
+                for (int k = 0; k < 8; k++) {
+                    const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+                    if (2 * cx + dx + x0 > l.nx || 2 * cy + dy + y0 > l.ny || q0 + dz > l.nz) continue;
+                    a += res[(q0 + dz) & 3][(2 * cy + dy) * TX + 2 * cx + dx];
+                }
+                a *= 0.125f;
+            }
+            cl.b[ci] = a;
+        };
+        // Planes p + 2 and p + 3 are in flight while plane p is processed (two register sets,
+        // alternating with the plane).  The barriers order LDS accesses only, so they do not wait
+        // for those loads (a full __syncthreads would drain them).
+        Fetched fa, fb;
+        auto body = [&](int p, Fetched& f) {
+            commit(p + 1, f);  // ring slots of planes p - 2 (red) and p - 3 (b, codes): no longer read
+            if (p + 3 <= ze + 1) fetch(p + 3, f);
+            lds_barrier();
+            black(p);
+            lds_barrier();
+            const int q = p - 1;
+            if (q >= zb) {
+                finish(q);
+                if (!UP && (((q - zb) & 1) == 1 || q == ze - 1)) {
+                    lds_barrier();
+                    restrict_pair(q - ((q - zb) & 1));
+                }
+            }
+        };
+        fetch(zb - 2, fa);
+        commit(zb - 2, fa);
+        fetch(zb - 1, fb);
+        commit(zb - 1, fb);
+        fetch(zb, fa);
+        fetch(zb + 1, fb);
+        for (int p = zb - 1; p <= ze; p += 2) {
+            body(p, fa);
+            if (p + 1 <= ze) body(p + 1, fb);
         }
     }
-}
-
-// residual of level f restricted to level c: b_c(I) = 1/8 sum over the fluid children of
-// (b - s A x); zero on non-fluid coarse cells.
-__global__ __launch_bounds__(256) void k_mg_restrict(Lvl f, Lvl c, const CGState* st) {
-    if (st->done) return;
-    const int x = 1 + blockIdx.x * MB + threadIdx.x, y = 1 + blockIdx.y * MBY + threadIdx.y, z = 1 + blockIdx.z;
-    if (x > c.nx || y > c.ny) return;
-    const int ci = lidx(c, x, y, z);
-    float acc = 0.f;
-    if (c.cls[ci] == CLS_FLUID) {
+    if (RZ) {  // every launched block writes its partial (0 when idle)
+        acc = wave_sum(acc);
+        if ((lane & 63) == 0) red[lane >> 6] = acc;
+        __syncthreads();
+        if (lane == 0) {
+            double tot = 0.0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int fx = 2 * x - 1 + (k & 1), fy = 2 * y - 1 + ((k >> 1) & 1), fz = 2 * z - 1 + (k >> 2);
-            if (fx > f.nx || fy > f.ny || fz > f.nz) continue;
-            const int fi = lidx(f, fx, fy, fz);
-            const unsigned rc = f.rc[fi];
-            const float sum = nb_sum(f, rc, fi, fx, fy, fz, nullptr);
-            const float ax = f.s * (sum - (float)((rc >> RC_CNT_SHIFT) & 7u) * f.x[fi]);
-            const float res = f.b[fi] - ax;
-            acc += (rc & RC_CG_ROW) ? res : 0.f;
+            for (int w = 0; w < POST_NT / 64; w++) tot += red[w];
+            p_rz[blockIdx.x] = tot;
         }
-        acc *= 0.125f;
     }
-    c.b[ci] = acc;
 }
 
 // coarsest level: symmetric red-black sweeps from x = 0 in one workgroup, on a copy of the level
 // (values, right-hand side, row codes, with the border) held in LDS
 constexpr int MG_CO_MAX = 1728;  // (10 + 2)^3: interiors up to 10 cells per axis
-__global__ __launch_bounds__(MG_CO_NT) void k_mg_coarsest(Lvl l, int sweeps, const CGState* st) {
+__global__ __launch_bounds__(MG_CO_NT) void k_mg_coarsest(Lvl l, float inv_s, int sweeps, const CGState* st) {
     if (st->done) return;
     __shared__ float xs[MG_CO_MAX], bs[MG_CO_MAX];
     __shared__ uint16_t rs[MG_CO_MAX];
@@ -256,12 +376,7 @@ __global__ __launch_bounds__(MG_CO_NT) void k_mg_coarsest(Lvl l, int sweeps, con
                 const int i = lidx(l, x, y, z);
                 const unsigned rc = rs[i];
                 const float v[6] = {xs[i - 1], xs[i + 1], xs[i - sy], xs[i + sy], xs[i + sz], xs[i - sz]};
-                const unsigned bits[6] = {RC_NB_XM, RC_NB_XP, RC_NB_YM, RC_NB_YP, RC_NB_ZP, RC_NB_ZM};
-                float sum = 0.f;
-#pragma unroll
-                for (int k = 0; k < 6; k++) sum += (rc & bits[k]) ? v[k] : 0.f;
-                const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
-                xs[i] = ((rc & RC_CG_ROW) && cnt > 0) ? (sum - bs[i] / l.s) / (float)cnt : 0.f;
+                xs[i] = row_update(rc, masked_sum6(rc, v), bs[i], inv_s);
             }
             __syncthreads();
         }
@@ -270,7 +385,6 @@ __global__ __launch_bounds__(MG_CO_NT) void k_mg_coarsest(Lvl l, int sweeps, con
 }
 
 inline dim3 cell_grid(const Lvl& l) { return dim3((l.W + MB - 1) / MB, (l.H + MBY - 1) / MBY, l.D); }
-inline dim3 color_grid(const Lvl& l) { return dim3(((l.nx + 1) / 2 + MB - 1) / MB, (l.ny + MBY - 1) / MBY, l.nz); }
 
 void free_mg(MG* m) {
     if (!m) return;
@@ -330,6 +444,8 @@ int ensure(efs_sim* s) {
     return 0;
 }
 
+Dom post_dom(const Lvl& l);
+
 // labels and row codes of every coarse level for the current fluid configuration
 int setup_levels(efs_sim* s) {
     MG* m = get(s);
@@ -340,34 +456,61 @@ int setup_levels(efs_sim* s) {
         hipLaunchKernelGGL(k_mg_rowcode, cell_grid(c), dim3(MB, MBY), 0, s->stream, c);
     }
     EFS_HIP(hipGetLastError());
+    // The CG kernels sum the r . z partials of its own launch (nbt blocks).  The fine up sweep can
+    // write them when its launch is no larger; the slots past it stay 0.
+    const int nbt = launch_blocks(dom_tiled(s, false));
+    const int nbp = launch_blocks(post_dom(m->L[0]));
+    m->fused_rz = EFS_MG_FUSED_RZ && m->L.size() >= 2 && nbp <= nbt && nbt <= s->max_blocks;
+    EFS_HIP(hipMemsetAsync(m->p_rz, 0, (size_t)s->max_blocks * sizeof(double), s->stream));
     return 0;
 }
 
-// z = M r: one V-cycle
+// launch domain of k_mg_sweep on level l: TX x PTY tiles, chunks of 8..32 planes (even, so that
+// no coarse cell straddles two chunks)
+Dom post_dom(const Lvl& l) {
+    Dom d{};
+    d.W = l.W; d.H = l.H; d.D = l.D;
+    d.x0 = d.y0 = d.z0 = 1;
+    d.nx = l.nx; d.ny = l.ny; d.nz = l.nz;
+    d.ty = PTY;
+    d.tiles_x = (l.nx + TX - 1) / TX;
+    d.tiles_y = (l.ny + PTY - 1) / PTY;
+    const int txy = d.tiles_x * d.tiles_y;
+    d.zc = std::min(l.nz, std::max(8, std::min(32, l.nz * txy / 1024)) & ~1);
+    d.chunks_z = (l.nz + d.zc - 1) / d.zc;
+    d.nblocks = txy * d.chunks_z;
+    return d;
+}
+
+// z = M r: one V-cycle.  With fused_rz the fine up sweep also writes the r . z partials.
 int vcycle(efs_sim* s, void*) {
     MG* m = get(s);
     const int nl = (int)m->L.size();
-    const dim3 blk(MB, MBY);
     const CGState* st = s->st;
-    // the fine-level passes of the up sweep are event-timed when kernel timing is on
-    auto smooth = [&](int k, int color, int mode, const Lvl& cl) {
-        const Lvl& l = m->L[k];
-        hipEvent_t a = nullptr, b = nullptr;
-        if (k == 0) efs_ktimer_begin(s, KT_OTHER, &a, &b);
-        hipLaunchKernelGGL(k_mg_smooth, color_grid(l), blk, 0, s->stream, l, color, mode, cl, st);
-        if (k == 0) efs_ktimer_end(s, KT_OTHER, a, b);
-    };
+    const dim3 pb(TX, PTY);
     for (int k = 0; k < nl - 1; k++) {
         const Lvl& l = m->L[k];
-        hipLaunchKernelGGL(k_mg_presmooth, color_grid(l), blk, 0, s->stream, l, st);
-        const Lvl& c = m->L[k + 1];
-        hipLaunchKernelGGL(k_mg_restrict, dim3((c.nx + MB - 1) / MB, (c.ny + MBY - 1) / MBY, c.nz), blk, 0, s->stream, l, c, st);
+        const Dom d = post_dom(l);
+        hipLaunchKernelGGL(k_mg_sweep<SW_DOWN>, dim3(launch_blocks(d)), pb, 0, s->stream, l, m->L[k + 1], 1.f / l.s, d, st,
+                           nullptr);
     }
-    hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(MG_CO_NT), 0, s->stream, m->L[nl - 1], EFS_MG_COARSE_SWEEPS, st);
+    hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(MG_CO_NT), 0, s->stream, m->L[nl - 1], 1.f / m->L[nl - 1].s,
+                       EFS_MG_COARSE_SWEEPS, st);
     for (int k = nl - 2; k >= 0; k--) {
         const Lvl& l = m->L[k];
-        smooth(k, 1, 2, m->L[k + 1]);
-        smooth(k, 0, 0, l);
+        const Dom d = post_dom(l);
+        const dim3 grid(launch_blocks(d));
+        if (k == 0) {  // the fine up sweep is event-timed when kernel timing is on
+            hipEvent_t a = nullptr, b = nullptr;
+            efs_ktimer_begin(s, KT_OTHER, &a, &b);
+            if (m->fused_rz)
+                hipLaunchKernelGGL(k_mg_sweep<SW_UP_RZ>, grid, pb, 0, s->stream, l, m->L[1], 1.f / l.s, d, st, m->p_rz);
+            else
+                hipLaunchKernelGGL(k_mg_sweep<SW_UP>, grid, pb, 0, s->stream, l, m->L[1], 1.f / l.s, d, st, nullptr);
+            efs_ktimer_end(s, KT_OTHER, a, b);
+        } else {
+            hipLaunchKernelGGL(k_mg_sweep<SW_UP>, grid, pb, 0, s->stream, l, m->L[k + 1], 1.f / l.s, d, st, nullptr);
+        }
     }
     EFS_HIP(hipGetLastError());
     return 0;
@@ -396,6 +539,7 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
     pc.z = m->z;
     pc.p_rz = m->p_rz;
     pc.batch = EFS_MG_BATCH;
+    pc.writes_rz = m->fused_rz;
     // The recurrence residual stops the iteration; with the true residual enabled and the relative
     // criterion, a solve whose FP64 true residual is still above the tolerance restarts from its
     // iterate (at most twice).
@@ -405,8 +549,8 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
     for (int attempt = 0; attempt < 3; attempt++) {
         const size_t kt0 = s->ktimers[KT_OTHER].start.size();
         rc = efs_cg_run_pc(s, o, false, std::max(1, s->P.max_steps - total), &pc);
-        // launches after convergence return at once: keep the 2 timed fine passes of each real V-cycle
-        efs_trim_timer(s->ktimers[KT_OTHER], kt0 + 2 * ((size_t)s->st_host->iter + 1));
+        // launches after convergence return at once: keep the timed fine up sweep of each real V-cycle
+        efs_trim_timer(s->ktimers[KT_OTHER], kt0 + ((size_t)s->st_host->iter + 1));
         total += o->iterations;
         const bool verify = s->P.true_residual && s->P.criterion == EFS_CRIT_RELATIVE;
         if (rc != 0 || !verify || o->true_rel_residual <= (double)s->P.solver_threshold) break;

@@ -76,6 +76,7 @@ __global__ __launch_bounds__(256) void k_rowcode(DevGrid g, uint16_t* rc, CGStat
         if (counted && rq[k] > 0.5f) code |= bits[k];
     }
     code |= cnt << RC_CNT_SHIFT;
+    if (!(ox || oy || oz) && !(b & BF_GT05) && r > 0.5f) code |= RC_CONTRIB;
     if (!(r <= 0.5f || (b & BF_EQ1))) code |= RC_CG_ROW;
     if (!(r <= 0.5f || (b & BF_GE05))) code |= RC_JAC_ROW;
     rc[c] = (uint16_t)code;
@@ -777,7 +778,7 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
     if (pc) {
         // z_0 = M r_0, partial r_0 . z_0
         if ((rc = pc->apply(s, pc->ctx))) return rc;
-        hipLaunchKernelGGL(k_cg_rz, dim3(nbt), bt, 0, s->stream, dt, s->cg_r, pc->z, s->st, pc->p_rz);
+        if (!pc->writes_rz) hipLaunchKernelGGL(k_cg_rz, dim3(nbt), bt, 0, s->stream, dt, s->cg_r, pc->z, s->st, pc->p_rz);
         dsrc = pc->z;
         p_rz = pc->p_rz;
     }
@@ -810,7 +811,8 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
             }
             if (pc) {
                 if ((rc = pc->apply(s, pc->ctx))) return rc;
-                hipLaunchKernelGGL(k_cg_rz, dim3(nbt), bt, 0, s->stream, dt, s->cg_r, pc->z, s->st, pc->p_rz);
+                if (!pc->writes_rz)
+                    hipLaunchKernelGGL(k_cg_rz, dim3(nbt), bt, 0, s->stream, dt, s->cg_r, pc->z, s->st, pc->p_rz);
             }
         }
         launched += n;

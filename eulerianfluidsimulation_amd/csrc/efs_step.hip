@@ -388,15 +388,16 @@ __device__ inline void diffuse_pair(float& rj, float& gj, unsigned bj, float& ri
     if (gj < 1e-5f || gi < 1e-5f) return;
     if (gj == gi) return;
     const float avg = (gj + gi) / 2.f;
-    if (gj > gi) {
-        const float dif = rj * (gj - gi) / (2.f * gj);
-        rj -= dif;
-        ri += dif;
-    } else {
-        const float dif = ri * (gi - gj) / (2.f * gi);
-        ri -= dif;
-        rj += dif;
-    }
+    // mass moves from the cell of larger gamma (s) to the other (t); written on values, since
+    // branches that update rj / ri through their references make the compiler select between
+    // the two addresses and keep both in scratch memory
+    const bool jg = gj > gi;
+    const float rs = jg ? rj : ri, gs = jg ? gj : gi, gt = jg ? gi : gj;
+    const float dif = rs * (gs - gt) / (2.f * gs);
+    const float nj = jg ? rj - dif : rj + dif;
+    const float ni = jg ? ri + dif : ri - dif;
+    rj = nj;
+    ri = ni;
     gj = avg;
     gi = avg;
 }
@@ -405,7 +406,10 @@ __device__ inline void diffuse_pair(float& rj, float& gj, unsigned bj, float& ri
 // line, consecutive threads on consecutive x (coalesced).  The loads of a line do not depend on
 // the sweep (cell i is read before anything writes it), so they are issued DF cells ahead in a
 // register ring indexed by compile-time constants.
-constexpr int DF = 8;
+#ifndef EFS_DF
+#define EFS_DF 8
+#endif
+constexpr int DF = EFS_DF;
 template <int AXIS>
 __global__ __launch_bounds__(64) void k_diffuse(DevGrid g) {
     const int a = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // x
