@@ -30,9 +30,10 @@ HBM_PEAK_GBS = 8000.0
 K1_BYTES = 2 + 4 + 4 + 4      # row code, r, d_{i-1} read; d_i written
 K2_BYTES = 2 + 4 + 4 + 4 + 4 + 4  # row code, d_i, x_i, r_i read; x_{i+1}, r_{i+1} written
 K2_BYTES_PC = 2 + 4 + 8 + 4 + 8 + 4  # preconditioned CG: the FP64 iterate is read and written
-# one red-black pass of the fine-level smoother: own-colour row code, right-hand side and the
-# written value (half the cells), the other colour's values (half the cells)
-MG_SMOOTH_BYTES = 0.5 * (2 + 4 + 4) + 0.5 * 4
+# the fused fine-level up sweep of the multigrid V-cycle (black then red pass in one kernel): row
+# code and right-hand side read, the red cells' parent correction read from the coarse level
+# (1/8 of the cells), z written
+MG_UP_BYTES = 2 + 4 + 4 / 8 + 4
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_sphere_pool_256_mg.md")
@@ -156,7 +157,7 @@ def main():
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     kt = {}
-    for k, name in ((_abi.KT_CG_DIR_SPMV, "cg_dir_spmv"), (_abi.KT_CG_UPDATE, "cg_update"), (_abi.KT_OTHER, "mg_smooth_fine")):
+    for k, name in ((_abi.KT_CG_DIR_SPMV, "cg_dir_spmv"), (_abi.KT_CG_UPDATE, "cg_update"), (_abi.KT_OTHER, "mg_up_fine")):
         n = ctypes.c_int64()
         ms = ctypes.c_double()
         lib.efs_kernel_time(sim.h, k, ctypes.byref(n), ctypes.byref(ms))
@@ -187,7 +188,7 @@ def main():
     real = max(1, sum(iters))
     avg_ms = total_ms / max(1, launches)
     bytes_per_launch = {"cg_dir_spmv": K1_BYTES, "cg_update": K2_BYTES_PC if ARGS.precond != "none" else K2_BYTES,
-                        "mg_smooth_fine": MG_SMOOTH_BYTES}[dom] * interior
+                        "mg_up_fine": MG_UP_BYTES}[dom] * interior
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     steps_per_s = ARGS.steps / elapsed
     its = float(np.mean(iters)) if iters else 0.0

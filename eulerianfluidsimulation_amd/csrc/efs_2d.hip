@@ -62,18 +62,20 @@ __device__ inline float sinterp2(const DevGrid& g, const float* __restrict__ f, 
     const float v2 = f[iy1 * g.W + ix2], v3 = f[iy2 * g.W + ix2];
     const float x1 = (float)ix1 + 0.5f, x2 = (float)ix2 + 0.5f;
     const float y1 = (float)iy1 + 0.5f, y2 = (float)iy2 + 0.5f;
+    // the clamped indices of an axis differ by 0 or 1 and i + 0.5f is exact, so each spacing is
+    // exactly 0 or 1: the reference's divisions by it are divisions by 1.0f and are left out
     const float dx2x1 = x2 - x1;
     float fxy1, fxy2;
     if (dx2x1 < 1e-9f) {
         fxy1 = v1;
         fxy2 = v3;
     } else {
-        fxy1 = ((x2 - px) / dx2x1) * v0 + ((px - x1) / dx2x1) * v2;
-        fxy2 = ((x2 - px) / dx2x1) * v1 + ((px - x1) / dx2x1) * v3;
+        fxy1 = (x2 - px) * v0 + (px - x1) * v2;
+        fxy2 = (x2 - px) * v1 + (px - x1) * v3;
     }
     const float dy2y1 = y2 - y1;
     if (dy2y1 < 1e-9f) return fxy1;
-    return ((y2 - py) / dy2y1) * fxy1 + ((py - y1) / dy2y1) * fxy2;
+    return (y2 - py) * fxy1 + (py - y1) * fxy2;
 }
 
 // VecField2D::bilinear_interpolation / interpolate (VecField2D.cpp:21-64).  Low indices are

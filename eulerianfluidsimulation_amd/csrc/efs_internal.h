@@ -114,12 +114,15 @@ __device__ inline float sinterp(const DevGrid& g, const float* __restrict__ f, f
     const float x1 = (float)ix1 + 0.5f, x2 = (float)ix2 + 0.5f;
     const float y1 = (float)iy1 + 0.5f, y2 = (float)iy2 + 0.5f;
     const float z1 = (float)iz1 + 0.5f, z2 = (float)iz2 + 0.5f;
+    // The clamped indices of an axis differ by 0 or 1 and i + 0.5f is exact, so each spacing
+    // (x2 - x1, ...) is exactly 0 or 1: the reference's divisions by it are divisions by 1.0f,
+    // the identity, and are left out.
     const float dz2z1 = z2 - z1;
     float f00, f01, f10, f11;
     if (dz2z1 < 1e-9f) {
         f00 = v1; f01 = v3; f10 = v5; f11 = v7;
     } else {
-        const float wz1 = (z2 - pz) / dz2z1, wz2 = (pz - z1) / dz2z1;
+        const float wz1 = z2 - pz, wz2 = pz - z1;
         f00 = wz1 * v0 + wz2 * v1;
         f01 = wz1 * v2 + wz2 * v3;
         f10 = wz1 * v4 + wz2 * v5;
@@ -130,13 +133,13 @@ __device__ inline float sinterp(const DevGrid& g, const float* __restrict__ f, f
     if (dy2y1 < 1e-9f) {
         fx1 = f01; fx2 = f11;
     } else {
-        const float wy1 = (y2 - py) / dy2y1, wy2 = (py - y1) / dy2y1;
+        const float wy1 = y2 - py, wy2 = py - y1;
         fx1 = wy1 * f00 + wy2 * f01;
         fx2 = wy1 * f10 + wy2 * f11;
     }
     const float dx2x1 = x2 - x1;
     if (dx2x1 < 1e-9f) return fx2;
-    const float wx1 = (x2 - px) / dx2x1, wx2 = (px - x1) / dx2x1;
+    const float wx1 = x2 - px, wx2 = px - x1;
     return wx1 * fx1 + wx2 * fx2;
 }
 

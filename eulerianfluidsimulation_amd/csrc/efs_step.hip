@@ -274,49 +274,57 @@ __global__ __launch_bounds__(256) void k_adv_fwd(DevGrid g, float dt, const unsi
     }
 }
 
-// Step 6 (all cells): gamma_i = own backward row + scattered forward entries.
-__global__ __launch_bounds__(256) void k_adv_rowsum(DevGrid g, const int* __restrict__ bp0, const float* __restrict__ bw,
-                                                    unsigned long long* __restrict__ acc, float* __restrict__ out) {
+// The backward entries are stored once, as step 4 leaves them; their row and column scalings
+// (steps 7 and 9) are applied where they are consumed, in the same order of float divisions, so
+// the values are those of scaling in place.  The forward entries, which few cells have, are
+// scaled in place.
+
+// Steps 6-8 for the backward entries (all cells): gamma_i = own backward row + scattered forward
+// entries; the row's entries divided by it add to the new column sums (beta).
+__global__ __launch_bounds__(256) void k_adv_rows(DevGrid g, const int* __restrict__ bp0, const float* __restrict__ bw,
+                                                  unsigned long long* __restrict__ gamma_acc, float* __restrict__ g6,
+                                                  unsigned long long* beta_acc) {
     FULL_XYZ(g);
-    long long s = (long long)acc[c];
-    acc[c] = 0ull;
-    if (bp0[c] >= 0) {
+    long long s = (long long)gamma_acc[c];
+    gamma_acc[c] = 0ull;
+    const int p0 = bp0[c];
+    float a[8];
+    if (p0 >= 0) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) s += to_fix(bw[(size_t)k * g.N + c]);
+        for (int k = 0; k < 8; k++) {
+            a[k] = bw[(size_t)k * g.N + c];
+            s += to_fix(a[k]);
+        }
     }
-    out[c] = (float)((double)s * FIX_INV);
+    const float gi = (float)((double)s * FIX_INV);
+    g6[c] = gi;
+    if (p0 >= 0) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float v = a[k];
+            if (gi > 1.e-5f) v /= gi;
+            fix_add(beta_acc, p0 + cornerOffset(g, k), v);
+        }
+    }
 }
 
-// Steps 7-8: divide by gamma (rows), scatter backward entries and add own forward entries into
-// the column sums (beta).
-__global__ __launch_bounds__(256) void k_adv_scale_rows(DevGrid g, const float* __restrict__ g6, const int* __restrict__ bp0,
-                                                        float* __restrict__ bw, const int* __restrict__ fq0,
-                                                        float* __restrict__ fw, unsigned long long* beta_acc) {
+// Steps 7-8 for the forward entries: divide by the gamma of their row and add the cell's own
+// column into beta.
+__global__ __launch_bounds__(256) void k_adv_rows_fwd(DevGrid g, const float* __restrict__ g6, const int* __restrict__ fq0,
+                                                      float* __restrict__ fw, unsigned long long* beta_acc) {
     INTERIOR_XYZ(g);
-    const int p0 = bp0[c];
-    if (p0 >= 0) {
-        const float gi = g6[c];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            float a = bw[(size_t)k * g.N + c];
-            if (gi > 1.e-5f) a /= gi;
-            bw[(size_t)k * g.N + c] = a;
-            fix_add(beta_acc, p0 + cornerOffset(g, k), a);
-        }
-    }
     const int q0 = fq0[c];
-    if (q0 >= 0) {
-        long long own = 0;
+    if (q0 < 0) return;
+    long long own = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const float gi = g6[q0 + cornerOffset(g, k)];
-            float a = fw[(size_t)k * g.N + c];
-            if (gi > 1.e-5f) a /= gi;
-            fw[(size_t)k * g.N + c] = a;
-            own += to_fix(a);
-        }
-        if (own != 0) atomicAdd(beta_acc + c, (unsigned long long)own);
+    for (int k = 0; k < 8; k++) {
+        const float gi = g6[q0 + cornerOffset(g, k)];
+        float a = fw[(size_t)k * g.N + c];
+        if (gi > 1.e-5f) a /= gi;
+        fw[(size_t)k * g.N + c] = a;
+        own += to_fix(a);
     }
+    if (own != 0) atomicAdd(beta_acc + c, (unsigned long long)own);
 }
 
 // beta field (all cells) from the fixed-point column sums; clears the accumulator.
@@ -327,40 +335,29 @@ __global__ void k_fix_to_field(int n, unsigned long long* __restrict__ acc, floa
     acc[i] = 0ull;
 }
 
-// Steps 9-11 (scatter half): divide by beta (columns); forward entries add to the next row
-// sums (gamma) and to rho' = A rho.
-__global__ __launch_bounds__(256) void k_adv_scale_cols(DevGrid g, const int* __restrict__ bp0, float* __restrict__ bw,
-                                                        const int* __restrict__ fq0, float* __restrict__ fw,
-                                                        unsigned long long* gamma_acc, unsigned long long* rho_acc) {
+// Steps 9-11 for the forward entries: divide by beta (column), add to the next row sums (gamma)
+// and to rho' = A rho.
+__global__ __launch_bounds__(256) void k_adv_cols_fwd(DevGrid g, const int* __restrict__ fq0, const float* __restrict__ fw,
+                                                      unsigned long long* gamma_acc, unsigned long long* rho_acc) {
     INTERIOR_XYZ(g);
-    const int p0 = bp0[c];
-    if (p0 >= 0) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const float bj = g.beta[p0 + cornerOffset(g, k)];
-            float a = bw[(size_t)k * g.N + c];
-            if (bj > 1.e-5f) a /= bj;
-            bw[(size_t)k * g.N + c] = a;
-        }
-    }
     const int q0 = fq0[c];
-    if (q0 >= 0) {
-        const float bj = g.beta[c];
-        const float rj = g.rho[c];
+    if (q0 < 0) return;
+    const float bj = g.beta[c];
+    const float rj = g.rho[c];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            float a = fw[(size_t)k * g.N + c];
-            if (bj > 1.e-5f) a /= bj;
-            const int i = q0 + cornerOffset(g, k);
-            fix_add(gamma_acc, i, a);
-            fix_add(rho_acc, i, a * rj);
-        }
+    for (int k = 0; k < 8; k++) {
+        float a = fw[(size_t)k * g.N + c];
+        if (bj > 1.e-5f) a /= bj;
+        const int i = q0 + cornerOffset(g, k);
+        fix_add(gamma_acc, i, a);
+        fix_add(rho_acc, i, a * rj);
     }
 }
 
-// Steps 10-11 (gather half), all cells: gamma = row sums; rho' = A rho on the interior.
+// Steps 9-11 for the backward entries, all cells: each entry divided by its row's gamma and its
+// column's beta; gamma = row sums; rho' = A rho on the interior.
 __global__ __launch_bounds__(256) void k_adv_final(DevGrid g, const int* __restrict__ bp0, const float* __restrict__ bw,
-                                                   unsigned long long* __restrict__ gamma_acc,
+                                                   const float* __restrict__ g6, unsigned long long* __restrict__ gamma_acc,
                                                    unsigned long long* __restrict__ rho_acc, float* __restrict__ rho_out) {
     FULL_XYZ(g);
     long long gs = (long long)gamma_acc[c];
@@ -370,11 +367,16 @@ __global__ __launch_bounds__(256) void k_adv_final(DevGrid g, const int* __restr
     const int p0 = bp0[c];
     const bool interior = x > 0 && y > 0 && z > 0 && x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
     if (interior && p0 >= 0) {
+        const float gi = g6[c];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const float a = bw[(size_t)k * g.N + c];
+            float a = bw[(size_t)k * g.N + c];
+            if (gi > 1.e-5f) a /= gi;
+            const int j = p0 + cornerOffset(g, k);
+            const float bj = g.beta[j];
+            if (bj > 1.e-5f) a /= bj;
             gs += to_fix(a);
-            rs += to_fix(a * g.rho[p0 + cornerOffset(g, k)]);
+            rs += to_fix(a * g.rho[j]);
         }
     }
     g.gamma[c] = (float)((double)gs * FIX_INV);
@@ -693,14 +695,13 @@ int efs_launch_rho_advect(efs_sim* s) {
     // step 5 (+ forward half of step 6)
     hipLaunchKernelGGL(k_adv_fwd, gi, blk, 0, s->stream, g, dt, s->acc0, s->fwd_p0, s->fwd_w, s->acc1);
     hipLaunchKernelGGL(k_fill_u64, g1, b1, 0, s->stream, s->N, s->acc0);
-    // step 6: gamma6 -> s1 (the advected gamma of step 1 is no longer needed)
-    hipLaunchKernelGGL(k_adv_rowsum, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->s1);
-    // steps 7-8
-    hipLaunchKernelGGL(k_adv_scale_rows, gi, blk, 0, s->stream, g, s->s1, s->bwd_p0, s->bwd_w, s->fwd_p0, s->fwd_w, s->acc0);
+    // steps 6-8: gamma6 -> s1 (the advected gamma of step 1 is no longer needed), new beta
+    hipLaunchKernelGGL(k_adv_rows, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->s1, s->acc0);
+    hipLaunchKernelGGL(k_adv_rows_fwd, gi, blk, 0, s->stream, g, s->s1, s->fwd_p0, s->fwd_w, s->acc0);
     hipLaunchKernelGGL(k_fix_to_field, g1, b1, 0, s->stream, s->N, s->acc0, s->beta);
     // steps 9-11
-    hipLaunchKernelGGL(k_adv_scale_cols, gi, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->fwd_p0, s->fwd_w, s->acc1, s->acc2);
-    hipLaunchKernelGGL(k_adv_final, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->acc2, s->s0);
+    hipLaunchKernelGGL(k_adv_cols_fwd, gi, blk, 0, s->stream, g, s->fwd_p0, s->fwd_w, s->acc1, s->acc2);
+    hipLaunchKernelGGL(k_adv_final, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->s1, s->acc1, s->acc2, s->s0);
     EFS_HIP(hipGetLastError());
     s->traces_valid = true;
     std::swap(s->rho, s->s0);
