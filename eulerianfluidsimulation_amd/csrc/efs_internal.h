@@ -65,7 +65,10 @@ __device__ inline double from_fix(unsigned long long a) { return (double)(long l
 // clamped to [0, n-2]; for the finite positions the traces produce ([1, n-2] - 0.5) the clamp
 // never binds, it only keeps non-finite velocities inside the arrays.
 // ---------------------------------------------------------------------------------------
-__device__ inline int lowIndex(float coord, float offset) { return (int)(floorf(coord - offset) + 1.0e-10f); }
+// The reference adds 1e-10f to the floor before truncating; floorf(v) is an integer and 1e-10f is
+// below half an ulp of every nonzero integer, so the sum truncates to the same int (0 -> 1e-10
+// -> 0) and the addition is left out.
+__device__ inline int lowIndex(float coord, float offset) { return (int)floorf(coord - offset); }
 
 __device__ inline float trilin(const DevGrid& g, const float* __restrict__ f, float px, float py, float pz,
                                float ox, float oy, float oz) {
@@ -160,6 +163,7 @@ __device__ inline float3 traceParticle(const DevGrid& g, float sx, float sy, flo
     const float subDt = dt / (float)substeps;
     const float half = subDt * 0.5f;
     float px = sx, py = sy, pz = sz;
+#pragma unroll 1
     for (int s = 0; s < substeps; s++) {
         const float3 k1 = vinterp(g, px - 0.5f, py - 0.5f, pz - 0.5f);
         const float mx = clampv(px + k1.x * half, 1.f, Wf - 2.f);
@@ -169,6 +173,7 @@ __device__ inline float3 traceParticle(const DevGrid& g, float sx, float sy, flo
         const float nx = px + k2.x * subDt, ny = py + k2.y * subDt, nz = pz + k2.z * subDt;
         if (inWall(g, nx, ny, nz)) {
             float sx_ = px, sy_ = py, sz_ = pz, ux = nx, uy = ny, uz = nz;
+#pragma unroll 1
             for (int b = 0; b < 8; b++) {
                 const float hx = (sx_ + ux) * 0.5f, hy = (sy_ + uy) * 0.5f, hz = (sz_ + uz) * 0.5f;
                 if (inWall(g, hx, hy, hz)) { ux = hx; uy = hy; uz = hz; }

@@ -61,41 +61,47 @@ __global__ __launch_bounds__(256) void k_decay(DevGrid g, float decay) {
 }
 
 // ---- vel_extrapolate (CPUSimulator3D.cpp:61-111) -------------------------------------------
-__global__ __launch_bounds__(256) void k_extrap_init(DevGrid g, uint8_t* valid) {
-    INTERIOR_XYZ(g);
-    valid[c] = (g.rho[c] >= 0.5f && !(g.bf[c] & BF_EQ1)) ? 1 : 0;
+// The reference runs 6 Jacobi passes (velocity and validity ping-ponged between two buffers): an
+// invalid, non-solid cell with valid neighbours becomes valid with their mean, every other cell
+// is copied.  A cell's velocity therefore changes at most once, in the pass where it becomes
+// valid, and what it reads then (the neighbours valid before that pass) never changes again.  So
+// the passes run in place: stamp[c] is the pass at which c became valid (0: valid from the
+// start, 255: not yet), and pass k updates the cells with stamp 255 that have a neighbour with
+// stamp < k.  A neighbour becoming valid in the same pass reads as 255 or k, both "not valid
+// before k", so the result does not depend on the order of the threads; a cell that is not on
+// the front costs one byte read instead of a copy.
+__global__ __launch_bounds__(256) void k_extrap_init(DevGrid g, uint8_t* __restrict__ stamp) {
+    FULL_XYZ(g);
+    const bool interior = x > 0 && y > 0 && z > 0 && x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
+    stamp[c] = (interior && g.rho[c] >= 0.5f && !(g.bf[c] & BF_EQ1)) ? 0 : 255;
 }
 
-__global__ __launch_bounds__(256) void k_extrap_pass(DevGrid g, const float* __restrict__ su, const float* __restrict__ sv,
-                                                     const float* __restrict__ sw, const uint8_t* __restrict__ sval,
-                                                     float* __restrict__ du, float* __restrict__ dv, float* __restrict__ dw,
-                                                     uint8_t* __restrict__ dval) {
+// The stamps and the boundary byte are loaded together, unconditionally (one memory round trip
+// for a cell off the front, instead of one per test).
+__global__ __launch_bounds__(256) void k_extrap_pass(DevGrid g, uint8_t* stamp, int pass) {
     INTERIOR_XYZ(g);
     const int sy = g.W, sz = g.W * g.H;
-    const int nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
-    // own values and the six neighbour validity bytes, loaded unconditionally
-    const float u0 = su[c], v0 = sv[c], w0 = sw[c];
-    const unsigned own = sval[c];
+    const int nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};  // the reference's order
+    const unsigned own = stamp[c], b = g.bf[c];
+    unsigned st[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) st[k] = stamp[nb[k]];
     unsigned m = 0;
 #pragma unroll
-    for (int k = 0; k < 6; k++) m |= (sval[nb[k]] ? 1u : 0u) << k;
-    const bool solid = g.bf[c] & BF_EQ1;
-    if (solid || own || m == 0) {
-        du[c] = u0; dv[c] = v0; dw[c] = w0;
-        dval[c] = (!solid && own) ? 1 : 0;
-        return;
-    }
-    // a front cell: mean of the valid neighbours, in the reference's order
+    for (int k = 0; k < 6; k++) m |= ((int)st[k] < pass ? 1u : 0u) << k;
+    // one combined test (short-circuit tests would let the compiler sink the loads below them)
+    const unsigned front = (unsigned)(own == 255) & (unsigned)((b & BF_EQ1) == 0) & (unsigned)(m != 0);
+    if (!front) return;
     float ax = 0.f, ay = 0.f, az = 0.f, n = 0;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         if (m & (1u << k)) {
-            ax = ax + su[nb[k]]; ay = ay + sv[nb[k]]; az = az + sw[nb[k]];
+            ax = ax + g.u[nb[k]]; ay = ay + g.v[nb[k]]; az = az + g.w[nb[k]];
             n++;
         }
     }
-    du[c] = ax / n; dv[c] = ay / n; dw[c] = az / n;
-    dval[c] = 1;
+    g.u[c] = ax / n; g.v[c] = ay / n; g.w[c] = az / n;
+    stamp[c] = (uint8_t)pass;
 }
 
 // ---- rho_sharpen (CPUSimulator3D.cpp:366-456) ----------------------------------------------
@@ -664,11 +670,9 @@ int efs_launch_decay(efs_sim* s) {
 int efs_launch_extrapolate(efs_sim* s) {
     const DevGrid g = s->grid();
     const dim3 grd = interior_grid(s), blk(BX, BY);
-    hipLaunchKernelGGL(k_extrap_init, grd, blk, 0, s->stream, g, s->valA);
-    for (int i = 0; i < 3; i++) {
-        hipLaunchKernelGGL(k_extrap_pass, grd, blk, 0, s->stream, g, s->u, s->v, s->w, s->valA, s->tu, s->tv, s->tw, s->valB);
-        hipLaunchKernelGGL(k_extrap_pass, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw, s->valB, s->u, s->v, s->w, s->valA);
-    }
+    hipLaunchKernelGGL(k_extrap_init, full_grid(s), blk, 0, s->stream, g, s->valA);
+    for (int pass = 1; pass <= 6; pass++)  // the reference's 3 double passes
+        hipLaunchKernelGGL(k_extrap_pass, grd, blk, 0, s->stream, g, s->valA, pass);
     EFS_HIP(hipGetLastError());
     return 0;
 }
