@@ -246,17 +246,20 @@ int run_project(efs_sim* s, efs_stats* st) {
     return status;
 }
 
-int run_stage(efs_sim* s, int stage, efs_stats* st) {
+// `fuse`: in a full 3D step, vel_decay runs inside vel_extrapolate and add_force inside vel_advect
+// when both stages of the pair are enabled (the same operations in the same order; the second
+// stage of each pair is then skipped by the caller).
+int run_stage(efs_sim* s, int stage, efs_stats* st, bool fuse = false) {
     if (s->dim == 2) return efs2_run_stage(s, stage, st);
     // every stage but the velocity advection that consumes them invalidates the stored traces
     // (rho_advect stores fresh ones)
     if (stage != EFS_STAGE_VEL_ADVECT) s->traces_valid = false;
     switch (stage) {
         case EFS_STAGE_VEL_DECAY: return efs_launch_decay(s);
-        case EFS_STAGE_VEL_EXTRAPOLATE: return efs_launch_extrapolate(s);
+        case EFS_STAGE_VEL_EXTRAPOLATE: return efs_launch_extrapolate(s, fuse);
         case EFS_STAGE_RHO_SHARPEN: return efs_launch_sharpen(s);
         case EFS_STAGE_RHO_ADVECT: return efs_launch_rho_advect(s);
-        case EFS_STAGE_VEL_ADVECT: return efs_launch_vel_advect(s);
+        case EFS_STAGE_VEL_ADVECT: return efs_launch_vel_advect(s, fuse);
         case EFS_STAGE_ADD_FORCE: return efs_launch_add_force(s);
         case EFS_STAGE_PROJECT: return run_project(s, st);
     }
@@ -435,9 +438,15 @@ int efs_step(efs_handle h, efs_stats* st) {
     }
     int status = 0;
     stage_event(h, 0);
+    const unsigned on = ~h->P.disable_mask;
+    auto enabled = [&](int k) { return (on >> k) & 1u; };
+    const bool fuse_decay = h->dim == 3 && enabled(EFS_STAGE_VEL_DECAY) && enabled(EFS_STAGE_VEL_EXTRAPOLATE);
+    const bool fuse_force = h->dim == 3 && enabled(EFS_STAGE_VEL_ADVECT) && enabled(EFS_STAGE_ADD_FORCE);
     for (int k = 0; k <= EFS_STAGE_PROJECT; k++) {
-        if (!(h->P.disable_mask & (1u << k))) {
-            const int rc = run_stage(h, k, st);
+        const bool folded = (k == EFS_STAGE_VEL_DECAY && fuse_decay) || (k == EFS_STAGE_ADD_FORCE && fuse_force);
+        if (enabled(k) && !folded) {
+            const bool fuse = (k == EFS_STAGE_VEL_EXTRAPOLATE && fuse_decay) || (k == EFS_STAGE_VEL_ADVECT && fuse_force);
+            const int rc = run_stage(h, k, st, fuse);
             if (rc < 0) return rc;
             if (k == EFS_STAGE_PROJECT) status = rc;
         }

@@ -308,10 +308,10 @@ int efs_hip_fail(hipError_t e, const char* what);
 
 // stage launchers (efs_step.hip)
 int efs_launch_decay(efs_sim* s);
-int efs_launch_extrapolate(efs_sim* s);
+int efs_launch_extrapolate(efs_sim* s, bool with_decay = false);  // with_decay: vel_decay first, fused
 int efs_launch_sharpen(efs_sim* s);
 int efs_launch_rho_advect(efs_sim* s);
-int efs_launch_vel_advect(efs_sim* s);
+int efs_launch_vel_advect(efs_sim* s, bool with_force = false);  // with_force: add_force after, fused
 int efs_launch_add_force(efs_sim* s);
 int efs_launch_walls_divergence(efs_sim* s);
 int efs_launch_gradient_walls(efs_sim* s);

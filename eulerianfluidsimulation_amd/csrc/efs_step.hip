@@ -70,10 +70,19 @@ __global__ __launch_bounds__(256) void k_decay(DevGrid g, float decay) {
 // stamp < k.  A neighbour becoming valid in the same pass reads as 255 or k, both "not valid
 // before k", so the result does not depend on the order of the threads; a cell that is not on
 // the front costs one byte read instead of a copy.
-__global__ __launch_bounds__(256) void k_extrap_init(DevGrid g, uint8_t* __restrict__ stamp) {
+// With DECAY the velocity decay of the step (k_decay) is applied in the same pass: it reads only
+// rho, which the stamps read too.
+template <bool DECAY>
+__global__ __launch_bounds__(256) void k_extrap_init(DevGrid g, uint8_t* __restrict__ stamp, float decay) {
     FULL_XYZ(g);
     const bool interior = x > 0 && y > 0 && z > 0 && x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
-    stamp[c] = (interior && g.rho[c] >= 0.5f && !(g.bf[c] & BF_EQ1)) ? 0 : 255;
+    const float r = g.rho[c];
+    if (DECAY && interior && r < 0.5f) {
+        g.u[c] = g.u[c] * decay;
+        g.v[c] = g.v[c] * decay;
+        g.w[c] = g.w[c] * decay;
+    }
+    stamp[c] = (interior && r >= 0.5f && !(g.bf[c] & BF_EQ1)) ? 0 : 255;
 }
 
 // The stamps and the boundary byte are loaded together, unconditionally (one memory round trip
@@ -555,14 +564,30 @@ __global__ __launch_bounds__(256) void k_vadv_from_traces(DevGrid g, float* __re
     tw[c] = v.z;
 }
 
+// With FORCE the add_force stage that follows (k_add_force) is applied to the new face values
+// before they are stored, in the same operations.
+template <bool FORCE>
 __global__ __launch_bounds__(256) void k_vadv_faces(DevGrid g, const float* __restrict__ tu, const float* __restrict__ tv,
-                                                    const float* __restrict__ tw) {
+                                                    const float* __restrict__ tw, float dt, float gdt, int gravity) {
     INTERIOR_XYZ(g);
     const int sy = g.W, sz = g.W * g.H;
     const bool bs = g.bf[c] & BF_EQ1;
-    g.u[c] = (x == 1 || bs || (g.bf[c - 1] & BF_EQ1)) ? tu[c] : 0.5f * (tu[c] + tu[c - 1]);
-    g.v[c] = (y == 1 || bs || (g.bf[c - sy] & BF_EQ1)) ? tv[c] : 0.5f * (tv[c] + tv[c - sy]);
-    g.w[c] = (z == 1 || bs || (g.bf[c - sz] & BF_EQ1)) ? tw[c] : 0.5f * (tw[c] + tw[c - sz]);
+    float u = (x == 1 || bs || (g.bf[c - 1] & BF_EQ1)) ? tu[c] : 0.5f * (tu[c] + tu[c - 1]);
+    float v = (y == 1 || bs || (g.bf[c - sy] & BF_EQ1)) ? tv[c] : 0.5f * (tv[c] + tv[c - sy]);
+    float w = (z == 1 || bs || (g.bf[c - sz] & BF_EQ1)) ? tw[c] : 0.5f * (tw[c] + tw[c - sz]);
+    if (FORCE && !bs) {
+        if (g.fu) {
+            u = u + g.fu[c] * dt;
+            v = v + g.fv[c] * dt;
+            w = w + g.fw[c] * dt;
+        } else {
+            u = u + 0.f * dt; v = v + 0.f * dt; w = w + 0.f * dt;
+        }
+        if (gravity) v -= gdt;
+    }
+    g.u[c] = u;
+    g.v[c] = v;
+    g.w[c] = w;
 }
 
 // ---- add_force (CPUSimulator3D.cpp:458-476) -----------------------------------------------
@@ -667,10 +692,13 @@ int efs_launch_decay(efs_sim* s) {
     return 0;
 }
 
-int efs_launch_extrapolate(efs_sim* s) {
+int efs_launch_extrapolate(efs_sim* s, bool with_decay) {
     const DevGrid g = s->grid();
     const dim3 grd = interior_grid(s), blk(BX, BY);
-    hipLaunchKernelGGL(k_extrap_init, full_grid(s), blk, 0, s->stream, g, s->valA);
+    if (with_decay)
+        hipLaunchKernelGGL(k_extrap_init<true>, full_grid(s), blk, 0, s->stream, g, s->valA, s->P.vel_decay);
+    else
+        hipLaunchKernelGGL(k_extrap_init<false>, full_grid(s), blk, 0, s->stream, g, s->valA, 1.f);
     for (int pass = 1; pass <= 6; pass++)  // the reference's 3 double passes
         hipLaunchKernelGGL(k_extrap_pass, grd, blk, 0, s->stream, g, s->valA, pass);
     EFS_HIP(hipGetLastError());
@@ -720,13 +748,17 @@ int efs_launch_rho_advect(efs_sim* s) {
     return 0;
 }
 
-int efs_launch_vel_advect(efs_sim* s) {
+int efs_launch_vel_advect(efs_sim* s, bool with_force) {
     const DevGrid g = s->grid();
     const dim3 grd = interior_grid(s), blk(BX, BY);
     if (s->traces_valid) hipLaunchKernelGGL(k_vadv_from_traces, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw);
     else hipLaunchKernelGGL(k_vadv_trace, grd, blk, 0, s->stream, g, s->P.dt, s->tu, s->tv, s->tw);
     s->traces_valid = false;
-    hipLaunchKernelGGL(k_vadv_faces, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw);
+    if (with_force)
+        hipLaunchKernelGGL(k_vadv_faces<true>, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw, s->P.dt,
+                           s->P.gravity_force * s->P.dt, s->P.gravity_enabled);
+    else
+        hipLaunchKernelGGL(k_vadv_faces<false>, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw, 0.f, 0.f, 0);
     EFS_HIP(hipGetLastError());
     return 0;
 }
