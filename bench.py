@@ -34,6 +34,9 @@ K2_BYTES_PC = 2 + 4 + 8 + 4 + 8 + 4  # preconditioned CG: the FP64 iterate is re
 # code and right-hand side read, the red cells' parent correction read from the coarse level
 # (1/8 of the cells), z written
 MG_UP_BYTES = 2 + 4 + 4 / 8 + 4
+# the fused fine-level down sweep: row code and right-hand side read, the restricted residual
+# written to the coarse level (1/8 of the cells)
+MG_DOWN_BYTES = 2 + 4 + 4 / 8
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_sphere_pool_256_mg.md")
@@ -157,7 +160,8 @@ def main():
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     kt = {}
-    for k, name in ((_abi.KT_CG_DIR_SPMV, "cg_dir_spmv"), (_abi.KT_CG_UPDATE, "cg_update"), (_abi.KT_OTHER, "mg_up_fine")):
+    for k, name in ((_abi.KT_CG_DIR_SPMV, "cg_dir_spmv"), (_abi.KT_CG_UPDATE, "cg_update"), (_abi.KT_OTHER, "mg_up_fine"),
+                    (_abi.KT_MG_DOWN, "mg_down_fine")):
         n = ctypes.c_int64()
         ms = ctypes.c_double()
         lib.efs_kernel_time(sim.h, k, ctypes.byref(n), ctypes.byref(ms))
@@ -188,7 +192,7 @@ def main():
     real = max(1, sum(iters))
     avg_ms = total_ms / max(1, launches)
     bytes_per_launch = {"cg_dir_spmv": K1_BYTES, "cg_update": K2_BYTES_PC if ARGS.precond != "none" else K2_BYTES,
-                        "mg_up_fine": MG_UP_BYTES}[dom] * interior
+                        "mg_up_fine": MG_UP_BYTES, "mg_down_fine": MG_DOWN_BYTES}[dom] * interior
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     steps_per_s = ARGS.steps / elapsed
     its = float(np.mean(iters)) if iters else 0.0

@@ -15,7 +15,7 @@ SOLVER_SELF_CG, SOLVER_JACOBI = 2, 3
 CRIT_REFERENCE, CRIT_RELATIVE = 0, 1
 PRECOND_NONE, PRECOND_LEARNED, PRECOND_MG = 0, 1, 2
 GUESS_DIVERGENCE, GUESS_PREVIOUS = 0, 1
-KT_CG_DIR_SPMV, KT_CG_UPDATE, KT_JACOBI, KT_OTHER = range(4)
+KT_CG_DIR_SPMV, KT_CG_UPDATE, KT_JACOBI, KT_OTHER, KT_MG_DOWN = range(5)
 
 
 class Params(ctypes.Structure):

@@ -281,7 +281,7 @@ struct efs_sim {
     // timing
     bool timing = false;
     bool ktiming = false;
-    TimerSlot ktimers[4];
+    TimerSlot ktimers[5];
     hipEvent_t ev_stage[8] = {};
     // learned preconditioner state (owned by the preconditioner module)
     void* learned = nullptr;
@@ -358,8 +358,8 @@ void efs_mg_free(efs_sim* s);
 int efs_learned_cg_solve(efs_sim* s, efs_stats* st);
 void efs_learned_free(efs_sim* s);
 
-// event-timed kernels: KT_OTHER times the fine-level multigrid up sweep
-enum { KT_CG_DIR_SPMV = 0, KT_CG_UPDATE = 1, KT_JACOBI = 2, KT_OTHER = 3 };
+// event-timed kernels: KT_OTHER times the fine-level multigrid up sweep, KT_MG_DOWN its down sweep
+enum { KT_CG_DIR_SPMV = 0, KT_CG_UPDATE = 1, KT_JACOBI = 2, KT_OTHER = 3, KT_MG_DOWN = 4 };
 void efs_ktimer_begin(efs_sim* s, int k, hipEvent_t* a, hipEvent_t* b);
 void efs_ktimer_end(efs_sim* s, int k, hipEvent_t a, hipEvent_t b);
 void efs_trim_timer(TimerSlot& t, size_t keep);  // drops the events of launches past `keep`

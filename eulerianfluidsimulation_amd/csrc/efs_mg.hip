@@ -252,11 +252,11 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
         auto black = [&](int p) {  // black value of the lane's pair in plane p
             if (!blk) return;
             const int q = ((bxu + byu + p) & 1) == 1 ? 0 : 1;
-            const int k = bk + q, x = bxu + q;
+            const int k = bk + q, km = k - 1, kp = k + 1, x = bxu + q;
             const float *r0 = ro[(p + 2) % 3], *r1 = ro[(p + 3) % 3], *r2 = ro[(p + 4) % 3];
             const unsigned rc = rs[(p + 4) & 3][k];
             const float b = bs[(p + 4) & 3][k];
-            const float sum = 0.f + r1[k - 1] + r1[k + 1] + r1[k - PX] + r1[k + PX] + r2[k] + r0[k];
+            const float sum = 0.f + r1[km] + r1[kp] + r1[k - PX] + r1[k + PX] + r2[k] + r0[k];
             const bool inside = x >= 1 && x <= l.nx && byu >= 1 && byu <= l.ny && p >= 1 && p <= l.nz;
             const float val = inside ? row_update(rc, sum, b, inv_s) : 0.f;
             bn[(p + 3) % 3][k] = (rc & RC_CONTRIB) ? val : 0.f;
@@ -273,12 +273,12 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
         auto finish = [&](int q) {  // red value (up) or red residual (down) of the lane's pair in plane q
             if (!fin) return;
             const int h = ((txu + tyu + q) & 1) == 0 ? 0 : 1;
-            const int k = tk + h, x = txu + h;
+            const int k = tk + h, km = k - 1, kp = k + 1, x = txu + h;
             const bool inside = x <= l.nx && tyu <= l.ny;
             const float *n0 = bn[(q + 2) % 3], *n1 = bn[(q + 3) % 3], *n2 = bn[(q + 4) % 3];
             const unsigned rc = rs[(q + 4) & 3][k];
             const float b = bs[(q + 4) & 3][k];
-            const float sum = 0.f + n1[k - 1] + n1[k + 1] + n1[k - PX] + n1[k + PX] + n2[k] + n0[k];
+            const float sum = 0.f + n1[km] + n1[kp] + n1[k - PX] + n1[k + PX] + n2[k] + n0[k];
             if (UP) {
                 if (!inside) return;
                 const float val = row_update(rc, sum, b, inv_s);
@@ -491,8 +491,11 @@ int vcycle(efs_sim* s, void*) {
     for (int k = 0; k < nl - 1; k++) {
         const Lvl& l = m->L[k];
         const Dom d = post_dom(l);
+        hipEvent_t a = nullptr, b = nullptr;
+        if (k == 0) efs_ktimer_begin(s, KT_MG_DOWN, &a, &b);  // the fine down sweep is event-timed too
         hipLaunchKernelGGL(k_mg_sweep<SW_DOWN>, dim3(launch_blocks(d)), pb, 0, s->stream, l, m->L[k + 1], 1.f / l.s, d, st,
                            nullptr);
+        if (k == 0) efs_ktimer_end(s, KT_MG_DOWN, a, b);
     }
     hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(MG_CO_NT), 0, s->stream, m->L[nl - 1], 1.f / m->L[nl - 1].s,
                        EFS_MG_COARSE_SWEEPS, st);
@@ -547,10 +550,11 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
     efs_stats* o = out ? out : &local;
     int total = 0;
     for (int attempt = 0; attempt < 3; attempt++) {
-        const size_t kt0 = s->ktimers[KT_OTHER].start.size();
+        const size_t kt0 = s->ktimers[KT_OTHER].start.size(), kt1 = s->ktimers[KT_MG_DOWN].start.size();
         rc = efs_cg_run_pc(s, o, false, std::max(1, s->P.max_steps - total), &pc);
         // launches after convergence return at once: keep the timed fine up sweep of each real V-cycle
         efs_trim_timer(s->ktimers[KT_OTHER], kt0 + ((size_t)s->st_host->iter + 1));
+        efs_trim_timer(s->ktimers[KT_MG_DOWN], kt1 + ((size_t)s->st_host->iter + 1));
         total += o->iterations;
         const bool verify = s->P.true_residual && s->P.criterion == EFS_CRIT_RELATIVE;
         if (rc != 0 || !verify || o->true_rel_residual <= (double)s->P.solver_threshold) break;
