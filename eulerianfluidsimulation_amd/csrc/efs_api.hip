@@ -231,7 +231,10 @@ void stage_event(efs_sim* s, int k) {
 
 int run_project(efs_sim* s, efs_stats* st) {
     int rc;
-    EFS_HIP(hipMemsetAsync(s->p, 0, (size_t)s->N * 4, s->stream));  // pressure.clear()
+    // pressure.clear(): Jacobi starts from it; the CG solves write every cell of p when they end
+    // (efs_cg_finalize), so for them the clear would be overwritten unread
+    const bool cg = s->P.solver_type != EFS_SOLVER_JACOBI && s->P.precond != EFS_PRECOND_LEARNED;
+    if (!cg) EFS_HIP(hipMemsetAsync(s->p, 0, (size_t)s->N * 4, s->stream));
     if ((rc = efs_launch_walls_divergence(s))) return rc;
     efs_stats local{};
     efs_stats* out = st ? st : &local;

@@ -273,6 +273,7 @@ struct efs_sim {
     double* cg_x64 = nullptr;   // FP64 iterate (reliable-update accumulator)
     bool p64_valid = false;     // the last solve left its FP64 iterate in cg_x64
     bool x64_has_previous = false;  // cg_x64 holds a finished solve (usable as the next initial guess)
+    int x0_pending = -1;            // initial iterate still to set by k_cg_init (0: b, 1: previous on CG rows)
     double* partials = nullptr;
     double *p_rr = nullptr, *p_bb = nullptr, *p_dad = nullptr;  // per-block partial sums of the CG kernels
     int max_blocks = 0;
@@ -336,9 +337,10 @@ struct CgPrecond {
 };
 int efs_cg_run_pc(efs_sim* s, efs_stats* st, bool border, int max_it, const CgPrecond* pc);
 int efs_cg_setup_x0(efs_sim* s);                   // x64 = b, xd = 0
+int efs_cg_defer_x0(efs_sim* s);                   // the same, done by the next efs_cg_init_residual
 int efs_cg_init_residual(efs_sim* s, bool border);  // r = b - A x64 (FP64), partials of r.r, b.b
 int efs_cg_fold_all(efs_sim* s);                    // x64 += xd, xd = 0
-int efs_cg_finalize(efs_sim* s);                    // x64 += xd, pressure = (float)x64
+int efs_cg_finalize(efs_sim* s, bool xd_zero = false);  // x64 += xd, pressure = (float)x64
 int efs_jacobi_solve(efs_sim* s, efs_stats* st);
 int efs_launch_spmv(efs_sim* s, const float* x, float* y);
 int efs_true_residual(efs_sim* s, double* num, double* den);

@@ -530,7 +530,7 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
     int border = 0;
     int rc = efs_launch_rowcode(s, &border);
     if (rc) return rc;
-    if ((rc = efs_cg_setup_x0(s))) return rc;
+    if ((rc = efs_cg_defer_x0(s))) return rc;  // x0 is set by the first residual kernel
     // fluid rows on the outer layer (never in practice) or a grid too small to coarsen: plain CG
     if (border || std::min(s->W, std::min(s->H, s->D)) < 6) return efs_cg_run(s, out, border != 0, s->P.max_steps);
     if ((rc = ensure(s))) return rc;

@@ -135,22 +135,39 @@ __device__ inline bool cg_check(CGState* st, int it, const double* __restrict__ 
 
 // r0 = b - A x0 (FP64, rounded; x0 = x64, which is b for the reference's initial guess), d0 = r0;
 // partials of r.r and b.b
+// With x0 >= 0 it also sets the initial iterate on the way (the work of k_cg_setup): x0 = 0 takes
+// x0 = b; x0 = 1 keeps the previous iterate on CG rows and takes b elsewhere.  Each cell writes
+// only its own x64, and only where the value becomes b, while neighbours read x64 only on CG rows
+// (x0 = 1) or not at all (x0 = 0): no thread reads a value another thread writes.
+template <int X0>
 __global__ __launch_bounds__(CG_NT) void k_cg_init(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
-                                                   const double* __restrict__ x64, float* __restrict__ r, float* __restrict__ d0,
-                                                   double* p_rr, double* p_bb) {
+                                                   double* __restrict__ x64, float* __restrict__ r, float* __restrict__ d0,
+                                                   double* p_rr, double* p_bb, float* __restrict__ xd) {
+    constexpr int x0 = X0;
     __shared__ double red[CG_NT / 64];
     int bx, by, bz;
     double vr = 0.0, vb = 0.0;
     if (block_coords(d, bx, by, bz)) {
         const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * d.ty + threadIdx.y;
         const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
+        auto X = [&](int q) -> double {
+            if (x0 < 0) return x64[q];
+            if (x0 == 0) return (double)b[q];
+            const double xv = x64[q], bv = (double)b[q];  // loaded together, then selected
+            return (rcode[q] & RC_CG_ROW) ? xv : bv;
+        };
         if (x < d.x0 + d.nx && y < d.y0 + d.ny) {
             for (int z = zb; z < ze; z++) {
                 const int c = (z * d.H + y) * d.W + x;
                 const Nb n = nb_offsets(d, x, y, z);
                 const double bc = b[c];
-                const double t = apply_row_d(rcode[c], x64[c], x64[c + n.xm], x64[c + n.xp], x64[c + n.ym], x64[c + n.yp],
-                                             x64[c + n.zp], x64[c + n.zm]);
+                const double xc = X(c);
+                if (x0 >= 0) {
+                    if (!(x0 == 1 && (rcode[c] & RC_CG_ROW))) x64[c] = xc;
+                    xd[c] = 0.f;
+                }
+                const double t = apply_row_d(rcode[c], xc, X(c + n.xm), X(c + n.xp), X(c + n.ym), X(c + n.yp), X(c + n.zp),
+                                             X(c + n.zm));
                 const float rr = (float)(bc - t);
                 r[c] = rr;
                 d0[c] = rr;
@@ -485,10 +502,14 @@ __global__ __launch_bounds__(CG_NT) void k_cg_rz(Dom d, const float* __restrict_
     write_partial(v, p_rz, red);
 }
 
-// whole array: x64 += xd; pressure = (float)x64
+// whole array: x64 += xd; pressure = (float)x64 (without xd: the increment is known to be 0)
 __global__ void k_cg_finalize(int n, double* __restrict__ x64, const float* __restrict__ xd, float* __restrict__ p) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (!xd) {
+        p[i] = (float)x64[i];
+        return;
+    }
     const double x = x64[i] + (double)xd[i];
     x64[i] = x;
     p[i] = (float)x;
@@ -502,10 +523,13 @@ __global__ void k_cg_fold_all(int n, double* __restrict__ x64, float* __restrict
     xd[i] = 0.f;
 }
 
-// FP64 true residual ||b - A x||^2 and ||b||^2 (x in FP64 if given, else the FP32 field).
-__global__ __launch_bounds__(256) void k_true_residual(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
-                                                       const float* __restrict__ xf, const double* __restrict__ xdbl,
-                                                       CGState* st, double* partials) {
+// FP64 true residual ||b - A x||^2 and ||b||^2 (x in FP64 if given, else the FP32 field): block
+// partials in partials[block] and partials[gridDim + block], plain stores (no fence), summed in a
+// fixed order by k_true_residual_sum.
+constexpr int TR_NT = 256;
+__global__ __launch_bounds__(TR_NT) void k_true_residual(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
+                                                         const float* __restrict__ xf, const double* __restrict__ xdbl,
+                                                         double* partials) {
     int bx, by, bz;
     double v[2] = {0.0, 0.0};
     if (block_coords(d, bx, by, bz)) {
@@ -524,12 +548,41 @@ __global__ __launch_bounds__(256) void k_true_residual(Dom d, const uint16_t* __
             }
         }
     }
-    double out[2];
-    if (grid_reduce<256, 2>(v, partials, &st->counter, out)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            st->true_num = out[0];
-            st->bnorm2 = out[1];
+    __shared__ double red[2][TR_NT / 64];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const double w = wave_sum(v[k]);
+        if ((lane & 63) == 0) red[k][lane >> 6] = w;
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            double t = 0.0;
+            for (int w = 0; w < TR_NT / 64; w++) t += red[k][w];
+            partials[(size_t)k * gridDim.x + blockIdx.x] = t;
         }
+    }
+}
+__global__ __launch_bounds__(TR_NT) void k_true_residual_sum(int nparts, const double* __restrict__ partials, CGState* st) {
+    __shared__ double red[TR_NT / 64];
+    const int lane = threadIdx.x;
+    double out[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        double t = 0.0;
+        for (int i = lane; i < nparts; i += TR_NT) t += partials[(size_t)k * nparts + i];
+        t = wave_sum(t);
+        if ((lane & 63) == 0) red[lane >> 6] = t;
+        __syncthreads();
+        out[k] = 0.0;
+        for (int w = 0; w < TR_NT / 64; w++) out[k] += red[w];
+        __syncthreads();
+    }
+    if (lane == 0) {
+        st->true_num = out[0];
+        st->bnorm2 = out[1];
     }
 }
 
@@ -701,8 +754,11 @@ int efs_launch_spmv(efs_sim* s, const float* x, float* y) {
 
 int efs_true_residual(efs_sim* s, double* num, double* den) {
     const Dom d = dom_simple(s, true);
-    hipLaunchKernelGGL(k_true_residual, dim3(launch_blocks(d)), dim3(TX, d.ty), 0, s->stream, d, s->rowcode, s->div, s->p,
-                       s->p64_valid ? s->cg_x64 : nullptr, s->st, s->partials);
+    static_assert(TX * 4 == TR_NT, "dom_simple tiles are TX x 4");
+    const int nb = launch_blocks(d);
+    hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(TX, d.ty), 0, s->stream, d, s->rowcode, s->div, s->p,
+                       s->p64_valid ? s->cg_x64 : nullptr, s->partials);
+    hipLaunchKernelGGL(k_true_residual_sum, dim3(1), dim3(TR_NT), 0, s->stream, nb, s->partials, s->st);
     EFS_HIP(hipGetLastError());
     EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
     EFS_HIP(hipStreamSynchronize(s->stream));
@@ -712,6 +768,7 @@ int efs_true_residual(efs_sim* s, double* num, double* den) {
 }
 
 int efs_cg_setup_x0(efs_sim* s) {
+    s->x0_pending = -1;
     // the previous iterate is usable only if the last solve left one (CG, same grid)
     const int prev = (s->P.initial_guess == EFS_GUESS_PREVIOUS && s->x64_has_previous) ? 1 : 0;
     hipLaunchKernelGGL(k_cg_setup, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->div, s->rowcode, prev, s->cg_x64,
@@ -720,10 +777,24 @@ int efs_cg_setup_x0(efs_sim* s) {
     return 0;
 }
 
+int efs_cg_defer_x0(efs_sim* s) {
+    s->x0_pending = (s->P.initial_guess == EFS_GUESS_PREVIOUS && s->x64_has_previous) ? 1 : 0;
+    return 0;
+}
+
 int efs_cg_init_residual(efs_sim* s, bool border) {
     const Dom dt = dom_tiled(s, border);
-    hipLaunchKernelGGL(k_cg_init, dim3(launch_blocks(dt)), dim3(TX, CG_TY), 0, s->stream, dt, s->rowcode, s->div, s->cg_x64, s->cg_r,
-                       s->cg_d0, s->p_rr, s->p_bb);
+    const dim3 grid(launch_blocks(dt)), blk(TX, CG_TY);
+    if (s->x0_pending == 0)
+        hipLaunchKernelGGL(k_cg_init<0>, grid, blk, 0, s->stream, dt, s->rowcode, s->div, s->cg_x64, s->cg_r, s->cg_d0, s->p_rr,
+                           s->p_bb, s->cg_xd);
+    else if (s->x0_pending == 1)
+        hipLaunchKernelGGL(k_cg_init<1>, grid, blk, 0, s->stream, dt, s->rowcode, s->div, s->cg_x64, s->cg_r, s->cg_d0, s->p_rr,
+                           s->p_bb, s->cg_xd);
+    else
+        hipLaunchKernelGGL(k_cg_init<-1>, grid, blk, 0, s->stream, dt, s->rowcode, s->div, s->cg_x64, s->cg_r, s->cg_d0, s->p_rr,
+                           s->p_bb, s->cg_xd);
+    s->x0_pending = -1;
     EFS_HIP(hipGetLastError());
     return 0;
 }
@@ -734,8 +805,9 @@ int efs_cg_fold_all(efs_sim* s) {
     return 0;
 }
 
-int efs_cg_finalize(efs_sim* s) {
-    hipLaunchKernelGGL(k_cg_finalize, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->cg_x64, s->cg_xd, s->p);
+int efs_cg_finalize(efs_sim* s, bool xd_zero) {
+    hipLaunchKernelGGL(k_cg_finalize, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->cg_x64,
+                       xd_zero ? nullptr : s->cg_xd, s->p);
     EFS_HIP(hipGetLastError());
     s->p64_valid = true;
     s->x64_has_previous = true;
@@ -746,8 +818,8 @@ int efs_cg_solve(efs_sim* s, efs_stats* out) {
     int border = 0;
     int rc = efs_launch_rowcode(s, &border);
     if (rc) return rc;
-    // x0 = b (CGSolver.cpp:429)
-    if ((rc = efs_cg_setup_x0(s))) return rc;
+    // x0 = b (CGSolver.cpp:429), set by the first residual kernel
+    if ((rc = efs_cg_defer_x0(s))) return rc;
     return efs_cg_run(s, out, border != 0, s->P.max_steps);
 }
 
@@ -794,7 +866,10 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
             hipLaunchKernelGGL(k_cg_dir_spmv, dim3(nbt), bt, 0, s->stream, dt, i, s->rowcode, dsrc, s->cg_d0, s->cg_d1,
                                s->st, s->p_rr, s->p_bb, s->p_dad, p_rz);
             efs_ktimer_end(s, KT_CG_DIR_SPMV, a, b);
-            if ((i & 0x3F) == 0) {
+            // residual replacement every 64 iterations (CGSolver.cpp:444-451), including i = 0 as in the
+            // reference; preconditioned CG, which has no reference schedule to follow, updates its
+            // FP64 iterate at i = 0 like any other iteration
+            if ((i & 0x3F) == 0 && !(pc && i == 0)) {
                 hipLaunchKernelGGL(k_cg_fold, dim3(nbt), bt, 0, s->stream, dt, i, s->cg_d0, s->cg_d1, s->cg_xd, s->cg_x64, s->st,
                                    s->p_dad);
                 hipLaunchKernelGGL(k_cg_residual, dim3(nbt), bt, 0, s->stream, dt, s->rowcode, s->div, s->cg_x64, s->cg_r, s->st,
@@ -826,9 +901,11 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
     const CGState fin = *s->st_host;
     // launches after convergence exit at once; keep the events of real iterations only
     efs_trim_timer(s->ktimers[KT_CG_DIR_SPMV], kt0 + fin.iter);
-    const size_t repl = (size_t)(fin.iter + 63) / 64;  // replacement iterations have no K2
+    size_t repl = (size_t)(fin.iter + 63) / 64;  // replacement iterations have no K2
+    if (pc && repl > 0) repl--;                   // (preconditioned CG: not i = 0)
     efs_trim_timer(s->ktimers[KT_CG_UPDATE], kt1 + (size_t)fin.iter - std::min<size_t>(repl, fin.iter));
-    if ((rc = efs_cg_finalize(s))) return rc;
+    // preconditioned CG updates the FP64 iterate directly and folds clear xd: it is 0 here
+    if ((rc = efs_cg_finalize(s, pc != nullptr))) return rc;
     if (out) {
         out->iterations = fin.iter;
         out->rr = fin.gamma;

@@ -41,54 +41,6 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// Block sum deposited in partials[k * gridDim + block]; the last block to arrive reduces all
-// partials in a fixed order (deterministic).  Returns true in the last block (all threads).
-template <int NT, int NV>
-__device__ inline bool grid_reduce(double (&v)[NV], double* partials, unsigned int* counter, double (&out)[NV]) {
-    __shared__ double red[NV][NT / 64];
-    __shared__ bool last;
-    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        const double s = wave_sum(v[k]);
-        if ((lane & 63) == 0) red[k][lane >> 6] = s;
-    }
-    __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = 0;
-            for (int wv = 0; wv < NT / 64; wv++) s += red[k][wv];
-            partials[(size_t)k * gridDim.x + blockIdx.x] = s;
-        }
-        __threadfence();
-        const unsigned int t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double s = 0;
-        for (int i = lane; i < (int)gridDim.x; i += NT) s += __builtin_nontemporal_load(&partials[(size_t)k * gridDim.x + i]);
-        s = wave_sum(s);
-        __syncthreads();
-        if ((lane & 63) == 0) red[k][lane >> 6] = s;
-    }
-    __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = 0;
-            for (int wv = 0; wv < NT / 64; wv++) s += red[k][wv];
-            out[k] = s;
-        }
-        *counter = 0;
-    }
-    return true;
-}
-
 // Row of the operator.  The reference sums the contributing neighbours in the order
 // x-, x+, y-, y+, z+, z- starting from 0.0f, then subtracts nCount * x (no contraction).
 __device__ inline float apply_row(unsigned rc, float xc, float xm, float xp, float ym, float yp, float zp, float zm) {
