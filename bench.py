@@ -30,14 +30,14 @@ HBM_PEAK_GBS = 8000.0
 K1_BYTES = 2 + 4 + 4 + 4      # row code, r, d_{i-1} read; d_i written
 K2_BYTES = 2 + 4 + 4 + 4 + 4 + 4  # row code, d_i, x_i, r_i read; x_{i+1}, r_{i+1} written
 K2_BYTES_PC = 2 + 4 + 8 + 4 + 8 + 4  # preconditioned CG: the FP64 iterate is read and written
-# the fused fine-level up sweep of the multigrid V-cycle (black then red pass in one kernel): row
-# code and right-hand side read, the red cells' parent correction read from the coarse level
-# (1/8 of the cells), z written
-MG_UP_BYTES = 2 + 4 + 4 / 8 + 4
-# the fused fine-level down sweep: row code and right-hand side read, the restricted residual
-# written to the coarse level (1/8 of the cells)
-MG_DOWN_BYTES = 2 + 4 + 4 / 8
-
+# the fine level of the multigrid V(2,2)-cycle, event-timed as two brackets (bytes per cell):
+# up: first sweep (row code 2 + b 4 + stored red 4 + parents' correction 4/8 read, x written 4)
+# and second sweep (row code 2 + b 4 + stored red 4 read, x written 4)
+MG_UP_BYTES = (2 + 4 + 4 + 4 / 8 + 4) + (2 + 4 + 4 + 4)
+# down: first sweep from zero (row code 2 + b 4 read, x written 4), second sweep (row code 2 +
+# b 4 + stored black 4 read, x written 4), residual restriction (row code 2 + b 4 + x 4 read,
+# coarse b written 4/8)
+MG_DOWN_BYTES = (2 + 4 + 4) + (2 + 4 + 4 + 4) + (2 + 4 + 4 + 4 / 8)
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_sphere_pool_256_mg.md")
 

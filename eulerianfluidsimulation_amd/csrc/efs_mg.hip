@@ -49,6 +49,9 @@
 #ifndef EFS_MG_FUSED_RZ
 #define EFS_MG_FUSED_RZ 1
 #endif
+#ifndef EFS_MG_NU
+#define EFS_MG_NU 2  // red-black Gauss-Seidel sweeps down and up per level: V(1,1) or V(2,2)
+#endif
 
 namespace {
 
@@ -62,6 +65,7 @@ struct Lvl {
     uint8_t* cls;    // cell class (levels >= 1)
     float* x;        // correction (level 0: z)
     float* b;        // right-hand side (level 0: the CG residual)
+    float* x2;       // second buffer of the V(2,2) sweeps
 };
 
 struct MG {
@@ -180,10 +184,25 @@ constexpr int NF = (PX / 2) * PY, NB = ((PX - 2) / 2) * (PY - 2), NT2 = (TX / 2)
 constexpr int NC = (TX / 2) * (PTY / 2);  // coarse cells of a tile plane pair
 static_assert(NF <= POST_NT, "one fetch pair per lane");
 static_assert(PTY % 2 == 0, "tiles cover whole coarse cells");
-enum { SW_UP = 0, SW_UP_RZ = 1, SW_DOWN = 2 };
+// Modes.  C0 is the colour whose values enter from the commit (halo 2), C1 the colour updated
+// first (halo 1); the finish pass then updates C0 on the tile (or, SW_DOWN, takes the residuals;
+// SW_DOWN0, stores the first pass's C0 values).
+//   V(1,1) cycle: SW_DOWN, then SW_UP / SW_UP_RZ (C0 red, recomputed from b; C1 black).
+//   V(2,2) cycle, x read from `xin` and written to `xout` (another buffer, so that no block reads
+//   values a neighbouring block has already replaced):
+//     SW_DOWN0  first down sweep from 0 (red from b, then black), stores both colours;
+//     SW_RB     second down sweep (red from the stored black, then black), stores both;
+//     then k_mg_restrict_x restricts the residual of the stored x;
+//     SW_BR_E   first up sweep (black from the stored red plus its parent's correction, then red);
+//     SW_BR / SW_BR_RZ  second up sweep (black from the stored red, then red).
+enum { SW_UP = 0, SW_UP_RZ = 1, SW_DOWN = 2, SW_DOWN0 = 3, SW_RB = 4, SW_BR_E = 5, SW_BR = 6, SW_BR_RZ = 7 };
 template <int MODE>
-__global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s, Dom d, const CGState* st, double* p_rz) {
-    constexpr bool UP = MODE != SW_DOWN, RZ = MODE == SW_UP_RZ;
+__global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s, Dom d, const CGState* st, double* p_rz,
+                                                      const float* __restrict__ xin, float* __restrict__ xout) {
+    constexpr bool UP = MODE != SW_DOWN, RZ = MODE == SW_UP_RZ || MODE == SW_BR_RZ;
+    constexpr bool USE_E = MODE == SW_UP || MODE == SW_UP_RZ || MODE == SW_BR_E;
+    constexpr bool FROM_X = MODE == SW_RB || MODE == SW_BR_E || MODE == SW_BR || MODE == SW_BR_RZ;
+    constexpr int C1 = MODE == SW_RB ? 0 : 1, C0 = 1 - C1;
     if (st->done) return;
     __shared__ float ro[3][PP], bn[3][PP], bs[4][PP];
     __shared__ uint16_t rs[4][PP];
@@ -218,7 +237,7 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
 
         struct Fetched {
             unsigned rc0, rc1;
-            float b0, b1, e;
+            float b0, b1, e, xv;
         };
         auto fetch = [&](int p, Fetched& f) {  // loads of plane p (clamped into the array)
             const int pz = min(max(p, 0), l.D - 1);
@@ -227,18 +246,17 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
             f.rc1 = l.rc[zo + fo1];
             f.b0 = l.b[zo + fo0];
             f.b1 = l.b[zo + fo1];
-            if (UP) {
-                const bool first_red = ((fxu + fyu + p) & 1) == 0;
-                f.e = cl.x[((pz + 1) >> 1) * cplane + (first_red ? fc0 : fc1)];
-            }
+            const bool first_c0 = ((fxu + fyu + p) & 1) == C0;
+            if (USE_E) f.e = cl.x[((pz + 1) >> 1) * cplane + (first_c0 ? fc0 : fc1)];
+            if (FROM_X) f.xv = xin[zo + (first_c0 ? fo0 : fo1)];
         };
         auto commit = [&](int p, const Fetched& f) {  // into the ring slots of plane p
             if (!fl) return;
-            const bool first_red = ((fxu + fyu + p) & 1) == 0;
-            const unsigned rc = first_red ? f.rc0 : f.rc1;
-            float v = row_update(rc, 0.f, first_red ? f.b0 : f.b1, inv_s);
-            if (UP) v = v + f.e;
-            ro[(p + 3) % 3][fk + (first_red ? 0 : 1)] = (rc & RC_CONTRIB) ? v : 0.f;
+            const bool first_c0 = ((fxu + fyu + p) & 1) == C0;
+            const unsigned rc = first_c0 ? f.rc0 : f.rc1;
+            float v = FROM_X ? f.xv : row_update(rc, 0.f, first_c0 ? f.b0 : f.b1, inv_s);
+            if (USE_E) v = v + f.e;
+            ro[(p + 3) % 3][fk + (first_c0 ? 0 : 1)] = (rc & RC_CONTRIB) ? v : 0.f;
             bs[(p + 4) & 3][fk] = f.b0;
             bs[(p + 4) & 3][fk + 1] = f.b1;
             rs[(p + 4) & 3][fk] = (uint16_t)f.rc0;
@@ -249,9 +267,9 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
             const float ax = l.s * (sum - (float)((rc >> RC_CNT_SHIFT) & 7u) * x);
             return (rc & RC_CG_ROW) ? b - ax : 0.f;
         };
-        auto black = [&](int p) {  // black value of the lane's pair in plane p
+        auto black = [&](int p) {  // the C1 value of the lane's pair in plane p
             if (!blk) return;
-            const int q = ((bxu + byu + p) & 1) == 1 ? 0 : 1;
+            const int q = ((bxu + byu + p) & 1) == C1 ? 0 : 1;
             const int k = bk + q, km = k - 1, kp = k + 1, x = bxu + q;
             const float *r0 = ro[(p + 2) % 3], *r1 = ro[(p + 3) % 3], *r2 = ro[(p + 4) % 3];
             const unsigned rc = rs[(p + 4) & 3][k];
@@ -263,26 +281,28 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
             const bool tile = brow_tile && x >= x0 && x < x0 + TX && p >= zb && p < ze;
             if (UP) {
                 if (tile && inside) {
-                    l.x[p * plane + byu * l.W + x] = val;
+                    xout[p * plane + byu * l.W + x] = val;
                     if (RZ) acc += (double)b * (double)val;
                 }
             } else if (tile) {
                 res[p & 3][(byu - y0) * TX + (x - x0)] = inside ? residual(rc, sum, val, b) : 0.f;
             }
         };
-        auto finish = [&](int q) {  // red value (up) or red residual (down) of the lane's pair in plane q
+        auto finish = [&](int q) {  // the C0 value (or its residual, SW_DOWN) of the lane's pair in plane q
             if (!fin) return;
-            const int h = ((txu + tyu + q) & 1) == 0 ? 0 : 1;
+            const int h = ((txu + tyu + q) & 1) == C0 ? 0 : 1;
             const int k = tk + h, km = k - 1, kp = k + 1, x = txu + h;
             const bool inside = x <= l.nx && tyu <= l.ny;
             const float *n0 = bn[(q + 2) % 3], *n1 = bn[(q + 3) % 3], *n2 = bn[(q + 4) % 3];
             const unsigned rc = rs[(q + 4) & 3][k];
             const float b = bs[(q + 4) & 3][k];
             const float sum = 0.f + n1[km] + n1[kp] + n1[k - PX] + n1[k + PX] + n2[k] + n0[k];
-            if (UP) {
+            if (MODE == SW_DOWN0) {
+                if (inside) xout[q * plane + tyu * l.W + x] = row_update(rc, 0.f, b, inv_s);
+            } else if (UP) {
                 if (!inside) return;
                 const float val = row_update(rc, sum, b, inv_s);
-                l.x[q * plane + tyu * l.W + x] = val;
+                xout[q * plane + tyu * l.W + x] = val;
                 if (RZ) acc += (double)b * (double)val;
             } else {
                 res[q & 3][(tyu - y0) * TX + (x - x0)] = inside ? residual(rc, sum, row_update(rc, 0.f, b, inv_s), b) : 0.f;
@@ -350,6 +370,31 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
     }
 }
 
+// residual b - s A x of level f (x stored, FP32) restricted to level c: b_c = 1/8 of the sum of
+// the fluid children's residuals (child order x, y, z), 0 on non-fluid coarse cells
+__global__ __launch_bounds__(256) void k_mg_restrict_x(Lvl f, const float* __restrict__ xf, Lvl c, const CGState* st) {
+    if (st->done) return;
+    const int x = 1 + blockIdx.x * MB + threadIdx.x, y = 1 + blockIdx.y * MBY + threadIdx.y, z = 1 + blockIdx.z;
+    if (x > c.nx || y > c.ny) return;
+    const int ci = lidx(c, x, y, z);
+    float acc = 0.f;
+    if (c.cls[ci] == CLS_FLUID) {
+        const int sy = f.W, sz = f.W * f.H;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int fx = 2 * x - 1 + (k & 1), fy = 2 * y - 1 + ((k >> 1) & 1), fz = 2 * z - 1 + (k >> 2);
+            if (fx > f.nx || fy > f.ny || fz > f.nz) continue;
+            const int i = lidx(f, fx, fy, fz);
+            const unsigned rc = f.rc[i];
+            const float v[6] = {xf[i - 1], xf[i + 1], xf[i - sy], xf[i + sy], xf[i + sz], xf[i - sz]};
+            const float ax = f.s * (masked_sum6(rc, v) - (float)((rc >> RC_CNT_SHIFT) & 7u) * xf[i]);
+            acc += (rc & RC_CG_ROW) ? f.b[i] - ax : 0.f;
+        }
+        acc *= 0.125f;
+    }
+    c.b[ci] = acc;
+}
+
 // coarsest level: symmetric red-black sweeps from x = 0 in one workgroup, on a copy of the level
 // (values, right-hand side, row codes, with the border) held in LDS
 constexpr int MG_CO_MAX = 1728;  // (10 + 2)^3: interiors up to 10 cells per axis
@@ -388,7 +433,9 @@ inline dim3 cell_grid(const Lvl& l) { return dim3((l.W + MB - 1) / MB, (l.H + MB
 
 void free_mg(MG* m) {
     if (!m) return;
-    for (size_t k = 1; k < m->L.size(); k++) {
+    for (size_t k = 0; k < m->L.size(); k++) {
+        if (m->L[k].x2) hipFree(m->L[k].x2);
+        if (k == 0) continue;  // level 0 uses the solver's buffers
         hipFree(m->L[k].rc);
         hipFree(m->L[k].cls);
         hipFree(m->L[k].x);
@@ -419,6 +466,11 @@ int ensure(efs_sim* s) {
     l0.cls = nullptr;
     l0.x = m->z;
     l0.b = s->cg_r;
+    l0.x2 = nullptr;
+    if (EFS_MG_NU == 2) {
+        EFS_HIP(hipMalloc(&l0.x2, (size_t)s->N * 4));
+        EFS_HIP(hipMemset(l0.x2, 0, (size_t)s->N * 4));
+    }
     m->L.push_back(l0);
     while (true) {
         const Lvl& f = m->L.back();
@@ -434,6 +486,11 @@ int ensure(efs_sim* s) {
         EFS_HIP(hipMalloc(&c.b, n * 4));
         EFS_HIP(hipMemset(c.x, 0, n * 4));  // the border stays 0
         EFS_HIP(hipMemset(c.b, 0, n * 4));
+        c.x2 = nullptr;
+        if (EFS_MG_NU == 2) {
+            EFS_HIP(hipMalloc(&c.x2, n * 4));
+            EFS_HIP(hipMemset(c.x2, 0, n * 4));
+        }
         m->L.push_back(c);
     }
     const Lvl& cl = m->L.back();
@@ -482,38 +539,55 @@ Dom post_dom(const Lvl& l) {
     return d;
 }
 
-// z = M r: one V-cycle.  With fused_rz the fine up sweep also writes the r . z partials.
+// z = M r: one V-cycle.  With fused_rz the fine up sweep also writes the r . z partials.  The fine
+// down and up sweeps are event-timed when kernel timing is on.
 int vcycle(efs_sim* s, void*) {
     MG* m = get(s);
     const int nl = (int)m->L.size();
     const CGState* st = s->st;
     const dim3 pb(TX, PTY);
+    hipEvent_t ta = nullptr, tb = nullptr;
     for (int k = 0; k < nl - 1; k++) {
         const Lvl& l = m->L[k];
+        const Lvl& c = m->L[k + 1];
         const Dom d = post_dom(l);
-        hipEvent_t a = nullptr, b = nullptr;
-        if (k == 0) efs_ktimer_begin(s, KT_MG_DOWN, &a, &b);  // the fine down sweep is event-timed too
-        hipLaunchKernelGGL(k_mg_sweep<SW_DOWN>, dim3(launch_blocks(d)), pb, 0, s->stream, l, m->L[k + 1], 1.f / l.s, d, st,
-                           nullptr);
-        if (k == 0) efs_ktimer_end(s, KT_MG_DOWN, a, b);
+        const dim3 grid(launch_blocks(d));
+        const float is = 1.f / l.s;
+        if (k == 0) efs_ktimer_begin(s, KT_MG_DOWN, &ta, &tb);
+        if (EFS_MG_NU == 2) {
+            hipLaunchKernelGGL(k_mg_sweep<SW_DOWN0>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, nullptr, l.x2);
+            hipLaunchKernelGGL(k_mg_sweep<SW_RB>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, l.x2, l.x);
+            hipLaunchKernelGGL(k_mg_restrict_x, dim3((c.nx + MB - 1) / MB, (c.ny + MBY - 1) / MBY, c.nz), dim3(MB, MBY), 0, s->stream,
+                               l, l.x, c, st);
+        } else {
+            hipLaunchKernelGGL(k_mg_sweep<SW_DOWN>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, nullptr, nullptr);
+        }
+        if (k == 0) efs_ktimer_end(s, KT_MG_DOWN, ta, tb);
     }
     hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(MG_CO_NT), 0, s->stream, m->L[nl - 1], 1.f / m->L[nl - 1].s,
                        EFS_MG_COARSE_SWEEPS, st);
     for (int k = nl - 2; k >= 0; k--) {
         const Lvl& l = m->L[k];
+        const Lvl& c = m->L[k + 1];
         const Dom d = post_dom(l);
         const dim3 grid(launch_blocks(d));
-        if (k == 0) {  // the fine up sweep is event-timed when kernel timing is on
-            hipEvent_t a = nullptr, b = nullptr;
-            efs_ktimer_begin(s, KT_OTHER, &a, &b);
-            if (m->fused_rz)
-                hipLaunchKernelGGL(k_mg_sweep<SW_UP_RZ>, grid, pb, 0, s->stream, l, m->L[1], 1.f / l.s, d, st, m->p_rz);
+        const float is = 1.f / l.s;
+        const bool rz = k == 0 && m->fused_rz;
+        double* prz = rz ? m->p_rz : nullptr;
+        if (k == 0) efs_ktimer_begin(s, KT_OTHER, &ta, &tb);
+        if (EFS_MG_NU == 2) {
+            hipLaunchKernelGGL(k_mg_sweep<SW_BR_E>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, l.x, l.x2);
+            if (rz)
+                hipLaunchKernelGGL(k_mg_sweep<SW_BR_RZ>, grid, pb, 0, s->stream, l, c, is, d, st, prz, l.x2, l.x);
             else
-                hipLaunchKernelGGL(k_mg_sweep<SW_UP>, grid, pb, 0, s->stream, l, m->L[1], 1.f / l.s, d, st, nullptr);
-            efs_ktimer_end(s, KT_OTHER, a, b);
+                hipLaunchKernelGGL(k_mg_sweep<SW_BR>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, l.x2, l.x);
         } else {
-            hipLaunchKernelGGL(k_mg_sweep<SW_UP>, grid, pb, 0, s->stream, l, m->L[k + 1], 1.f / l.s, d, st, nullptr);
+            if (rz)
+                hipLaunchKernelGGL(k_mg_sweep<SW_UP_RZ>, grid, pb, 0, s->stream, l, c, is, d, st, prz, nullptr, l.x);
+            else
+                hipLaunchKernelGGL(k_mg_sweep<SW_UP>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, nullptr, l.x);
         }
+        if (k == 0) efs_ktimer_end(s, KT_OTHER, ta, tb);
     }
     EFS_HIP(hipGetLastError());
     return 0;
