@@ -53,6 +53,8 @@ __host__ __device__ inline int cidx(const DevGrid& g, int x, int y, int z) { ret
 
 template <class T> __device__ inline T clampv(T v, T lo, T hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
 
+constexpr int kBufDword3 = 0x00020000;  // gfx9 buffer descriptor word 3 for raw access
+
 __device__ inline long long to_fix(float v) { return (long long)rint((double)v * FIX_SCALE); }
 __device__ inline void fix_add(unsigned long long* acc, int i, float v) {
     const long long q = to_fix(v);

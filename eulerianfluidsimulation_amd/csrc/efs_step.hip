@@ -263,15 +263,48 @@ __global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __r
 }
 
 // Step 5: forward trace for columns with beta < 1; the new entries are scattered into the
-// row sums of step 6 (gamma_acc).
-__global__ __launch_bounds__(256) void k_adv_fwd(DevGrid g, float dt, const unsigned long long* __restrict__ beta_acc,
-                                                 int* __restrict__ fq0, float* __restrict__ fw, unsigned long long* gamma_acc) {
-    INTERIOR_XYZ(g);
-    const float bj = (float)from_fix(beta_acc[c]);
-    if (bj >= 1) {
-        fq0[c] = -1;
-        return;
+// row sums of step 6 (gamma_acc).  A third to a half of the cells have beta < 1, spread over
+// most waves, so each workgroup (a 64 x FWD_ROWS tile) first packs its tracing cells into the
+// lowest lanes through LDS and the waves left without a cell exit.  Every cell's work is
+// independent of the lane it lands on and the scatters are fixed-point sums, so the results do
+// not depend on the packing order.
+#ifndef EFS_FWD_ROWS
+#define EFS_FWD_ROWS 8
+#endif
+constexpr int FWD_ROWS = EFS_FWD_ROWS;
+__global__ __launch_bounds__(64 * FWD_ROWS) void k_adv_fwd(DevGrid g, float dt, const unsigned long long* __restrict__ beta_acc,
+                                                           int* __restrict__ fq0, float* __restrict__ fw,
+                                                           unsigned long long* gamma_acc) {
+    __shared__ unsigned short s_slot[64 * FWD_ROWS];
+    __shared__ float s_bj[64 * FWD_ROWS];
+    __shared__ int s_n;
+    const int lane = threadIdx.x, t = threadIdx.x + 64 * threadIdx.y;
+    if (t == 0) s_n = 0;
+    __syncthreads();
+    const int x0 = 1 + blockIdx.x * 64, y0 = 1 + blockIdx.y * FWD_ROWS, z = 1 + blockIdx.z;
+    bool act = false;
+    float bj = 0.f;
+    if (x0 + lane <= g.W - 2 && y0 + (int)threadIdx.y <= g.H - 2) {
+        const int c = cidx(g, x0 + lane, y0 + threadIdx.y, z);
+        bj = (float)from_fix(beta_acc[c]);
+        act = bj < 1;
+        if (!act) fq0[c] = -1;
     }
+    const unsigned long long bal = __ballot(act);
+    int base = 0;
+    if (lane == 0 && bal) base = atomicAdd(&s_n, __popcll(bal));
+    base = __shfl(base, 0);
+    if (act) {
+        const int slot = base + __popcll(bal & ((1ull << lane) - 1ull));
+        s_slot[slot] = (unsigned short)t;
+        s_bj[slot] = bj;
+    }
+    __syncthreads();
+    if (t >= s_n) return;
+    const int src = s_slot[t];
+    bj = s_bj[t];
+    const int x = x0 + (src & 63), y = y0 + (src >> 6);
+    const int c = cidx(g, x, y, z);
     const float3 next = traceParticle(g, (float)x, (float)y, (float)z, +dt, 4);
     float m[8];
     int q0;
@@ -399,81 +432,95 @@ __global__ __launch_bounds__(256) void k_adv_final(DevGrid g, const int* __restr
 }
 
 // Step 12: one pair (j = lower neighbour, i = cell) of the in-place diffusion sweep
-// (CPUSimulator3D.cpp:281-303), carried in registers along the line.
+// (CPUSimulator3D.cpp:281-303), carried in registers along the line.  Written without branches:
+// the update is computed for every pair and kept only where the reference does not return early
+// (the same float operations on the kept path; the discarded one may divide by zero), since a
+// sweep runs about one wave per SIMD and every branch instruction lengthens its serial step.
 __device__ inline void diffuse_pair(float& rj, float& gj, unsigned bj, float& ri, float& gi, unsigned bi) {
-    if ((bi & BF_EQ1) || (bj & BF_EQ1)) return;
-    if (gj < 1e-5f || gi < 1e-5f) return;
-    if (gj == gi) return;
+    const bool skip = (((bi | bj) & BF_EQ1) != 0) | (gj < 1e-5f) | (gi < 1e-5f) | (gj == gi);
     const float avg = (gj + gi) / 2.f;
-    // mass moves from the cell of larger gamma (s) to the other (t); written on values, since
-    // branches that update rj / ri through their references make the compiler select between
-    // the two addresses and keep both in scratch memory
+    // mass moves from the cell of larger gamma (s) to the other (t)
     const bool jg = gj > gi;
     const float rs = jg ? rj : ri, gs = jg ? gj : gi, gt = jg ? gi : gj;
     const float dif = rs * (gs - gt) / (2.f * gs);
     const float nj = jg ? rj - dif : rj + dif;
     const float ni = jg ? ri + dif : ri - dif;
-    rj = nj;
-    ri = ni;
-    gj = avg;
-    gi = avg;
+    rj = skip ? rj : nj;
+    ri = skip ? ri : ni;
+    gj = skip ? gj : avg;
+    gi = skip ? gi : avg;
 }
 
 // Sweeps along y (AXIS 1, lines over (x, z)) and z (AXIS 2, lines over (x, y)): one thread per
-// line, consecutive threads on consecutive x (coalesced).  The loads of a line do not depend on
-// the sweep (cell i is read before anything writes it), so they are issued DF cells ahead in a
-// register ring indexed by compile-time constants.
+// line, consecutive threads on consecutive x (coalesced), DZ_LANES active lanes per wave.  The
+// loads of a line do not depend on the sweep (cell i is read before anything writes it), so they
+// are issued DF cells ahead in a register ring indexed by compile-time constants.  Every access
+// is a buffer access at the lane's line start (vector offset) plus the step's offset along the
+// line, which is the same for all lanes and kept in a scalar register.
 #ifndef EFS_DF
 #define EFS_DF 8
 #endif
-constexpr int DF = EFS_DF;
+#ifndef EFS_DZ_LANES
+#define EFS_DZ_LANES 64
+#endif
+constexpr int DF = EFS_DF, DZ_LANES = EFS_DZ_LANES;
 template <int AXIS>
 __global__ __launch_bounds__(64) void k_diffuse(DevGrid g) {
-    const int a = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // x
+    if ((int)threadIdx.x >= DZ_LANES) return;
+    const int a = 1 + blockIdx.x * DZ_LANES + threadIdx.x;  // x
     const int b = 1 + blockIdx.y;
     if (a > g.W - 2) return;
     int c0, stride, len;
     if (AXIS == 1) {  // b = z
-        c0 = cidx(g, a, 1, b); stride = g.W; len = g.H;
+        c0 = cidx(g, a, 0, b); stride = g.W; len = g.H;
     } else {          // b = y
-        c0 = cidx(g, a, b, 1); stride = g.W * g.H; len = g.D;
+        c0 = cidx(g, a, b, 0); stride = g.W * g.H; len = g.D;
     }
-    // cells 1 .. len-2 of the line; the pair (k-1, k) for k = 2 .. len-2
-    const int ncell = len - 2;
+    const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(g.rho, (short)0, g.N * 4, kBufDword3);
+    const __amdgpu_buffer_rsrc_t rg = __builtin_amdgcn_make_buffer_rsrc(g.gamma, (short)0, g.N * 4, kBufDword3);
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc((void*)g.bf, (short)0, g.N, kBufDword3);
+    const unsigned vf = (unsigned)c0 * 4u, vb = (unsigned)c0;
+    auto ldr = [&](int k) { return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, vf, k * stride * 4, 0)); };
+    auto ldg = [&](int k) { return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rg, vf, k * stride * 4, 0)); };
+    auto ldb = [&](int k) { return (unsigned)__builtin_amdgcn_raw_buffer_load_b8(rb, vb, k * stride, 0); };
+    auto st = [&](int k, float r, float gm) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, r), rr, vf, k * stride * 4, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, gm), rg, vf, k * stride * 4, 0);
+    };
+    // line positions 1 .. len-2; the pair (k-1, k) for k = 2 .. len-2
+    const int last = len - 2;
     float rq[DF], gq[DF];
     unsigned bq[DF];
 #pragma unroll
-    for (int k = 0; k < DF; k++) {
-        const int kk = min(1 + k, ncell - 1);  // cell index along the line (clamped: surplus loads unused)
-        rq[k] = g.rho[c0 + kk * stride];
-        gq[k] = g.gamma[c0 + kk * stride];
-        bq[k] = g.bf[c0 + kk * stride];
+    for (int q = 0; q < DF; q++) {
+        const int kk = min(2 + q, last);  // clamped: surplus loads are unused
+        rq[q] = ldr(kk);
+        gq[q] = ldg(kk);
+        bq[q] = ldb(kk);
     }
-    int cj = c0;
-    float rj = g.rho[cj], gj = g.gamma[cj];
-    unsigned bj = g.bf[cj];
-    int k = 1;  // next cell to pair with j (line position k)
-    while (k < ncell) {
+    float rj = ldr(1), gj = ldg(1);
+    unsigned bj = ldb(1);
+    int k = 2;  // line position of cell i, paired with j = k - 1
+    auto step = [&](int q) {
+        float ri = rq[q], gi = gq[q];
+        const unsigned bi = bq[q];
+        const int kn = min(k + DF, last);
+        rq[q] = ldr(kn);
+        gq[q] = ldg(kn);
+        bq[q] = ldb(kn);
+        diffuse_pair(rj, gj, bj, ri, gi, bi);
+        st(k - 1, rj, gj);  // cell j is final
+        rj = ri; gj = gi; bj = bi;
+        k++;
+    };
+    while (k + DF <= last + 1) {
 #pragma unroll
-        for (int q = 0; q < DF; q++) {
-            if (k < ncell) {
-                float ri = rq[q], gi = gq[q];
-                const unsigned bi = bq[q];
-                const int kn = min(k + DF, ncell - 1);
-                rq[q] = g.rho[c0 + kn * stride];
-                gq[q] = g.gamma[c0 + kn * stride];
-                bq[q] = g.bf[c0 + kn * stride];
-                diffuse_pair(rj, gj, bj, ri, gi, bi);
-                g.rho[cj] = rj;
-                g.gamma[cj] = gj;
-                cj += stride;
-                rj = ri; gj = gi; bj = bi;
-                k++;
-            }
-        }
+        for (int q = 0; q < DF; q++) step(q);
     }
-    g.rho[cj] = rj;
-    g.gamma[cj] = gj;
+#pragma unroll
+    for (int q = 0; q < DF; q++)
+        if (k <= last) step(q);
+    st(last, rj, gj);
 }
 
 // Sweep along x (lines over (y, z)): a line is contiguous, so consecutive threads would touch
@@ -482,7 +529,10 @@ __global__ __launch_bounds__(64) void k_diffuse(DevGrid g) {
 // of the tile, and the finished cells are written back coalesced.  The last cell of a tile is
 // still paired with the first of the next one, so it stays in registers and is stored on its
 // own.
-constexpr int DX_ROWS = 16, DX_TILE = 64;
+#ifndef EFS_DX_ROWS
+#define EFS_DX_ROWS 16
+#endif
+constexpr int DX_ROWS = EFS_DX_ROWS, DX_TILE = 64;
 __global__ __launch_bounds__(64) void k_diffuse_x(DevGrid g) {
     __shared__ float sr[DX_ROWS][DX_TILE + 1], sg[DX_ROWS][DX_TILE + 1];
     __shared__ uint8_t sb[DX_ROWS][DX_TILE + 4];
@@ -725,7 +775,8 @@ int efs_launch_rho_advect(efs_sim* s) {
     // steps 1-4
     hipLaunchKernelGGL(k_adv_bwd, gi, blk, 0, s->stream, g, dt, s->s1, s->bwd_p0, s->bwd_w, s->acc0, s->tu, s->tv, s->tw);
     // step 5 (+ forward half of step 6)
-    hipLaunchKernelGGL(k_adv_fwd, gi, blk, 0, s->stream, g, dt, s->acc0, s->fwd_p0, s->fwd_w, s->acc1);
+    hipLaunchKernelGGL(k_adv_fwd, dim3((s->W - 2 + 63) / 64, (s->H - 2 + FWD_ROWS - 1) / FWD_ROWS, s->D - 2),
+                       dim3(64, FWD_ROWS), 0, s->stream, g, dt, s->acc0, s->fwd_p0, s->fwd_w, s->acc1);
     hipLaunchKernelGGL(k_fill_u64, g1, b1, 0, s->stream, s->N, s->acc0);
     // steps 6-8: gamma6 -> s1 (the advected gamma of step 1 is no longer needed), new beta
     hipLaunchKernelGGL(k_adv_rows, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->s1, s->acc0);
@@ -741,8 +792,8 @@ int efs_launch_rho_advect(efs_sim* s) {
     const DevGrid g2 = s->grid();
     for (int it = 0; it < 3; it++) {
         hipLaunchKernelGGL(k_diffuse_x, dim3((s->H - 2 + DX_ROWS - 1) / DX_ROWS, s->D - 2), dim3(64), 0, s->stream, g2);
-        hipLaunchKernelGGL(k_diffuse<1>, dim3((s->W - 2 + 63) / 64, s->D - 2), dim3(64), 0, s->stream, g2);
-        hipLaunchKernelGGL(k_diffuse<2>, dim3((s->W - 2 + 63) / 64, s->H - 2), dim3(64), 0, s->stream, g2);
+        hipLaunchKernelGGL(k_diffuse<1>, dim3((s->W - 2 + DZ_LANES - 1) / DZ_LANES, s->D - 2), dim3(64), 0, s->stream, g2);
+        hipLaunchKernelGGL(k_diffuse<2>, dim3((s->W - 2 + DZ_LANES - 1) / DZ_LANES, s->H - 2), dim3(64), 0, s->stream, g2);
     }
     EFS_HIP(hipGetLastError());
     return 0;
