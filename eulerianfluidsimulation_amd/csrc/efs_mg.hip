@@ -182,7 +182,7 @@ constexpr int PX = TX + 4, PY = PTY + 4, PP = PX * PY;  // a plane of the tile w
 constexpr int POST_NT = TX * PTY;
 constexpr int NF = (PX / 2) * PY, NB = ((PX - 2) / 2) * (PY - 2), NT2 = (TX / 2) * PTY;  // pair lanes per phase
 constexpr int NC = (TX / 2) * (PTY / 2);  // coarse cells of a tile plane pair
-static_assert(NF <= POST_NT, "one fetch pair per lane");
+static_assert(POST_NT <= PP && PP <= 2 * POST_NT, "one or two fetch cells per lane");
 static_assert(PTY % 2 == 0, "tiles cover whole coarse cells");
 // Modes.  C0 is the colour whose values enter from the commit (halo 2), C1 the colour updated
 // first (halo 1); the finish pass then updates C0 on the tile (or, SW_DOWN, takes the residuals;
@@ -215,14 +215,23 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
         const int x0 = 1 + bx * TX, y0 = 1 + by * PTY;
         const int zb = 1 + bz * d.zc, ze = min(zb + d.zc, 1 + l.nz);  // planes [zb, ze)
         const int plane = l.W * l.H, cplane = cl.W * cl.H;
-        // fetch lanes: pair (hx, hx + 1), hx = 2 f, in row hy of the halo-2 plane
-        const bool fl = lane < NF;
-        const int fj = lane % (PX / 2), fhy = min(lane / (PX / 2), PY - 1);
-        const int fk = fhy * PX + 2 * fj;
-        const int fxu = x0 - 2 + 2 * fj, fyu = y0 - 2 + fhy;  // unclamped (colour)
-        const int fx0 = min(max(fxu, 0), l.W - 1), fx1 = min(max(fxu + 1, 0), l.W - 1), fy = min(max(fyu, 0), l.H - 1);
-        const int fo0 = fy * l.W + fx0, fo1 = fy * l.W + fx1;
-        const int fc0 = ((fy + 1) >> 1) * cl.W + ((fx0 + 1) >> 1), fc1 = ((fy + 1) >> 1) * cl.W + ((fx1 + 1) >> 1);
+        // fetch lanes: one cell of the halo-2 plane each, cells k = lane and k = lane + POST_NT in
+        // row-major order, so that every load of a plane is coalesced along the rows
+        int fk[2], fo[2], fc[2], fpar[2];
+        bool fv[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int k = lane + i * POST_NT;
+            fv[i] = k < PP;
+            const int kk = fv[i] ? k : 0;
+            const int hx = kk % PX, hy = kk / PX;
+            const int xu = x0 - 2 + hx, yu = y0 - 2 + hy;  // unclamped (colour)
+            const int fx = min(max(xu, 0), l.W - 1), fy = min(max(yu, 0), l.H - 1);
+            fk[i] = kk;
+            fo[i] = fy * l.W + fx;
+            fc[i] = ((fy + 1) >> 1) * cl.W + ((fx + 1) >> 1);
+            fpar[i] = (xu + yu) & 1;
+        }
         // black lanes: pair (hx, hx + 1), hx = 1 + 2 j, in row hy = 1 .. PY - 2
         const bool blk = lane < NB;
         const int bj = lane % ((PX - 2) / 2), bhy = 1 + min(lane / ((PX - 2) / 2), PY - 3);
@@ -236,48 +245,66 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
         const int txu = x0 + 2 * tj, tyu = y0 - 2 + thy;
 
         struct Fetched {
-            unsigned rc0, rc1;
-            float b0, b1, e, xv;
+            unsigned rc[2];
+            float b[2], e[2], xv[2];
         };
+        // buffer loads: the lane's cell offset is the vector offset, the plane's the scalar one
+        const int nb = l.W * l.H * l.D;
+        const __amdgpu_buffer_rsrc_t r_rc = __builtin_amdgcn_make_buffer_rsrc(l.rc, (short)0, nb * 2, kBufDword3);
+        const __amdgpu_buffer_rsrc_t r_b = __builtin_amdgcn_make_buffer_rsrc(l.b, (short)0, nb * 4, kBufDword3);
+        const __amdgpu_buffer_rsrc_t r_x = __builtin_amdgcn_make_buffer_rsrc((void*)xin, (short)0, nb * 4, kBufDword3);
+        const __amdgpu_buffer_rsrc_t r_e =
+            __builtin_amdgcn_make_buffer_rsrc(cl.x, (short)0, cl.W * cl.H * cl.D * 4, kBufDword3);
         auto fetch = [&](int p, Fetched& f) {  // loads of plane p (clamped into the array)
             const int pz = min(max(p, 0), l.D - 1);
-            const int zo = pz * plane;
-            f.rc0 = l.rc[zo + fo0];
-            f.rc1 = l.rc[zo + fo1];
-            f.b0 = l.b[zo + fo0];
-            f.b1 = l.b[zo + fo1];
-            const bool first_c0 = ((fxu + fyu + p) & 1) == C0;
-            if (USE_E) f.e = cl.x[((pz + 1) >> 1) * cplane + (first_c0 ? fc0 : fc1)];
-            if (FROM_X) f.xv = xin[zo + (first_c0 ? fo0 : fo1)];
+            const int zo = pz * plane, czo = ((pz + 1) >> 1) * cplane;
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                if (i == 1 && !fv[1]) continue;  // every lane has a first cell (POST_NT <= PP)
+                f.rc[i] = __builtin_amdgcn_raw_buffer_load_b16(r_rc, (unsigned)fo[i] * 2u, zo * 2, 0);
+                f.b[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_b, (unsigned)fo[i] * 4u, zo * 4, 0));
+                if (USE_E)
+                    f.e[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_e, (unsigned)fc[i] * 4u, czo * 4, 0));
+                if (FROM_X)
+                    f.xv[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_x, (unsigned)fo[i] * 4u, zo * 4, 0));
+            }
         };
-        auto commit = [&](int p, const Fetched& f) {  // into the ring slots of plane p
-            if (!fl) return;
-            const bool first_c0 = ((fxu + fyu + p) & 1) == C0;
-            const unsigned rc = first_c0 ? f.rc0 : f.rc1;
-            float v = FROM_X ? f.xv : row_update(rc, 0.f, first_c0 ? f.b0 : f.b1, inv_s);
-            if (USE_E) v = v + f.e;
-            ro[(p + 3) % 3][fk + (first_c0 ? 0 : 1)] = (rc & RC_CONTRIB) ? v : 0.f;
-            bs[(p + 4) & 3][fk] = f.b0;
-            bs[(p + 4) & 3][fk + 1] = f.b1;
-            rs[(p + 4) & 3][fk] = (uint16_t)f.rc0;
-            rs[(p + 4) & 3][fk + 1] = (uint16_t)f.rc1;
+        // Ring slots: plane p sits in slot p mod 3 of the three-plane rings (red, black) and p mod 4
+        // of the four-plane rings (b, codes); the mod-3 slots are carried along the march.
+        auto commit = [&](int p, const Fetched& f, int sp) {  // into the ring slots of plane p (sp: its mod-3 slot)
+            float* ro_p = ro[sp];
+            float* bs_p = bs[p & 3];
+            uint16_t* rs_p = rs[p & 3];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                if (i == 1 && !fv[1]) continue;
+                const unsigned rc = f.rc[i];
+                float v = FROM_X ? f.xv[i] : row_update(rc, 0.f, f.b[i], inv_s);
+                if (USE_E) v = v + f.e[i];
+                // only the C0 cells of the red ring are read (the neighbours of C1 cells)
+                const bool c0 = ((fpar[i] + p) & 1) == C0;
+                ro_p[fk[i]] = (c0 && (rc & RC_CONTRIB)) ? v : 0.f;
+                bs_p[fk[i]] = f.b[i];
+                rs_p[fk[i]] = (uint16_t)rc;
+            }
         };
         // residual of a fluid row after the down sweep (b - s (sum - n x)), else 0
         auto residual = [&](unsigned rc, float sum, float x, float b) -> float {
             const float ax = l.s * (sum - (float)((rc >> RC_CNT_SHIFT) & 7u) * x);
             return (rc & RC_CG_ROW) ? b - ax : 0.f;
         };
-        auto black = [&](int p) {  // the C1 value of the lane's pair in plane p
+        // the C1 value of the lane's pair in plane p; sm, s, sp: the mod-3 slots of planes p - 1, p, p + 1
+        auto black = [&](int p, int sm, int s, int sp) {
             if (!blk) return;
             const int q = ((bxu + byu + p) & 1) == C1 ? 0 : 1;
             const int k = bk + q, km = k - 1, kp = k + 1, x = bxu + q;
-            const float *r0 = ro[(p + 2) % 3], *r1 = ro[(p + 3) % 3], *r2 = ro[(p + 4) % 3];
-            const unsigned rc = rs[(p + 4) & 3][k];
-            const float b = bs[(p + 4) & 3][k];
+            const float *r0 = ro[sm], *r1 = ro[s], *r2 = ro[sp];
+            const unsigned rc = rs[p & 3][k];
+            const float b = bs[p & 3][k];
             const float sum = 0.f + r1[km] + r1[kp] + r1[k - PX] + r1[k + PX] + r2[k] + r0[k];
             const bool inside = x >= 1 && x <= l.nx && byu >= 1 && byu <= l.ny && p >= 1 && p <= l.nz;
             const float val = inside ? row_update(rc, sum, b, inv_s) : 0.f;
-            bn[(p + 3) % 3][k] = (rc & RC_CONTRIB) ? val : 0.f;
+            bn[s][k] = (rc & RC_CONTRIB) ? val : 0.f;
             const bool tile = brow_tile && x >= x0 && x < x0 + TX && p >= zb && p < ze;
             if (UP) {
                 if (tile && inside) {
@@ -288,14 +315,16 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
                 res[p & 3][(byu - y0) * TX + (x - x0)] = inside ? residual(rc, sum, val, b) : 0.f;
             }
         };
-        auto finish = [&](int q) {  // the C0 value (or its residual, SW_DOWN) of the lane's pair in plane q
+        // the C0 value (or its residual, SW_DOWN) of the lane's pair in plane q; tm, t, tp: the mod-3
+        // slots of planes q - 1, q, q + 1
+        auto finish = [&](int q, int tm, int t, int tp) {
             if (!fin) return;
             const int h = ((txu + tyu + q) & 1) == C0 ? 0 : 1;
             const int k = tk + h, km = k - 1, kp = k + 1, x = txu + h;
             const bool inside = x <= l.nx && tyu <= l.ny;
-            const float *n0 = bn[(q + 2) % 3], *n1 = bn[(q + 3) % 3], *n2 = bn[(q + 4) % 3];
-            const unsigned rc = rs[(q + 4) & 3][k];
-            const float b = bs[(q + 4) & 3][k];
+            const float *n0 = bn[tm], *n1 = bn[t], *n2 = bn[tp];
+            const unsigned rc = rs[q & 3][k];
+            const float b = bs[q & 3][k];
             const float sum = 0.f + n1[km] + n1[kp] + n1[k - PX] + n1[k + PX] + n2[k] + n0[k];
             if (MODE == SW_DOWN0) {
                 if (inside) xout[q * plane + tyu * l.W + x] = row_update(rc, 0.f, b, inv_s);
@@ -331,15 +360,17 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
         // alternating with the plane).  The barriers order LDS accesses only, so they do not wait
         // for those loads (a full __syncthreads would drain them).
         Fetched fa, fb;
-        auto body = [&](int p, Fetched& f) {
-            commit(p + 1, f);  // ring slots of planes p - 2 (red) and p - 3 (b, codes): no longer read
+        auto next3 = [](int m) { return m == 2 ? 0 : m + 1; };
+        auto body = [&](int p, Fetched& f, int m) {  // m = p mod 3
+            const int mp = next3(m), mm = m == 0 ? 2 : m - 1;
+            commit(p + 1, f, mp);  // ring slots of planes p - 2 (red) and p - 3 (b, codes): no longer read
             if (p + 3 <= ze + 1) fetch(p + 3, f);
             lds_barrier();
-            black(p);
+            black(p, mm, m, mp);
             lds_barrier();
             const int q = p - 1;
             if (q >= zb) {
-                finish(q);
+                finish(q, mp, mm, m);  // planes q - 1 = p - 2 (slot of p + 1), q, q + 1
                 if (!UP && (((q - zb) & 1) == 1 || q == ze - 1)) {
                     lds_barrier();
                     restrict_pair(q - ((q - zb) & 1));
@@ -347,14 +378,17 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
             }
         };
         fetch(zb - 2, fa);
-        commit(zb - 2, fa);
+        commit(zb - 2, fa, (zb + 1) % 3);
         fetch(zb - 1, fb);
-        commit(zb - 1, fb);
+        commit(zb - 1, fb, (zb + 2) % 3);
         fetch(zb, fa);
         fetch(zb + 1, fb);
+        int m = (zb - 1) % 3;  // zb >= 1
         for (int p = zb - 1; p <= ze; p += 2) {
-            body(p, fa);
-            if (p + 1 <= ze) body(p + 1, fb);
+            body(p, fa, m);
+            m = next3(m);
+            if (p + 1 <= ze) body(p + 1, fb, m);
+            m = next3(m);
         }
     }
     if (RZ) {  // every launched block writes its partial (0 when idle)
