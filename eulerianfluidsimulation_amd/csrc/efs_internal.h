@@ -258,9 +258,10 @@ struct efs_sim {
     uint8_t* bf = nullptr;
     // scratch
     float *tu = nullptr, *tv = nullptr, *tw = nullptr;   // tmpVec (cell-centred advected velocity)
-    // rho_advect leaves its backward traces (the positions traceParticle returns for every
-    // interior cell centre) in tu/tv/tw; vel_advect traces the same cells with the same velocity
-    // and walls, so it reuses them when nothing has changed the velocity, the walls or dt since
+    // rho_advect leaves in tu/tv/tw the velocity sampled at its backward traces from every
+    // interior cell centre; vel_advect traces the same cells with the same velocity, walls and dt
+    // and samples the same velocity there, so it reuses them when no other stage or upload has
+    // run in between (every other stage and every upload clears the flag)
     bool traces_valid = false;
     float *s0 = nullptr, *s1 = nullptr;                   // scalar scratch
     uint8_t *valA = nullptr, *valB = nullptr;             // extrapolation validity

@@ -235,14 +235,18 @@ __global__ void k_add_fix(int n, float* __restrict__ f, unsigned long long* __re
 // bwd_w[k * N + i] = A(i, p0_i + corner k) for row i; fwd_w[k * N + j] = A(q0_j + corner k, j).
 
 // Steps 1-4: backward trace, advected gamma, row weights scaled by it, column sums (beta).
+// Also samples the velocity at the traced position, as vel_advect does from the same trace
+// (CPUSimulator3D.cpp:486-493), into su/sv/sw: vel_advect reuses those samples when it directly
+// follows with the velocity unchanged.
 __global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __restrict__ gadv, int* __restrict__ bp0,
-                                                 float* __restrict__ bw, unsigned long long* beta_acc, float* __restrict__ px,
-                                                 float* __restrict__ py, float* __restrict__ pz) {
+                                                 float* __restrict__ bw, unsigned long long* beta_acc, float* __restrict__ su,
+                                                 float* __restrict__ sv, float* __restrict__ sw) {
     INTERIOR_XYZ(g);
     const float3 prev = traceParticle(g, (float)x, (float)y, (float)z, -dt, 4);
-    px[c] = prev.x;
-    py[c] = prev.y;
-    pz[c] = prev.z;
+    const float3 vs = vinterp(g, prev.x - 0.5f, prev.y - 0.5f, prev.z - 0.5f);
+    su[c] = vs.x;
+    sv[c] = vs.y;
+    sw[c] = vs.z;
     const float ga = (g.bf[c] & BF_EQ1) ? 0.f : sinterp(g, g.gamma, prev.x, prev.y, prev.z);
     gadv[c] = ga;
     float m[8];
@@ -604,16 +608,6 @@ __global__ __launch_bounds__(256) void k_vadv_trace(DevGrid g, float dt, float* 
     tw[c] = v.z;
 }
 
-// the same, from the traced positions rho_advect stored in tu/tv/tw (in place)
-__global__ __launch_bounds__(256) void k_vadv_from_traces(DevGrid g, float* __restrict__ tu, float* __restrict__ tv,
-                                                          float* __restrict__ tw) {
-    INTERIOR_XYZ(g);
-    const float3 v = vinterp(g, tu[c] - 0.5f, tv[c] - 0.5f, tw[c] - 0.5f);
-    tu[c] = v.x;
-    tv[c] = v.y;
-    tw[c] = v.z;
-}
-
 // With FORCE the add_force stage that follows (k_add_force) is applied to the new face values
 // before they are stored, in the same operations.
 template <bool FORCE>
@@ -802,8 +796,8 @@ int efs_launch_rho_advect(efs_sim* s) {
 int efs_launch_vel_advect(efs_sim* s, bool with_force) {
     const DevGrid g = s->grid();
     const dim3 grd = interior_grid(s), blk(BX, BY);
-    if (s->traces_valid) hipLaunchKernelGGL(k_vadv_from_traces, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw);
-    else hipLaunchKernelGGL(k_vadv_trace, grd, blk, 0, s->stream, g, s->P.dt, s->tu, s->tv, s->tw);
+    // the samples at the backward traces are in tu/tv/tw already when rho_advect has just run
+    if (!s->traces_valid) hipLaunchKernelGGL(k_vadv_trace, grd, blk, 0, s->stream, g, s->P.dt, s->tu, s->tv, s->tw);
     s->traces_valid = false;
     if (with_force)
         hipLaunchKernelGGL(k_vadv_faces<true>, grd, blk, 0, s->stream, g, s->tu, s->tv, s->tw, s->P.dt,
