@@ -329,13 +329,15 @@ int efs_launch_rowcode(efs_sim* s, int* border_rows);
 int efs_cg_solve(efs_sim* s, efs_stats* st);
 int efs_cg_run(efs_sim* s, efs_stats* st, bool border, int max_it);
 // A preconditioner for efs_cg_run_pc: apply() writes z = M r (r = s->cg_r) into z, in stream
-// order; batch = iterations launched between host convergence checks.
+// order; batch = iterations launched between host convergence checks, first_batch (when > 0)
+// those launched before the first check.
 struct CgPrecond {
     int (*apply)(efs_sim* s, void* ctx);
     void* ctx;
     float* z;
     double* p_rz;
     int batch;
+    int first_batch = 0;
     bool writes_rz = false;  // apply also writes the r . z partials (no separate k_cg_rz)
 };
 int efs_cg_run_pc(efs_sim* s, efs_stats* st, bool border, int max_it, const CgPrecond* pc);

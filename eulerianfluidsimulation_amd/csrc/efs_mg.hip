@@ -40,8 +40,15 @@
 #ifndef EFS_MG_MIN_N
 #define EFS_MG_MIN_N 8
 #endif
+// Iterations launched before the first host convergence check: the previous solve's count
+// (EFS_MG_FIRST_BATCH for the first solve), since consecutive steps need about as many; then
+// EFS_MG_BATCH between checks.  Iterations launched past convergence return at once but still
+// dispatch every kernel of a V-cycle, so a batch sized to the expected count saves their launches.
+#ifndef EFS_MG_FIRST_BATCH
+#define EFS_MG_FIRST_BATCH 8
+#endif
 #ifndef EFS_MG_BATCH
-#define EFS_MG_BATCH 8
+#define EFS_MG_BATCH 2
 #endif
 #ifndef EFS_MG_IEEE_DIV
 #define EFS_MG_IEEE_DIV 0
@@ -74,6 +81,7 @@ struct MG {
     float* z = nullptr;
     double* p_rz = nullptr;
     bool fused_rz = false;  // the fine up sweep writes the r . z partials
+    int last_its = 0;       // iterations of the previous solve
 };
 
 MG* get(efs_sim* s) { return (MG*)s->mg; }
@@ -650,6 +658,7 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
     pc.z = m->z;
     pc.p_rz = m->p_rz;
     pc.batch = EFS_MG_BATCH;
+    pc.first_batch = m->last_its > 0 ? m->last_its : EFS_MG_FIRST_BATCH;
     pc.writes_rz = m->fused_rz;
     // The recurrence residual stops the iteration; with the true residual enabled and the relative
     // criterion, a solve whose FP64 true residual is still above the tolerance restarts from its
@@ -664,6 +673,7 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
         efs_trim_timer(s->ktimers[KT_OTHER], kt0 + ((size_t)s->st_host->iter + 1));
         efs_trim_timer(s->ktimers[KT_MG_DOWN], kt1 + ((size_t)s->st_host->iter + 1));
         total += o->iterations;
+        if (attempt == 0) m->last_its = o->iterations;
         const bool verify = s->P.true_residual && s->P.criterion == EFS_CRIT_RELATIVE;
         if (rc != 0 || !verify || o->true_rel_residual <= (double)s->P.solver_threshold) break;
     }

@@ -855,10 +855,12 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
         p_rz = pc->p_rz;
     }
     const int batch = pc ? pc->batch : 64;
+    int next_batch = (pc && pc->first_batch > 0) ? pc->first_batch : batch;
     int launched = 0;
     const size_t kt0 = s->ktimers[KT_CG_DIR_SPMV].start.size(), kt1 = s->ktimers[KT_CG_UPDATE].start.size();
     while (true) {
-        const int n = std::min(batch, max_it - launched);
+        const int n = std::min(next_batch, max_it - launched);
+        next_batch = batch;
         for (int k = 0; k < n; k++) {
             const int i = launched + k;
             hipEvent_t a, b;
