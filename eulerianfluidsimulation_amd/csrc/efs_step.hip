@@ -615,19 +615,27 @@ __global__ __launch_bounds__(256) void k_vadv_faces(DevGrid g, const float* __re
                                                     const float* __restrict__ tw, float dt, float gdt, int gravity) {
     INTERIOR_XYZ(g);
     const int sy = g.W, sz = g.W * g.H;
-    const bool bs = g.bf[c] & BF_EQ1;
-    float u = (x == 1 || bs || (g.bf[c - 1] & BF_EQ1)) ? tu[c] : 0.5f * (tu[c] + tu[c - 1]);
-    float v = (y == 1 || bs || (g.bf[c - sy] & BF_EQ1)) ? tv[c] : 0.5f * (tv[c] + tv[c - sy]);
-    float w = (z == 1 || bs || (g.bf[c - sz] & BF_EQ1)) ? tw[c] : 0.5f * (tw[c] + tw[c - sz]);
-    if (FORCE && !bs) {
-        if (g.fu) {
-            u = u + g.fu[c] * dt;
-            v = v + g.fv[c] * dt;
-            w = w + g.fw[c] * dt;
-        } else {
-            u = u + 0.f * dt; v = v + 0.f * dt; w = w + 0.f * dt;
-        }
-        if (gravity) v -= gdt;
+    // every load first (all in the array for an interior cell), then selects: tests that guard
+    // loads would serialise them behind one another
+    const unsigned b0 = g.bf[c], bxm = g.bf[c - 1], bym = g.bf[c - sy], bzm = g.bf[c - sz];
+    const float u0 = tu[c], um = tu[c - 1], v0 = tv[c], vm = tv[c - sy], w0 = tw[c], wm = tw[c - sz];
+    float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+    if (FORCE && g.fu) {
+        f0 = g.fu[c];
+        f1 = g.fv[c];
+        f2 = g.fw[c];
+    }
+    const bool bs = (b0 & BF_EQ1) != 0;
+    float u = ((x == 1) | bs | ((bxm & BF_EQ1) != 0)) ? u0 : 0.5f * (u0 + um);
+    float v = ((y == 1) | bs | ((bym & BF_EQ1) != 0)) ? v0 : 0.5f * (v0 + vm);
+    float w = ((z == 1) | bs | ((bzm & BF_EQ1) != 0)) ? w0 : 0.5f * (w0 + wm);
+    if (FORCE) {
+        // vel + f dt (vel + 0 dt with no force field), then gravity; solid cells keep the face value
+        float fu = u + f0 * dt, fv = v + f1 * dt, fw = w + f2 * dt;
+        if (gravity) fv -= gdt;
+        u = bs ? u : fu;
+        v = bs ? v : fv;
+        w = bs ? w : fw;
     }
     g.u[c] = u;
     g.v[c] = v;
@@ -694,27 +702,29 @@ __global__ __launch_bounds__(256) void k_gradient_walls(DevGrid g, const double*
     FULL_XYZ(g);
     if (x == 0 || y == 0 || z == 0) return;
     const int sy = g.W, sz = g.W * g.H;
-    const bool s0 = g.bf[c] & BF_EQ1;
+    // every load first (the -x/-y/-z neighbours are in the array for x, y, z >= 1), then selects
+    const unsigned b0 = g.bf[c], bxm = g.bf[c - 1], bym = g.bf[c - sy], bzm = g.bf[c - sz];
     float u = g.u[c], v = g.v[c], w = g.w[c];
-    const bool interior = x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
-    if (interior && !s0) {
-        if (p64) {
-            const double pc = p64[c];
-            const double gx = (x > 1) ? pc - p64[c - 1] : 0.0;
-            const double gy = (y > 1) ? pc - p64[c - sy] : 0.0;
-            const double gz = (z > 1) ? pc - p64[c - sz] : 0.0;
-            u = (float)((double)u - gx); v = (float)((double)v - gy); w = (float)((double)w - gz);
-        } else {
-            const float pc = g.p[c];
-            const float gx = (x > 1) ? pc - g.p[c - 1] : 0.f;
-            const float gy = (y > 1) ? pc - g.p[c - sy] : 0.f;
-            const float gz = (z > 1) ? pc - g.p[c - sz] : 0.f;
-            u = u - gx; v = v - gy; w = w - gz;
-        }
+    const bool s0 = (b0 & BF_EQ1) != 0;
+    const bool upd = (x < g.W - 1) & (y < g.H - 1) & (z < g.D - 1) & !s0;
+    if (p64) {
+        const double pc = p64[c], pxm = p64[c - 1], pym = p64[c - sy], pzm = p64[c - sz];
+        const double gx = (x > 1) ? pc - pxm : 0.0;
+        const double gy = (y > 1) ? pc - pym : 0.0;
+        const double gz = (z > 1) ? pc - pzm : 0.0;
+        const float nu = (float)((double)u - gx), nv = (float)((double)v - gy), nw = (float)((double)w - gz);
+        u = upd ? nu : u; v = upd ? nv : v; w = upd ? nw : w;
+    } else {
+        const float pc = g.p[c], pxm = g.p[c - 1], pym = g.p[c - sy], pzm = g.p[c - sz];
+        const float gx = (x > 1) ? pc - pxm : 0.f;
+        const float gy = (y > 1) ? pc - pym : 0.f;
+        const float gz = (z > 1) ? pc - pzm : 0.f;
+        const float nu = u - gx, nv = v - gy, nw = w - gz;
+        u = upd ? nu : u; v = upd ? nv : v; w = upd ? nw : w;
     }
-    g.u[c] = wall_face(u, s0, g.bf[c - 1] & BF_EQ1);
-    g.v[c] = wall_face(v, s0, g.bf[c - sy] & BF_EQ1);
-    g.w[c] = wall_face(w, s0, g.bf[c - sz] & BF_EQ1);
+    g.u[c] = wall_face(u, s0, (bxm & BF_EQ1) != 0);
+    g.v[c] = wall_face(v, s0, (bym & BF_EQ1) != 0);
+    g.w[c] = wall_face(w, s0, (bzm & BF_EQ1) != 0);
 }
 
 inline int blocks1d(int n) { return (n + 255) / 256; }
