@@ -44,6 +44,12 @@
 // (EFS_MG_FIRST_BATCH for the first solve), since consecutive steps need about as many; then
 // EFS_MG_BATCH between checks.  Iterations launched past convergence return at once but still
 // dispatch every kernel of a V-cycle, so a batch sized to the expected count saves their launches.
+// Shortest z chunk of a sweep block.  Levels with few tiles get short chunks, so that more
+// blocks run shorter plane marches (a march step costs about as much on a small level as on the
+// fine one, and the halo planes are recomputed exactly whatever the chunk).
+#ifndef EFS_MG_MIN_ZC
+#define EFS_MG_MIN_ZC 2
+#endif
 #ifndef EFS_MG_FIRST_BATCH
 #define EFS_MG_FIRST_BATCH 8
 #endif
@@ -564,8 +570,8 @@ int setup_levels(efs_sim* s) {
     return 0;
 }
 
-// launch domain of k_mg_sweep on level l: TX x PTY tiles, chunks of 8..32 planes (even, so that
-// no coarse cell straddles two chunks)
+// launch domain of k_mg_sweep on level l: TX x PTY tiles, chunks of EFS_MG_MIN_ZC..32 planes
+// (even, so that no coarse cell straddles two chunks)
 Dom post_dom(const Lvl& l) {
     Dom d{};
     d.W = l.W; d.H = l.H; d.D = l.D;
@@ -575,7 +581,7 @@ Dom post_dom(const Lvl& l) {
     d.tiles_x = (l.nx + TX - 1) / TX;
     d.tiles_y = (l.ny + PTY - 1) / PTY;
     const int txy = d.tiles_x * d.tiles_y;
-    d.zc = std::min(l.nz, std::max(8, std::min(32, l.nz * txy / 1024)) & ~1);
+    d.zc = std::min(l.nz, std::max(EFS_MG_MIN_ZC, std::min(32, l.nz * txy / 1024)) & ~1);
     d.chunks_z = (l.nz + d.zc - 1) / d.zc;
     d.nblocks = txy * d.chunks_z;
     return d;
