@@ -457,19 +457,22 @@ __global__ __launch_bounds__(MG_CO_NT) void k_mg_coarsest(Lvl l, float inv_s, in
         rs[t] = l.rc[t];
     }
     __syncthreads();
+    // each thread owns at most one interior cell (at most 10^3 <= MG_CO_NT): its index and colour
+    // are computed once, outside the sweeps
     const int n = l.nx * l.ny * l.nz;
     const int sy = l.W, sz = l.W * l.H;
+    const int t = threadIdx.x;
+    const bool own = t < n;
+    const int cx = 1 + t % l.nx, cy = 1 + (t / l.nx) % l.ny, cz = 1 + t / (l.nx * l.ny);
+    const int ci = own ? lidx(l, cx, cy, cz) : 0, ccol = (cx + cy + cz) & 1;
     for (int sw = 0; sw < 2 * sweeps; sw++) {
         for (int h = 0; h < 2; h++) {
             // forward sweeps red then black, backward sweeps black then red
             const int color = (sw & 1) ? 1 - h : h;
-            for (int t = threadIdx.x; t < n; t += MG_CO_NT) {
-                const int x = 1 + t % l.nx, y = 1 + (t / l.nx) % l.ny, z = 1 + t / (l.nx * l.ny);
-                if (((x + y + z) & 1) != color) continue;
-                const int i = lidx(l, x, y, z);
-                const unsigned rc = rs[i];
-                const float v[6] = {xs[i - 1], xs[i + 1], xs[i - sy], xs[i + sy], xs[i + sz], xs[i - sz]};
-                xs[i] = row_update(rc, masked_sum6(rc, v), bs[i], inv_s);
+            if (own && ccol == color) {
+                const unsigned rc = rs[ci];
+                const float v[6] = {xs[ci - 1], xs[ci + 1], xs[ci - sy], xs[ci + sy], xs[ci + sz], xs[ci - sz]};
+                xs[ci] = row_update(rc, masked_sum6(rc, v), bs[ci], inv_s);
             }
             __syncthreads();
         }
