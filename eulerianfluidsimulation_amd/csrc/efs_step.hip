@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __r
 // independent of the lane it lands on and the scatters are fixed-point sums, so the results do
 // not depend on the packing order.
 #ifndef EFS_FWD_ROWS
-#define EFS_FWD_ROWS 8
+#define EFS_FWD_ROWS 4
 #endif
 constexpr int FWD_ROWS = EFS_FWD_ROWS;
 __global__ __launch_bounds__(64 * FWD_ROWS) void k_adv_fwd(DevGrid g, float dt, const unsigned long long* __restrict__ beta_acc,
