@@ -368,11 +368,16 @@ __global__ __launch_bounds__(256) void k_adv_rows_fwd(DevGrid g, const float* __
     const int q0 = fq0[c];
     if (q0 < 0) return;
     long long own = 0;
+    float e[8], gk[8];  // every load first, then the divisions in the same order
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        const float gi = g6[q0 + cornerOffset(g, k)];
-        float a = fw[(size_t)k * g.N + c];
-        if (gi > 1.e-5f) a /= gi;
+        gk[k] = g6[q0 + cornerOffset(g, k)];
+        e[k] = fw[(size_t)k * g.N + c];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        float a = e[k];
+        if (gk[k] > 1.e-5f) a /= gk[k];
         fw[(size_t)k * g.N + c] = a;
         own += to_fix(a);
     }
@@ -419,16 +424,24 @@ __global__ __launch_bounds__(256) void k_adv_final(DevGrid g, const int* __restr
     const int p0 = bp0[c];
     const bool interior = x > 0 && y > 0 && z > 0 && x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
     if (interior && p0 >= 0) {
+        // every load of the row first (the entries, and beta and rho at the 8 columns), then the
+        // divisions and sums in the same order
         const float gi = g6[c];
+        float e[8], bc[8], rc[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            float a = bw[(size_t)k * g.N + c];
-            if (gi > 1.e-5f) a /= gi;
             const int j = p0 + cornerOffset(g, k);
-            const float bj = g.beta[j];
-            if (bj > 1.e-5f) a /= bj;
+            e[k] = bw[(size_t)k * g.N + c];
+            bc[k] = g.beta[j];
+            rc[k] = g.rho[j];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float a = e[k];
+            if (gi > 1.e-5f) a /= gi;
+            if (bc[k] > 1.e-5f) a /= bc[k];
             gs += to_fix(a);
-            rs += to_fix(a * g.rho[j]);
+            rs += to_fix(a * rc[k]);
         }
     }
     g.gamma[c] = (float)((double)gs * FIX_INV);
