@@ -151,6 +151,25 @@ __global__ __launch_bounds__(256) void k_mg_rowcode(Lvl c) {
     c.rc[ci] = (uint16_t)code;
 }
 
+// Over-relaxation of the V(2,2) smoother: x <- x + w (gs - x) with the same w in every pass,
+// so the symmetric cycle (red-black down, black-red up) stays symmetric; w = 1 is Gauss-Seidel.
+#ifndef EFS_MG_OMEGA
+#define EFS_MG_OMEGA 1.2f
+#endif
+// Relaxed update of a row whose old value is xo and Gauss-Seidel value gs.  Only the rows the
+// Gauss-Seidel pass updates (fluid rows with counted neighbours) are relaxed; every other cell
+// keeps the pass's value, 0, as in Gauss-Seidel (a fluid row with no counted neighbour is a zero
+// row of A, and a nonzero value there would enter r.z but not A d).  Fluid rows without
+// RC_CONTRIB (next to a boundary value in (0.5, 1), possible only on the fine level) are stored
+// masked, so their old value is not kept: they take the plain Gauss-Seidel value, which keeps
+// the cycle symmetric with a per-row factor.
+template <bool SOR>
+__device__ inline float relax(unsigned rc, float gs, float xo) {
+    if (!SOR) return gs;
+    const bool upd = (rc & RC_CONTRIB) && (rc & RC_CG_ROW) && ((rc >> RC_CNT_SHIFT) & 7u) > 0;
+    return upd ? xo + (float)EFS_MG_OMEGA * (gs - xo) : gs;
+}
+
 // Gauss-Seidel row update: (sum - b / s) / n on fluid rows with neighbours, else 0.
 __device__ inline float row_update(unsigned rc, float sum, float b, float inv_s) {
     const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
@@ -217,6 +236,8 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
     constexpr bool USE_E = MODE == SW_UP || MODE == SW_UP_RZ || MODE == SW_BR_E;
     constexpr bool FROM_X = MODE == SW_RB || MODE == SW_BR_E || MODE == SW_BR || MODE == SW_BR_RZ;
     constexpr int C1 = MODE == SW_RB ? 0 : 1, C0 = 1 - C1;
+    // the V(2,2) sweeps over-relax; their old values come from the stored x (or 0 from the start)
+    constexpr bool SOR = MODE >= SW_DOWN0 && EFS_MG_OMEGA != 1.0f;
     if (st->done) return;
     __shared__ float ro[3][PP], bn[3][PP], bs[4][PP];
     __shared__ uint16_t rs[4][PP];
@@ -293,11 +314,12 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
             for (int i = 0; i < 2; i++) {
                 if (i == 1 && !fv[1]) continue;
                 const unsigned rc = f.rc[i];
-                float v = FROM_X ? f.xv[i] : row_update(rc, 0.f, f.b[i], inv_s);
+                float v = FROM_X ? f.xv[i] : relax<SOR>(rc, row_update(rc, 0.f, f.b[i], inv_s), 0.f);
                 if (USE_E) v = v + f.e[i];
-                // only the C0 cells of the red ring are read (the neighbours of C1 cells)
+                // the C0 cells of the red ring are the neighbours of C1 cells; with over-relaxation
+                // the C1 cells keep their own old value there too (0 when sweeping from x = 0)
                 const bool c0 = ((fpar[i] + p) & 1) == C0;
-                ro_p[fk[i]] = (c0 && (rc & RC_CONTRIB)) ? v : 0.f;
+                ro_p[fk[i]] = ((c0 || (SOR && FROM_X)) && (rc & RC_CONTRIB)) ? v : 0.f;
                 bs_p[fk[i]] = f.b[i];
                 rs_p[fk[i]] = (uint16_t)rc;
             }
@@ -317,7 +339,7 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
             const float b = bs[p & 3][k];
             const float sum = 0.f + r1[km] + r1[kp] + r1[k - PX] + r1[k + PX] + r2[k] + r0[k];
             const bool inside = x >= 1 && x <= l.nx && byu >= 1 && byu <= l.ny && p >= 1 && p <= l.nz;
-            const float val = inside ? row_update(rc, sum, b, inv_s) : 0.f;
+            const float val = inside ? relax<SOR>(rc, row_update(rc, sum, b, inv_s), r1[k]) : 0.f;
             bn[s][k] = (rc & RC_CONTRIB) ? val : 0.f;
             const bool tile = brow_tile && x >= x0 && x < x0 + TX && p >= zb && p < ze;
             if (UP) {
@@ -331,7 +353,9 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
         };
         // the C0 value (or its residual, SW_DOWN) of the lane's pair in plane q; tm, t, tp: the mod-3
         // slots of planes q - 1, q, q + 1
-        auto finish = [&](int q, int tm, int t, int tp) {
+        // xo: the cell's old value, read from the red ring by the caller before the barrier that
+        // precedes the next commit (the ring has three planes, so plane q's slot is refilled then)
+        auto finish = [&](int q, int tm, int t, int tp, float xo) {
             if (!fin) return;
             const int h = ((txu + tyu + q) & 1) == C0 ? 0 : 1;
             const int k = tk + h, km = k - 1, kp = k + 1, x = txu + h;
@@ -341,10 +365,10 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
             const float b = bs[q & 3][k];
             const float sum = 0.f + n1[km] + n1[kp] + n1[k - PX] + n1[k + PX] + n2[k] + n0[k];
             if (MODE == SW_DOWN0) {
-                if (inside) xout[q * plane + tyu * l.W + x] = row_update(rc, 0.f, b, inv_s);
+                if (inside) xout[q * plane + tyu * l.W + x] = relax<SOR>(rc, row_update(rc, 0.f, b, inv_s), 0.f);
             } else if (UP) {
                 if (!inside) return;
-                const float val = row_update(rc, sum, b, inv_s);
+                const float val = relax<SOR>(rc, row_update(rc, sum, b, inv_s), xo);
                 xout[q * plane + tyu * l.W + x] = val;
                 if (RZ) acc += (double)b * (double)val;
             } else {
@@ -380,11 +404,13 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
             commit(p + 1, f, mp);  // ring slots of planes p - 2 (red) and p - 3 (b, codes): no longer read
             if (p + 3 <= ze + 1) fetch(p + 3, f);
             lds_barrier();
+            const int q = p - 1;
+            float xo = 0.f;  // the finish cell's old value in plane q (over-relaxation)
+            if (SOR && fin && q >= zb) xo = ro[mm][tk + (((txu + tyu + q) & 1) == C0 ? 0 : 1)];
             black(p, mm, m, mp);
             lds_barrier();
-            const int q = p - 1;
             if (q >= zb) {
-                finish(q, mp, mm, m);  // planes q - 1 = p - 2 (slot of p + 1), q, q + 1
+                finish(q, mp, mm, m, xo);  // planes q - 1 = p - 2 (slot of p + 1), q, q + 1
                 if (!UP && (((q - zb) & 1) == 1 || q == ze - 1)) {
                     lds_barrier();
                     restrict_pair(q - ((q - zb) & 1));
@@ -472,7 +498,8 @@ __global__ __launch_bounds__(MG_CO_NT) void k_mg_coarsest(Lvl l, float inv_s, in
             if (own && ccol == color) {
                 const unsigned rc = rs[ci];
                 const float v[6] = {xs[ci - 1], xs[ci + 1], xs[ci - sy], xs[ci + sy], xs[ci + sz], xs[ci - sz]};
-                xs[ci] = row_update(rc, masked_sum6(rc, v), bs[ci], inv_s);
+                xs[ci] = relax<EFS_MG_NU == 2 && EFS_MG_OMEGA != 1.0f>(rc, row_update(rc, masked_sum6(rc, v), bs[ci], inv_s),
+                                                                     xs[ci]);
             }
             __syncthreads();
         }
