@@ -190,8 +190,9 @@ __device__ inline float3 traceParticle(const DevGrid& g, float sx, float sy, flo
 
 // Corner weights of a traced position (CPUSimulator3D.cpp:140-163).  m[k] for corners in the
 // reference order 000,100,010,110,001,101,011,111 (x fastest); returns the sum.
-__device__ inline float cornerWeights(const DevGrid& g, float px, float py, float pz, int& p0, float m[8]) {
-    int p0x = (int)floorf(px), p0y = (int)floorf(py), p0z = (int)floorf(pz);
+__device__ inline float cornerWeights(const DevGrid& g, float px, float py, float pz, int& p0, float m[8], int& p0x,
+                                     int& p0y, int& p0z) {
+    p0x = (int)floorf(px), p0y = (int)floorf(py), p0z = (int)floorf(pz);
     p0x = clampv(p0x, 0, g.W - 2); p0y = clampv(p0y, 0, g.H - 2); p0z = clampv(p0z, 0, g.D - 2);
     const float dx0 = 1 - (px - p0x), dy0 = 1 - (py - p0y), dz0 = 1 - (pz - p0z);
     const float dx1 = 1 - (p0x + 1 - px), dy1 = 1 - (p0y + 1 - py), dz1 = 1 - (p0z + 1 - pz);

@@ -231,6 +231,99 @@ __global__ void k_add_fix(int n, float* __restrict__ f, unsigned long long* __re
     }
 }
 
+// ---- block-private scatter box -------------------------------------------------------------
+// The column and row sums of the advection matrix are fixed-point integer sums (modulo 2^64),
+// so their terms may be added in any grouping.  The 8-corner scatters of a 64 x 4 block land,
+// where the traced positions move together, on about 65 x 5 x 2 cells, which each receive up
+// to eight terms from the block.  The block first adds its terms into a box of LDS anchored at
+// its smallest target corner and then adds each nonzero box cell to HBM with one atomic, which
+// cuts the device atomics about threefold.  A lane whose corners reach past the box adds its
+// terms to HBM directly.
+constexpr int SB_X = 68, SB_Y = 8, SB_Z = 4, SB_N = SB_X * SB_Y * SB_Z;
+constexpr int SB_WAVES = (BX * BY) / 64;
+
+struct ScatterBox {
+    unsigned long long cell[SB_N];
+    int wmin[SB_WAVES][3];
+};
+
+__device__ inline void sbox_clear(ScatterBox& b, int t) {
+    for (int i = t; i < SB_N; i += BX * BY) b.cell[i] = 0ull;
+}
+
+// Each wave's smallest target corner (lanes without terms give none); read after a barrier.
+__device__ inline void sbox_vote(ScatterBox& b, int wave, bool live, int px, int py, int pz) {
+    int mx = live ? px : INT_MAX, my = live ? py : INT_MAX, mz = live ? pz : INT_MAX;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = min(mx, __shfl_xor(mx, o));
+        my = min(my, __shfl_xor(my, o));
+        mz = min(mz, __shfl_xor(mz, o));
+    }
+    if (__lane_id() == 0) {
+        b.wmin[wave][0] = mx;
+        b.wmin[wave][1] = my;
+        b.wmin[wave][2] = mz;
+    }
+}
+
+// Box origin (the block's smallest target corner; INT_MAX when no lane has terms).
+__device__ inline int3 sbox_origin(const ScatterBox& b) {
+    int3 o = make_int3(INT_MAX, INT_MAX, INT_MAX);
+#pragma unroll
+    for (int w = 0; w < SB_WAVES; w++) {
+        o.x = min(o.x, b.wmin[w][0]);
+        o.y = min(o.y, b.wmin[w][1]);
+        o.z = min(o.z, b.wmin[w][2]);
+    }
+    return o;
+}
+
+// Adds the 8 corner terms of a lane with target corner (px, py, pz) = linear index p0.
+__device__ inline void sbox_add(const DevGrid& g, ScatterBox& b, int3 o, int px, int py, int pz, int p0,
+                                const long long q[8], unsigned long long* acc) {
+    const int rx = px - o.x, ry = py - o.y, rz = pz - o.z;
+    if (rx <= SB_X - 2 && ry <= SB_Y - 2 && rz <= SB_Z - 2) {
+        const int s = rx + SB_X * (ry + SB_Y * rz);
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (q[k] != 0) atomicAdd(&b.cell[s + (k & 1) + ((k >> 1) & 1) * SB_X + ((k >> 2) & 1) * SB_X * SB_Y],
+                                     (unsigned long long)q[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (q[k] != 0) atomicAdd(acc + p0 + cornerOffset(g, k), (unsigned long long)q[k]);
+    }
+}
+
+// Adds the nonzero box cells to HBM; after a barrier that follows every sbox_add.  With CLEAR
+// each thread zeroes the cells it read, so after a further barrier the box takes new terms.
+template <bool CLEAR = false>
+__device__ inline void sbox_flush(const DevGrid& g, ScatterBox& b, int3 o, int t, unsigned long long* acc) {
+    if (o.x == INT_MAX) return;
+    const int base = cidx(g, o.x, o.y, o.z);
+    for (int i = t; i < SB_N; i += BX * BY) {
+        const unsigned long long v = b.cell[i];
+        if (v == 0ull) continue;
+        if (CLEAR) b.cell[i] = 0ull;
+        const int ix = i % SB_X, iy = (i / SB_X) % SB_Y, iz = i / (SB_X * SB_Y);
+        atomicAdd(acc + base + ix + g.W * (iy + g.H * iz), v);
+    }
+}
+
+// (x, y, z) of a linear cell index: quotients by W and W * H from a double estimate corrected
+// once (the estimate is within one of the quotient for indices below 2^31).
+__device__ inline int3 cell_coords(const DevGrid& g, int i) {
+    const int wh = g.W * g.H;
+    int z = (int)((double)i * (1.0 / (double)wh));
+    int r = i - z * wh;
+    if (r < 0) { z--; r += wh; } else if (r >= wh) { z++; r -= wh; }
+    int y = (int)((double)r * (1.0 / (double)g.W));
+    int x = r - y * g.W;
+    if (x < 0) { y--; x += g.W; } else if (x >= g.W) { y++; x -= g.W; }
+    return make_int3(x, y, z);
+}
+
 // ---- rho_advect (CPUSimulator3D.cpp:113-364) ----------------------------------------------
 // bwd_w[k * N + i] = A(i, p0_i + corner k) for row i; fwd_w[k * N + j] = A(q0_j + corner k, j).
 
@@ -241,29 +334,42 @@ __global__ void k_add_fix(int n, float* __restrict__ f, unsigned long long* __re
 __global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __restrict__ gadv, int* __restrict__ bp0,
                                                  float* __restrict__ bw, unsigned long long* beta_acc, float* __restrict__ su,
                                                  float* __restrict__ sv, float* __restrict__ sw) {
-    INTERIOR_XYZ(g);
-    const float3 prev = traceParticle(g, (float)x, (float)y, (float)z, -dt, 4);
-    const float3 vs = vinterp(g, prev.x - 0.5f, prev.y - 0.5f, prev.z - 0.5f);
-    su[c] = vs.x;
-    sv[c] = vs.y;
-    sw[c] = vs.z;
-    const float ga = (g.bf[c] & BF_EQ1) ? 0.f : sinterp(g, g.gamma, prev.x, prev.y, prev.z);
-    gadv[c] = ga;
-    float m[8];
-    int p0;
-    const float sum = cornerWeights(g, prev.x, prev.y, prev.z, p0, m);
-    if (sum == 0) {
-        bp0[c] = -1;
-        return;
-    }
-    bp0[c] = p0;
+    __shared__ ScatterBox box;
+    const int t = threadIdx.x + BX * threadIdx.y;
+    sbox_clear(box, t);
+    const int x = 1 + blockIdx.x * BX + threadIdx.x, y = 1 + blockIdx.y * BY + threadIdx.y, z = 1 + blockIdx.z;
+    bool live = false;
+    int p0 = 0, px = 0, py = 0, pz = 0;
+    long long q[8];
+    if (x <= g.W - 2 && y <= g.H - 2) {
+        const int c = cidx(g, x, y, z);
+        const float3 prev = traceParticle(g, (float)x, (float)y, (float)z, -dt, 4);
+        const float3 vs = vinterp(g, prev.x - 0.5f, prev.y - 0.5f, prev.z - 0.5f);
+        su[c] = vs.x;
+        sv[c] = vs.y;
+        sw[c] = vs.z;
+        const float ga = (g.bf[c] & BF_EQ1) ? 0.f : sinterp(g, g.gamma, prev.x, prev.y, prev.z);
+        gadv[c] = ga;
+        float m[8];
+        const float sum = cornerWeights(g, prev.x, prev.y, prev.z, p0, m, px, py, pz);
+        live = sum != 0;
+        bp0[c] = live ? p0 : -1;
+        if (live) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        float a = m[k] / sum;
-        if (ga > 1.e-5f) a /= ga;
-        bw[(size_t)k * g.N + c] = a;
-        fix_add(beta_acc, p0 + cornerOffset(g, k), a);
+            for (int k = 0; k < 8; k++) {
+                float a = m[k] / sum;
+                if (ga > 1.e-5f) a /= ga;
+                bw[(size_t)k * g.N + c] = a;
+                q[k] = to_fix(a);
+            }
+        }
     }
+    sbox_vote(box, threadIdx.y, live, px, py, pz);
+    __syncthreads();
+    const int3 o = sbox_origin(box);
+    if (live) sbox_add(g, box, o, px, py, pz, p0, q, beta_acc);
+    __syncthreads();
+    sbox_flush(g, box, o, t, beta_acc);
 }
 
 // Step 5: forward trace for columns with beta < 1; the new entries are scattered into the
@@ -275,15 +381,25 @@ __global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __r
 #ifndef EFS_FWD_ROWS
 #define EFS_FWD_ROWS 4
 #endif
+#ifndef EFS_FWD_BOX
+#define EFS_FWD_BOX 1
+#endif
 constexpr int FWD_ROWS = EFS_FWD_ROWS;
+static_assert(64 * FWD_ROWS == BX * BY, "the forward trace shares the scatter box of the 64 x 4 blocks");
 __global__ __launch_bounds__(64 * FWD_ROWS) void k_adv_fwd(DevGrid g, float dt, const unsigned long long* __restrict__ beta_acc,
                                                            int* __restrict__ fq0, float* __restrict__ fw,
                                                            unsigned long long* gamma_acc) {
     __shared__ unsigned short s_slot[64 * FWD_ROWS];
     __shared__ float s_bj[64 * FWD_ROWS];
     __shared__ int s_n;
+#if EFS_FWD_BOX
+    __shared__ ScatterBox box;
+#endif
     const int lane = threadIdx.x, t = threadIdx.x + 64 * threadIdx.y;
     if (t == 0) s_n = 0;
+#if EFS_FWD_BOX
+    sbox_clear(box, t);
+#endif
     __syncthreads();
     const int x0 = 1 + blockIdx.x * 64, y0 = 1 + blockIdx.y * FWD_ROWS, z = 1 + blockIdx.z;
     bool act = false;
@@ -304,26 +420,44 @@ __global__ __launch_bounds__(64 * FWD_ROWS) void k_adv_fwd(DevGrid g, float dt, 
         s_bj[slot] = bj;
     }
     __syncthreads();
+#if !EFS_FWD_BOX
     if (t >= s_n) return;
-    const int src = s_slot[t];
-    bj = s_bj[t];
-    const int x = x0 + (src & 63), y = y0 + (src >> 6);
-    const int c = cidx(g, x, y, z);
-    const float3 next = traceParticle(g, (float)x, (float)y, (float)z, +dt, 4);
-    float m[8];
-    int q0;
-    const float sum = cornerWeights(g, next.x, next.y, next.z, q0, m);
-    if (sum == 0) {
-        fq0[c] = -1;
-        return;
-    }
-    fq0[c] = q0;
+#endif
+    bool live = false;
+    int q0 = 0, qx = 0, qy = 0, qz = 0;
+    long long q[8];
+    if (t < s_n) {
+        const int src = s_slot[t];
+        bj = s_bj[t];
+        const int x = x0 + (src & 63), y = y0 + (src >> 6);
+        const int c = cidx(g, x, y, z);
+        const float3 next = traceParticle(g, (float)x, (float)y, (float)z, +dt, 4);
+        float m[8];
+        const float sum = cornerWeights(g, next.x, next.y, next.z, q0, m, qx, qy, qz);
+        live = sum != 0;
+        fq0[c] = live ? q0 : -1;
+        if (live) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const float a = (1 - bj) * (m[k] / sum);
-        fw[(size_t)k * g.N + c] = a;
-        fix_add(gamma_acc, q0 + cornerOffset(g, k), a);
+            for (int k = 0; k < 8; k++) {
+                const float a = (1 - bj) * (m[k] / sum);
+                fw[(size_t)k * g.N + c] = a;
+                q[k] = to_fix(a);
+            }
+        }
     }
+#if EFS_FWD_BOX
+    sbox_vote(box, threadIdx.y, live, qx, qy, qz);
+    __syncthreads();
+    const int3 o = sbox_origin(box);
+    if (live) sbox_add(g, box, o, qx, qy, qz, q0, q, gamma_acc);
+    __syncthreads();
+    sbox_flush(g, box, o, t, gamma_acc);
+#else
+    if (live)
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (q[k] != 0) atomicAdd(gamma_acc + q0 + cornerOffset(g, k), (unsigned long long)q[k]);
+#endif
 }
 
 // The backward entries are stored once, as step 4 leaves them; their row and column scalings
@@ -336,28 +470,46 @@ __global__ __launch_bounds__(64 * FWD_ROWS) void k_adv_fwd(DevGrid g, float dt, 
 __global__ __launch_bounds__(256) void k_adv_rows(DevGrid g, const int* __restrict__ bp0, const float* __restrict__ bw,
                                                   unsigned long long* __restrict__ gamma_acc, float* __restrict__ g6,
                                                   unsigned long long* beta_acc) {
-    FULL_XYZ(g);
-    long long s = (long long)gamma_acc[c];
-    gamma_acc[c] = 0ull;
-    const int p0 = bp0[c];
-    float a[8];
-    if (p0 >= 0) {
+    __shared__ ScatterBox box;
+    const int t = threadIdx.x + BX * threadIdx.y;
+    sbox_clear(box, t);
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, z = blockIdx.z;
+    bool live = false;
+    int p0 = -1;
+    int3 pc = make_int3(0, 0, 0);
+    long long q[8];
+    if (x < g.W && y < g.H) {
+        const int c = cidx(g, x, y, z);
+        long long s = (long long)gamma_acc[c];
+        gamma_acc[c] = 0ull;
+        p0 = bp0[c];
+        live = p0 >= 0;
+        float a[8];
+        if (live) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            a[k] = bw[(size_t)k * g.N + c];
-            s += to_fix(a[k]);
+            for (int k = 0; k < 8; k++) {
+                a[k] = bw[(size_t)k * g.N + c];
+                s += to_fix(a[k]);
+            }
+        }
+        const float gi = (float)((double)s * FIX_INV);
+        g6[c] = gi;
+        if (live) {
+            pc = cell_coords(g, p0);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                float v = a[k];
+                if (gi > 1.e-5f) v /= gi;
+                q[k] = to_fix(v);
+            }
         }
     }
-    const float gi = (float)((double)s * FIX_INV);
-    g6[c] = gi;
-    if (p0 >= 0) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            float v = a[k];
-            if (gi > 1.e-5f) v /= gi;
-            fix_add(beta_acc, p0 + cornerOffset(g, k), v);
-        }
-    }
+    sbox_vote(box, threadIdx.y, live, pc.x, pc.y, pc.z);
+    __syncthreads();
+    const int3 o = sbox_origin(box);
+    if (live) sbox_add(g, box, o, pc.x, pc.y, pc.z, p0, q, beta_acc);
+    __syncthreads();
+    sbox_flush(g, box, o, t, beta_acc);
 }
 
 // Steps 7-8 for the forward entries: divide by the gamma of their row and add the cell's own
@@ -396,19 +548,42 @@ __global__ void k_fix_to_field(int n, unsigned long long* __restrict__ acc, floa
 // and to rho' = A rho.
 __global__ __launch_bounds__(256) void k_adv_cols_fwd(DevGrid g, const int* __restrict__ fq0, const float* __restrict__ fw,
                                                       unsigned long long* gamma_acc, unsigned long long* rho_acc) {
-    INTERIOR_XYZ(g);
-    const int q0 = fq0[c];
-    if (q0 < 0) return;
-    const float bj = g.beta[c];
-    const float rj = g.rho[c];
+    __shared__ ScatterBox box;
+    const int t = threadIdx.x + BX * threadIdx.y;
+    sbox_clear(box, t);
+    const int x = 1 + blockIdx.x * BX + threadIdx.x, y = 1 + blockIdx.y * BY + threadIdx.y, z = 1 + blockIdx.z;
+    bool live = false;
+    int q0 = -1;
+    int3 qc = make_int3(0, 0, 0);
+    long long qg[8], qr[8];
+    if (x <= g.W - 2 && y <= g.H - 2) {
+        const int c = cidx(g, x, y, z);
+        q0 = fq0[c];
+        live = q0 >= 0;
+        if (live) {
+            const float bj = g.beta[c];
+            const float rj = g.rho[c];
+            qc = cell_coords(g, q0);
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        float a = fw[(size_t)k * g.N + c];
-        if (bj > 1.e-5f) a /= bj;
-        const int i = q0 + cornerOffset(g, k);
-        fix_add(gamma_acc, i, a);
-        fix_add(rho_acc, i, a * rj);
+            for (int k = 0; k < 8; k++) {
+                float a = fw[(size_t)k * g.N + c];
+                if (bj > 1.e-5f) a /= bj;
+                qg[k] = to_fix(a);
+                qr[k] = to_fix(a * rj);
+            }
+        }
     }
+    // the gamma terms, then the rho terms through the same box
+    sbox_vote(box, threadIdx.y, live, qc.x, qc.y, qc.z);
+    __syncthreads();
+    const int3 o = sbox_origin(box);
+    if (live) sbox_add(g, box, o, qc.x, qc.y, qc.z, q0, qg, gamma_acc);
+    __syncthreads();
+    sbox_flush<true>(g, box, o, t, gamma_acc);
+    __syncthreads();
+    if (live) sbox_add(g, box, o, qc.x, qc.y, qc.z, q0, qr, rho_acc);
+    __syncthreads();
+    sbox_flush(g, box, o, t, rho_acc);
 }
 
 // Steps 9-11 for the backward entries, all cells: each entry divided by its row's gamma and its
