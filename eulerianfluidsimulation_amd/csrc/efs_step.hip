@@ -25,6 +25,16 @@ constexpr int BX = 64, BY = 4;
 inline dim3 interior_grid(const efs_sim* s) { return dim3((s->W - 2 + BX - 1) / BX, (s->H - 2 + BY - 1) / BY, s->D - 2); }
 inline dim3 full_grid(const efs_sim* s) { return dim3((s->W + BX - 1) / BX, (s->H + BY - 1) / BY, s->D); }
 
+// Block shape of the advection passes that scatter (the traces and the row and column sums):
+// ABX x ABY cells of one z plane, 256 threads.
+#ifndef EFS_ADV_BX
+#define EFS_ADV_BX 64
+#endif
+constexpr int ABX = EFS_ADV_BX, ABY = 256 / ABX;
+static_assert(ABX * ABY == 256 && ABX % 16 == 0, "advection blocks are 256 threads, at least 16 wide");
+inline dim3 adv_interior_grid(const efs_sim* s) { return dim3((s->W - 2 + ABX - 1) / ABX, (s->H - 2 + ABY - 1) / ABY, s->D - 2); }
+inline dim3 adv_full_grid(const efs_sim* s) { return dim3((s->W + ABX - 1) / ABX, (s->H + ABY - 1) / ABY, s->D); }
+
 #define INTERIOR_XYZ(g)                                                             \
     const int x = 1 + blockIdx.x * BX + threadIdx.x, y = 1 + blockIdx.y * BY + threadIdx.y; \
     const int z = 1 + blockIdx.z;                                                   \
@@ -233,14 +243,14 @@ __global__ void k_add_fix(int n, float* __restrict__ f, unsigned long long* __re
 
 // ---- block-private scatter box -------------------------------------------------------------
 // The column and row sums of the advection matrix are fixed-point integer sums (modulo 2^64),
-// so their terms may be added in any grouping.  The 8-corner scatters of a 64 x 4 block land,
-// where the traced positions move together, on about 65 x 5 x 2 cells, which each receive up
-// to eight terms from the block.  The block first adds its terms into a box of LDS anchored at
+// so their terms may be added in any grouping.  The 8-corner scatters of an ABX x ABY block
+// land, where the traced positions move together, on about (ABX + 1) x (ABY + 1) x 2 cells,
+// which each receive up to eight terms from the block.  The block first adds its terms into a box of LDS anchored at
 // its smallest target corner and then adds each nonzero box cell to HBM with one atomic, which
 // cuts the device atomics about threefold.  A lane whose corners reach past the box adds its
 // terms to HBM directly.
-constexpr int SB_X = 68, SB_Y = 8, SB_Z = 4, SB_N = SB_X * SB_Y * SB_Z;
-constexpr int SB_WAVES = (BX * BY) / 64;
+constexpr int SB_X = ABX + 4, SB_Y = ABY + 4, SB_Z = 4, SB_N = SB_X * SB_Y * SB_Z;
+constexpr int SB_WAVES = (ABX * ABY) / 64;
 
 struct ScatterBox {
     unsigned long long cell[SB_N];
@@ -248,7 +258,7 @@ struct ScatterBox {
 };
 
 __device__ inline void sbox_clear(ScatterBox& b, int t) {
-    for (int i = t; i < SB_N; i += BX * BY) b.cell[i] = 0ull;
+    for (int i = t; i < SB_N; i += ABX * ABY) b.cell[i] = 0ull;
 }
 
 // Each wave's smallest target corner (lanes without terms give none); read after a barrier.
@@ -302,7 +312,7 @@ template <bool CLEAR = false>
 __device__ inline void sbox_flush(const DevGrid& g, ScatterBox& b, int3 o, int t, unsigned long long* acc) {
     if (o.x == INT_MAX) return;
     const int base = cidx(g, o.x, o.y, o.z);
-    for (int i = t; i < SB_N; i += BX * BY) {
+    for (int i = t; i < SB_N; i += ABX * ABY) {
         const unsigned long long v = b.cell[i];
         if (v == 0ull) continue;
         if (CLEAR) b.cell[i] = 0ull;
@@ -335,9 +345,9 @@ __global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __r
                                                  float* __restrict__ bw, unsigned long long* beta_acc, float* __restrict__ su,
                                                  float* __restrict__ sv, float* __restrict__ sw) {
     __shared__ ScatterBox box;
-    const int t = threadIdx.x + BX * threadIdx.y;
+    const int t = threadIdx.x + ABX * threadIdx.y;
     sbox_clear(box, t);
-    const int x = 1 + blockIdx.x * BX + threadIdx.x, y = 1 + blockIdx.y * BY + threadIdx.y, z = 1 + blockIdx.z;
+    const int x = 1 + blockIdx.x * ABX + threadIdx.x, y = 1 + blockIdx.y * ABY + threadIdx.y, z = 1 + blockIdx.z;
     bool live = false;
     int p0 = 0, px = 0, py = 0, pz = 0;
     long long q[8];
@@ -364,7 +374,7 @@ __global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __r
             }
         }
     }
-    sbox_vote(box, threadIdx.y, live, px, py, pz);
+    sbox_vote(box, t >> 6, live, px, py, pz);
     __syncthreads();
     const int3 o = sbox_origin(box);
     if (live) sbox_add(g, box, o, px, py, pz, p0, q, beta_acc);
@@ -374,38 +384,32 @@ __global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __r
 
 // Step 5: forward trace for columns with beta < 1; the new entries are scattered into the
 // row sums of step 6 (gamma_acc).  A third to a half of the cells have beta < 1, spread over
-// most waves, so each workgroup (a 64 x FWD_ROWS tile) first packs its tracing cells into the
-// lowest lanes through LDS and the waves left without a cell exit.  Every cell's work is
-// independent of the lane it lands on and the scatters are fixed-point sums, so the results do
-// not depend on the packing order.
-#ifndef EFS_FWD_ROWS
-#define EFS_FWD_ROWS 4
-#endif
+// most waves, so each workgroup (an ABX x ABY tile) first packs its tracing cells into the
+// lowest lanes through LDS.  Every cell's work is independent of the lane it lands on and the
+// scatters are fixed-point sums, so the results do not depend on the packing order.
 #ifndef EFS_FWD_BOX
 #define EFS_FWD_BOX 1
 #endif
-constexpr int FWD_ROWS = EFS_FWD_ROWS;
-static_assert(64 * FWD_ROWS == BX * BY, "the forward trace shares the scatter box of the 64 x 4 blocks");
-__global__ __launch_bounds__(64 * FWD_ROWS) void k_adv_fwd(DevGrid g, float dt, const unsigned long long* __restrict__ beta_acc,
-                                                           int* __restrict__ fq0, float* __restrict__ fw,
-                                                           unsigned long long* gamma_acc) {
-    __shared__ unsigned short s_slot[64 * FWD_ROWS];
-    __shared__ float s_bj[64 * FWD_ROWS];
+__global__ __launch_bounds__(256) void k_adv_fwd(DevGrid g, float dt, const unsigned long long* __restrict__ beta_acc,
+                                                 int* __restrict__ fq0, float* __restrict__ fw,
+                                                 unsigned long long* gamma_acc) {
+    __shared__ unsigned short s_slot[256];
+    __shared__ float s_bj[256];
     __shared__ int s_n;
 #if EFS_FWD_BOX
     __shared__ ScatterBox box;
 #endif
-    const int lane = threadIdx.x, t = threadIdx.x + 64 * threadIdx.y;
+    const int t = threadIdx.x + ABX * threadIdx.y, lane = t & 63;
     if (t == 0) s_n = 0;
 #if EFS_FWD_BOX
     sbox_clear(box, t);
 #endif
     __syncthreads();
-    const int x0 = 1 + blockIdx.x * 64, y0 = 1 + blockIdx.y * FWD_ROWS, z = 1 + blockIdx.z;
+    const int x0 = 1 + blockIdx.x * ABX, y0 = 1 + blockIdx.y * ABY, z = 1 + blockIdx.z;
     bool act = false;
     float bj = 0.f;
-    if (x0 + lane <= g.W - 2 && y0 + (int)threadIdx.y <= g.H - 2) {
-        const int c = cidx(g, x0 + lane, y0 + threadIdx.y, z);
+    if (x0 + (int)threadIdx.x <= g.W - 2 && y0 + (int)threadIdx.y <= g.H - 2) {
+        const int c = cidx(g, x0 + threadIdx.x, y0 + threadIdx.y, z);
         bj = (float)from_fix(beta_acc[c]);
         act = bj < 1;
         if (!act) fq0[c] = -1;
@@ -429,7 +433,7 @@ __global__ __launch_bounds__(64 * FWD_ROWS) void k_adv_fwd(DevGrid g, float dt, 
     if (t < s_n) {
         const int src = s_slot[t];
         bj = s_bj[t];
-        const int x = x0 + (src & 63), y = y0 + (src >> 6);
+        const int x = x0 + src % ABX, y = y0 + src / ABX;
         const int c = cidx(g, x, y, z);
         const float3 next = traceParticle(g, (float)x, (float)y, (float)z, +dt, 4);
         float m[8];
@@ -446,7 +450,7 @@ __global__ __launch_bounds__(64 * FWD_ROWS) void k_adv_fwd(DevGrid g, float dt, 
         }
     }
 #if EFS_FWD_BOX
-    sbox_vote(box, threadIdx.y, live, qx, qy, qz);
+    sbox_vote(box, t >> 6, live, qx, qy, qz);
     __syncthreads();
     const int3 o = sbox_origin(box);
     if (live) sbox_add(g, box, o, qx, qy, qz, q0, q, gamma_acc);
@@ -471,9 +475,9 @@ __global__ __launch_bounds__(256) void k_adv_rows(DevGrid g, const int* __restri
                                                   unsigned long long* __restrict__ gamma_acc, float* __restrict__ g6,
                                                   unsigned long long* beta_acc) {
     __shared__ ScatterBox box;
-    const int t = threadIdx.x + BX * threadIdx.y;
+    const int t = threadIdx.x + ABX * threadIdx.y;
     sbox_clear(box, t);
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, z = blockIdx.z;
+    const int x = blockIdx.x * ABX + threadIdx.x, y = blockIdx.y * ABY + threadIdx.y, z = blockIdx.z;
     bool live = false;
     int p0 = -1;
     int3 pc = make_int3(0, 0, 0);
@@ -504,7 +508,7 @@ __global__ __launch_bounds__(256) void k_adv_rows(DevGrid g, const int* __restri
             }
         }
     }
-    sbox_vote(box, threadIdx.y, live, pc.x, pc.y, pc.z);
+    sbox_vote(box, t >> 6, live, pc.x, pc.y, pc.z);
     __syncthreads();
     const int3 o = sbox_origin(box);
     if (live) sbox_add(g, box, o, pc.x, pc.y, pc.z, p0, q, beta_acc);
@@ -549,9 +553,9 @@ __global__ void k_fix_to_field(int n, unsigned long long* __restrict__ acc, floa
 __global__ __launch_bounds__(256) void k_adv_cols_fwd(DevGrid g, const int* __restrict__ fq0, const float* __restrict__ fw,
                                                       unsigned long long* gamma_acc, unsigned long long* rho_acc) {
     __shared__ ScatterBox box;
-    const int t = threadIdx.x + BX * threadIdx.y;
+    const int t = threadIdx.x + ABX * threadIdx.y;
     sbox_clear(box, t);
-    const int x = 1 + blockIdx.x * BX + threadIdx.x, y = 1 + blockIdx.y * BY + threadIdx.y, z = 1 + blockIdx.z;
+    const int x = 1 + blockIdx.x * ABX + threadIdx.x, y = 1 + blockIdx.y * ABY + threadIdx.y, z = 1 + blockIdx.z;
     bool live = false;
     int q0 = -1;
     int3 qc = make_int3(0, 0, 0);
@@ -574,7 +578,7 @@ __global__ __launch_bounds__(256) void k_adv_cols_fwd(DevGrid g, const int* __re
         }
     }
     // the gamma terms, then the rho terms through the same box
-    sbox_vote(box, threadIdx.y, live, qc.x, qc.y, qc.z);
+    sbox_vote(box, t >> 6, live, qc.x, qc.y, qc.z);
     __syncthreads();
     const int3 o = sbox_origin(box);
     if (live) sbox_add(g, box, o, qc.x, qc.y, qc.z, q0, qg, gamma_acc);
@@ -962,20 +966,20 @@ int efs_launch_sharpen(efs_sim* s) {
 int efs_launch_rho_advect(efs_sim* s) {
     const DevGrid g = s->grid();
     const dim3 gi = interior_grid(s), gf = full_grid(s), blk(BX, BY);
+    const dim3 ai = adv_interior_grid(s), af = adv_full_grid(s), ablk(ABX, ABY);
     const dim3 g1(blocks1d(s->N)), b1(256);
     const float dt = s->P.dt;
     // steps 1-4
-    hipLaunchKernelGGL(k_adv_bwd, gi, blk, 0, s->stream, g, dt, s->s1, s->bwd_p0, s->bwd_w, s->acc0, s->tu, s->tv, s->tw);
+    hipLaunchKernelGGL(k_adv_bwd, ai, ablk, 0, s->stream, g, dt, s->s1, s->bwd_p0, s->bwd_w, s->acc0, s->tu, s->tv, s->tw);
     // step 5 (+ forward half of step 6)
-    hipLaunchKernelGGL(k_adv_fwd, dim3((s->W - 2 + 63) / 64, (s->H - 2 + FWD_ROWS - 1) / FWD_ROWS, s->D - 2),
-                       dim3(64, FWD_ROWS), 0, s->stream, g, dt, s->acc0, s->fwd_p0, s->fwd_w, s->acc1);
+    hipLaunchKernelGGL(k_adv_fwd, ai, ablk, 0, s->stream, g, dt, s->acc0, s->fwd_p0, s->fwd_w, s->acc1);
     hipLaunchKernelGGL(k_fill_u64, g1, b1, 0, s->stream, s->N, s->acc0);
     // steps 6-8: gamma6 -> s1 (the advected gamma of step 1 is no longer needed), new beta
-    hipLaunchKernelGGL(k_adv_rows, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->s1, s->acc0);
+    hipLaunchKernelGGL(k_adv_rows, af, ablk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->s1, s->acc0);
     hipLaunchKernelGGL(k_adv_rows_fwd, gi, blk, 0, s->stream, g, s->s1, s->fwd_p0, s->fwd_w, s->acc0);
     hipLaunchKernelGGL(k_fix_to_field, g1, b1, 0, s->stream, s->N, s->acc0, s->beta);
     // steps 9-11
-    hipLaunchKernelGGL(k_adv_cols_fwd, gi, blk, 0, s->stream, g, s->fwd_p0, s->fwd_w, s->acc1, s->acc2);
+    hipLaunchKernelGGL(k_adv_cols_fwd, ai, ablk, 0, s->stream, g, s->fwd_p0, s->fwd_w, s->acc1, s->acc2);
     hipLaunchKernelGGL(k_adv_final, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->s1, s->acc1, s->acc2, s->s0);
     EFS_HIP(hipGetLastError());
     s->traces_valid = true;
