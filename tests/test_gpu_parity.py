@@ -141,6 +141,35 @@ def test_rho_advect(case):
     sim.close()
 
 
+@pytest.mark.parametrize("dims", [(22, 22, 22), (70, 22, 18)])
+def test_rho_advect_scattered_flow(hiplib, dims):
+    """Uniform flow (about 2.7 cells a step) in the lower half and random flow (about 4 cells a
+    step, N(0, 240) cells/s) in the upper half, so that the blocks of the scatter passes both sum
+    their corner terms in LDS and add the terms of lanes whose corners reach past their box
+    directly; (70, 22, 18) also has a partial block along x."""
+    sc = scenes.Scene(name="scatter", dims=dims, prims=[scenes._box((-1, -1, -1), (0.6, 0.5, 2))],
+                      dt=1.0 / 60.0, gravity_force=9.81 * 20, steps=2, seed=3)
+    state = oracle_state(sc, 2)
+    W, H, D = dims
+    rng = np.random.default_rng(7)
+    vel = np.empty((D, H, W, 3), np.float32)
+    vel[: D // 2] = np.array([60.0, -150.0, 30.0], np.float32)
+    vel[D // 2:] = rng.normal(0.0, 240.0, size=vel[D // 2:].shape).astype(np.float32)
+    state["vel"] = vel
+    sim, o = fresh(sc, state)
+    o.params.key64 = 1
+    o.set_params(o.params)
+    sim.rho_advect()
+    o.stage(3)
+    g = sim.grid()
+    dr, dg, db = maxdiff(g.rho(), o.field("rho")), maxdiff(g.gamma(), o.field("gamma")), maxdiff(g.beta(), o.field("beta"))
+    m0, m1 = float(state["rho"].astype(np.float64).sum()), float(g.rho().astype(np.float64).sum())
+    print(f"[scatter {dims}] rho_advect max diff rho {dr:.3e} gamma {dg:.3e} beta {db:.3e}; mass {m0} -> {m1}", flush=True)
+    assert dr < 1e-4 and dg < 1e-4 and db < 1e-4
+    assert abs(m1 - m0) <= 1e-4 * max(1.0, m0)
+    sim.close()
+
+
 def test_rho_sharpen(case):
     name, sc, state = case
     sim, o = fresh(sc, state)
