@@ -516,64 +516,54 @@ __global__ __launch_bounds__(256) void k_adv_rows(DevGrid g, const int* __restri
     sbox_flush(g, box, o, t, beta_acc);
 }
 
-// Steps 7-8 for the forward entries: divide by the gamma of their row and add the cell's own
-// column into beta.
-__global__ __launch_bounds__(256) void k_adv_rows_fwd(DevGrid g, const float* __restrict__ g6, const int* __restrict__ fq0,
-                                                      float* __restrict__ fw, unsigned long long* beta_acc) {
-    INTERIOR_XYZ(g);
-    const int q0 = fq0[c];
-    if (q0 < 0) return;
-    long long own = 0;
-    float e[8], gk[8];  // every load first, then the divisions in the same order
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        gk[k] = g6[q0 + cornerOffset(g, k)];
-        e[k] = fw[(size_t)k * g.N + c];
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        float a = e[k];
-        if (gk[k] > 1.e-5f) a /= gk[k];
-        fw[(size_t)k * g.N + c] = a;
-        own += to_fix(a);
-    }
-    if (own != 0) atomicAdd(beta_acc + c, (unsigned long long)own);
-}
-
-// beta field (all cells) from the fixed-point column sums; clears the accumulator.
-__global__ void k_fix_to_field(int n, unsigned long long* __restrict__ acc, float* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[i] = (float)from_fix(acc[i]);
-    acc[i] = 0ull;
-}
-
-// Steps 9-11 for the forward entries: divide by beta (column), add to the next row sums (gamma)
-// and to rho' = A rho.
-__global__ __launch_bounds__(256) void k_adv_cols_fwd(DevGrid g, const int* __restrict__ fq0, const float* __restrict__ fw,
+// Steps 7-11 for the forward entries, with the beta field.  After k_adv_rows the column sum of
+// every column j holds its backward entries; the thread of column j (all cells) adds its own
+// forward entries, divided by the gamma of their rows (steps 7-8), stores beta_j and clears the
+// accumulator.  A column with forward entries then divides them by beta_j and adds them to the
+// next row sums (gamma) and to rho' = A rho (steps 9-11) through the scatter box.  The scaled
+// entries stay in registers: nothing after this pass reads them.
+__global__ __launch_bounds__(256) void k_adv_fwd_cols(DevGrid g, const float* __restrict__ g6, const int* __restrict__ fq0,
+                                                      const float* __restrict__ fw, unsigned long long* __restrict__ beta_acc,
                                                       unsigned long long* gamma_acc, unsigned long long* rho_acc) {
     __shared__ ScatterBox box;
     const int t = threadIdx.x + ABX * threadIdx.y;
     sbox_clear(box, t);
-    const int x = 1 + blockIdx.x * ABX + threadIdx.x, y = 1 + blockIdx.y * ABY + threadIdx.y, z = 1 + blockIdx.z;
+    const int x = blockIdx.x * ABX + threadIdx.x, y = blockIdx.y * ABY + threadIdx.y, z = blockIdx.z;
     bool live = false;
     int q0 = -1;
     int3 qc = make_int3(0, 0, 0);
     long long qg[8], qr[8];
-    if (x <= g.W - 2 && y <= g.H - 2) {
+    if (x < g.W && y < g.H) {
         const int c = cidx(g, x, y, z);
-        q0 = fq0[c];
+        long long sum = (long long)beta_acc[c];
+        beta_acc[c] = 0ull;
+        q0 = fq0[c];  // -1 on the border and on columns without forward entries
         live = q0 >= 0;
+        float a[8];
         if (live) {
-            const float bj = g.beta[c];
+            float gk[8];  // every load first, then the divisions in the same order
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                gk[k] = g6[q0 + cornerOffset(g, k)];
+                a[k] = fw[(size_t)k * g.N + c];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (gk[k] > 1.e-5f) a[k] /= gk[k];
+                sum += to_fix(a[k]);
+            }
+        }
+        const float bj = (float)from_fix((unsigned long long)sum);
+        g.beta[c] = bj;
+        if (live) {
             const float rj = g.rho[c];
             qc = cell_coords(g, q0);
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                float a = fw[(size_t)k * g.N + c];
-                if (bj > 1.e-5f) a /= bj;
-                qg[k] = to_fix(a);
-                qr[k] = to_fix(a * rj);
+                float v = a[k];
+                if (bj > 1.e-5f) v /= bj;
+                qg[k] = to_fix(v);
+                qr[k] = to_fix(v * rj);
             }
         }
     }
@@ -965,7 +955,7 @@ int efs_launch_sharpen(efs_sim* s) {
 
 int efs_launch_rho_advect(efs_sim* s) {
     const DevGrid g = s->grid();
-    const dim3 gi = interior_grid(s), gf = full_grid(s), blk(BX, BY);
+    const dim3 gf = full_grid(s), blk(BX, BY);
     const dim3 ai = adv_interior_grid(s), af = adv_full_grid(s), ablk(ABX, ABY);
     const dim3 g1(blocks1d(s->N)), b1(256);
     const float dt = s->P.dt;
@@ -976,10 +966,9 @@ int efs_launch_rho_advect(efs_sim* s) {
     hipLaunchKernelGGL(k_fill_u64, g1, b1, 0, s->stream, s->N, s->acc0);
     // steps 6-8: gamma6 -> s1 (the advected gamma of step 1 is no longer needed), new beta
     hipLaunchKernelGGL(k_adv_rows, af, ablk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->s1, s->acc0);
-    hipLaunchKernelGGL(k_adv_rows_fwd, gi, blk, 0, s->stream, g, s->s1, s->fwd_p0, s->fwd_w, s->acc0);
-    hipLaunchKernelGGL(k_fix_to_field, g1, b1, 0, s->stream, s->N, s->acc0, s->beta);
-    // steps 9-11
-    hipLaunchKernelGGL(k_adv_cols_fwd, ai, ablk, 0, s->stream, g, s->fwd_p0, s->fwd_w, s->acc1, s->acc2);
+    // steps 7-11 for the forward entries, and beta
+    hipLaunchKernelGGL(k_adv_fwd_cols, af, ablk, 0, s->stream, g, s->s1, s->fwd_p0, s->fwd_w, s->acc0, s->acc1, s->acc2);
+    // steps 9-11 for the backward entries
     hipLaunchKernelGGL(k_adv_final, gf, blk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->s1, s->acc1, s->acc2, s->s0);
     EFS_HIP(hipGetLastError());
     s->traces_valid = true;
