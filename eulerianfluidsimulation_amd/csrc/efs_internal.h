@@ -53,7 +53,30 @@ __host__ __device__ inline int cidx(const DevGrid& g, int x, int y, int z) { ret
 
 template <class T> __device__ inline T clampv(T v, T lo, T hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
 
-constexpr int kBufDword3 = 0x00020000;  // gfx9 buffer descriptor word 3 for raw access
+constexpr int kBufDword3 = 0x00020000;
+
+// Workgroup i runs on XCD i mod 8 (the hardware deals workgroups round-robin over the 8 XCDs,
+// each with its own L2).  The velocity face pass and the pressure gradient, streaming stencils
+// that read the -z neighbour, take their block from a 1D grid in which every XCD owns a
+// contiguous range of a plane's block columns and walks it plane by plane, so that the plane
+// before and most neighbouring block rows were read through the same L2 (gradient 167 -> 118 us,
+// face pass 108 -> 87 us at 256^3).  The traces, the row codes and the other per-cell passes
+// measured slower so.
+// Blocks past the last column of a range exit at once.
+constexpr int NXCD = 8;
+struct XcdBlock {
+    int bx, by, bz;
+};
+__device__ inline bool xcd_block(int gx, int gy, XcdBlock& b) {
+    const int ncol = gx * gy, per = (ncol + NXCD - 1) / NXCD;
+    const int k = (int)blockIdx.x / NXCD, col = ((int)blockIdx.x % NXCD) * per + k % per;
+    b.bz = k / per;
+    b.bx = col % gx;
+    b.by = col / gx;
+    return col < ncol;
+}
+inline dim3 xcd_grid(int gx, int gy, int nz) { return dim3(NXCD * ((gx * gy + NXCD - 1) / NXCD) * nz); }
+  // gfx9 buffer descriptor word 3 for raw access
 
 __device__ inline long long to_fix(float v) { return (long long)rint((double)v * FIX_SCALE); }
 __device__ inline void fix_add(unsigned long long* acc, int i, float v) {

@@ -156,6 +156,9 @@ __global__ __launch_bounds__(CG_NT) void k_cg_init(Dom d, const uint16_t* __rest
             const double xv = x64[q], bv = (double)b[q];  // loaded together, then selected
             return (rcode[q] & RC_CG_ROW) ? xv : bv;
         };
+        // a neighbour that enters a row (its RC_NB bit: counted, so boundary <= 0.5, and
+        // rho > 0.5) is a CG row itself, so with x0 = 1 its x0 is the previous iterate
+        auto Xn = [&](int q) -> double { return x0 == 0 ? (double)b[q] : x64[q]; };
         if (x < d.x0 + d.nx && y < d.y0 + d.ny) {
             for (int z = zb; z < ze; z++) {
                 const int c = (z * d.H + y) * d.W + x;
@@ -166,8 +169,8 @@ __global__ __launch_bounds__(CG_NT) void k_cg_init(Dom d, const uint16_t* __rest
                     if (!(x0 == 1 && (rcode[c] & RC_CG_ROW))) x64[c] = xc;
                     xd[c] = 0.f;
                 }
-                const double t = apply_row_d(rcode[c], xc, X(c + n.xm), X(c + n.xp), X(c + n.ym), X(c + n.yp), X(c + n.zp),
-                                             X(c + n.zm));
+                const double t = apply_row_d(rcode[c], xc, Xn(c + n.xm), Xn(c + n.xp), Xn(c + n.ym), Xn(c + n.yp), Xn(c + n.zp),
+                                             Xn(c + n.zm));
                 const float rr = (float)(bc - t);
                 r[c] = rr;
                 d0[c] = rr;

@@ -25,26 +25,6 @@ constexpr int BX = 64, BY = 4;
 inline dim3 interior_grid(const efs_sim* s) { return dim3((s->W - 2 + BX - 1) / BX, (s->H - 2 + BY - 1) / BY, s->D - 2); }
 inline dim3 full_grid(const efs_sim* s) { return dim3((s->W + BX - 1) / BX, (s->H + BY - 1) / BY, s->D); }
 
-// Workgroup i runs on XCD i mod 8 (the hardware deals workgroups round-robin over the 8 XCDs,
-// each with its own L2).  The streaming stencils that read the -z neighbour (the velocity face
-// pass and the pressure gradient) take their block from a 1D grid in which every XCD owns a
-// contiguous range of a plane's block columns and walks it plane by plane, so that the plane
-// before and most neighbouring block rows were read through the same L2 (167 -> 118 us and
-// 108 -> 87 us at 256^3).  The traces and the other per-cell passes measured slower so.
-// Blocks past the last column of a range exit at once.
-constexpr int NXCD = 8;
-struct XcdBlock {
-    int bx, by, bz;
-};
-__device__ inline bool xcd_block(int gx, int gy, XcdBlock& b) {
-    const int ncol = gx * gy, per = (ncol + NXCD - 1) / NXCD;
-    const int k = (int)blockIdx.x / NXCD, col = ((int)blockIdx.x % NXCD) * per + k % per;
-    b.bz = k / per;
-    b.bx = col % gx;
-    b.by = col / gx;
-    return col < ncol;
-}
-inline dim3 xcd_grid(int gx, int gy, int nz) { return dim3(NXCD * ((gx * gy + NXCD - 1) / NXCD) * nz); }
 inline dim3 interior_grid_xcd(const efs_sim* s) { return xcd_grid((s->W - 2 + BX - 1) / BX, (s->H - 2 + BY - 1) / BY, s->D - 2); }
 inline dim3 full_grid_xcd(const efs_sim* s) { return xcd_grid((s->W + BX - 1) / BX, (s->H + BY - 1) / BY, s->D); }
 
