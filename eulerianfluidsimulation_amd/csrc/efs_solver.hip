@@ -538,16 +538,21 @@ __global__ __launch_bounds__(TR_NT) void k_true_residual(Dom d, const uint16_t* 
     if (block_coords(d, bx, by, bz)) {
         const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * d.ty + threadIdx.y;
         const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
-        if (x < d.x0 + d.nx && y < d.y0 + d.ny) {
+        if (x < d.x0 + d.nx && y < d.y0 + d.ny && zb < ze) {
+            auto X = [&](int q) -> double { return xdbl ? xdbl[q] : (double)xf[q]; };
+            // the cell's value and its -z neighbour's are the last plane's cell and +z values
+            const int c0 = (zb * d.H + y) * d.W + x;
+            double xc = X(c0), xzm = X(c0 + nb_offsets(d, x, y, zb).zm);
             for (int z = zb; z < ze; z++) {
                 const int c = (z * d.H + y) * d.W + x;
                 const Nb n = nb_offsets(d, x, y, z);
-                auto X = [&](int q) -> double { return xdbl ? xdbl[q] : (double)xf[q]; };
-                const double t = apply_row_d(rcode[c], X(c), X(c + n.xm), X(c + n.xp), X(c + n.ym), X(c + n.yp), X(c + n.zp),
-                                             X(c + n.zm));
+                const double xzp = X(c + n.zp);
+                const double t = apply_row_d(rcode[c], xc, X(c + n.xm), X(c + n.xp), X(c + n.ym), X(c + n.yp), xzp, xzm);
                 const double rr = (double)b[c] - t;
                 v[0] += rr * rr;
                 v[1] += (double)b[c] * (double)b[c];
+                xzm = xc;
+                xc = xzp;
             }
         }
     }
