@@ -152,6 +152,13 @@ __global__ __launch_bounds__(256) void k_sharpen_delta(DevGrid g, float delT, fl
     INTERIOR_XYZ(g);
     const int sy = g.W, sz = g.W * g.H;
     const float r = g.rho[c];
+    // The tests at the end decide these cells without the rest: rho < 1e-5 gives -rho; rho > 0.5
+    // gives 0, since then the weight is >= 0 or NaN and the increment is >= 0 or NaN, so
+    // rho + dr < 0 never holds.  Air and liquid waves stop here; NaN goes on as before.
+    if (r > 0.5f || (double)r < 1e-5) {
+        del[c] = ((double)r < 1e-5) ? -r : 0.f;
+        return;
+    }
     const float rxm = g.rho[c - 1], rxp = g.rho[c + 1], rym = g.rho[c - sy], ryp = g.rho[c + sy];
     const float rzm = g.rho[c - sz], rzp = g.rho[c + sz];
     const float dpx = -(r - rxm) * delT, dmx = -(rxp - r) * delT;
