@@ -445,24 +445,44 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
 }
 
 // residual b - s A x of level f (x stored, FP32) restricted to level c: b_c = 1/8 of the sum of
-// the fluid children's residuals (child order x, y, z), 0 on non-fluid coarse cells
+// the fluid children's residuals (child order x, y, z), 0 on non-fluid coarse cells.  A block
+// takes a 64 x 4 tile of fine cells in the two fine planes of one coarse plane: each thread
+// computes the residuals of its two cells with row-coalesced loads, and the tile's 32 x 2 coarse
+// cells then add their children from LDS in the child order.  (One thread per coarse cell reading
+// its 8 children itself made every load a stride-2 gather.)
 __global__ __launch_bounds__(256) void k_mg_restrict_x(Lvl f, const float* __restrict__ xf, Lvl c, const CGState* st) {
     if (st->done) return;
-    const int x = 1 + blockIdx.x * MB + threadIdx.x, y = 1 + blockIdx.y * MBY + threadIdx.y, z = 1 + blockIdx.z;
-    if (x > c.nx || y > c.ny) return;
-    const int ci = lidx(c, x, y, z);
-    float acc = 0.f;
-    if (c.cls[ci] == CLS_FLUID) {
-        const int sy = f.W, sz = f.W * f.H;
+    __shared__ float res[2][MBY][MB];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int fx = 1 + blockIdx.x * MB + tx, fy = 1 + blockIdx.y * MBY + ty, Z = 1 + blockIdx.z;
+    const int sy = f.W, sz = f.W * f.H;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int fx = 2 * x - 1 + (k & 1), fy = 2 * y - 1 + ((k >> 1) & 1), fz = 2 * z - 1 + (k >> 2);
-            if (fx > f.nx || fy > f.ny || fz > f.nz) continue;
-            const int i = lidx(f, fx, fy, fz);
-            const unsigned rc = f.rc[i];
+    for (int dz = 0; dz < 2; dz++) {
+        const int fz = 2 * Z - 1 + dz;
+        float r = 0.f;
+        const int i = lidx(f, fx, fy, fz);
+        const unsigned rc = (fx <= f.nx && fy <= f.ny && fz <= f.nz) ? f.rc[i] : 0u;
+        if (rc & RC_CG_ROW) {  // other rows add 0 (air waves skip the loads)
             const float v[6] = {xf[i - 1], xf[i + 1], xf[i - sy], xf[i + sy], xf[i + sz], xf[i - sz]};
             const float ax = f.s * (masked_sum6(rc, v) - (float)((rc >> RC_CNT_SHIFT) & 7u) * xf[i]);
-            acc += (rc & RC_CG_ROW) ? f.b[i] - ax : 0.f;
+            r = f.b[i] - ax;
+        }
+        res[dz][ty][tx] = r;
+    }
+    __syncthreads();
+    const int t = tx + MB * ty;
+    if (t >= (MB / 2) * (MBY / 2)) return;
+    const int cx = t % (MB / 2), cy = t / (MB / 2);
+    const int X = 1 + blockIdx.x * (MB / 2) + cx, Y = 1 + blockIdx.y * (MBY / 2) + cy;
+    if (X > c.nx || Y > c.ny) return;
+    const int ci = lidx(c, X, Y, Z);
+    float acc = 0.f;
+    if (c.cls[ci] == CLS_FLUID) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+            if (2 * X - 1 + dx > f.nx || 2 * Y - 1 + dy > f.ny || 2 * Z - 1 + dz > f.nz) continue;
+            acc += res[dz][2 * cy + dy][2 * cx + dx];
         }
         acc *= 0.125f;
     }
@@ -635,7 +655,7 @@ int vcycle(efs_sim* s, void*) {
         if (EFS_MG_NU == 2) {
             hipLaunchKernelGGL(k_mg_sweep<SW_DOWN0>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, nullptr, l.x2);
             hipLaunchKernelGGL(k_mg_sweep<SW_RB>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, l.x2, l.x);
-            hipLaunchKernelGGL(k_mg_restrict_x, dim3((c.nx + MB - 1) / MB, (c.ny + MBY - 1) / MBY, c.nz), dim3(MB, MBY), 0, s->stream,
+            hipLaunchKernelGGL(k_mg_restrict_x, dim3((l.nx + MB - 1) / MB, (l.ny + MBY - 1) / MBY, c.nz), dim3(MB, MBY), 0, s->stream,
                                l, l.x, c, st);
         } else {
             hipLaunchKernelGGL(k_mg_sweep<SW_DOWN>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, nullptr, nullptr);
