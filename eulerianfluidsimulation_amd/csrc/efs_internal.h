@@ -53,7 +53,7 @@ __host__ __device__ inline int cidx(const DevGrid& g, int x, int y, int z) { ret
 
 template <class T> __device__ inline T clampv(T v, T lo, T hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
 
-constexpr int kBufDword3 = 0x00020000;
+constexpr int kBufDword3 = 0x00020000;  // gfx9 buffer descriptor word 3 for raw access
 
 // Workgroup i runs on XCD i mod 8 (the hardware deals workgroups round-robin over the 8 XCDs,
 // each with its own L2).  The velocity face pass and the pressure gradient, streaming stencils
@@ -61,7 +61,7 @@ constexpr int kBufDword3 = 0x00020000;
 // contiguous range of a plane's block columns and walks it plane by plane, so that the plane
 // before and most neighbouring block rows were read through the same L2 (gradient 167 -> 118 us,
 // face pass 108 -> 87 us at 256^3).  The traces, the row codes and the other per-cell passes
-// measured slower so.
+// measured slower so, and so did a division-free variant that gives each XCD whole block rows.
 // Blocks past the last column of a range exit at once.
 constexpr int NXCD = 8;
 struct XcdBlock {
@@ -76,7 +76,6 @@ __device__ inline bool xcd_block(int gx, int gy, XcdBlock& b) {
     return col < ncol;
 }
 inline dim3 xcd_grid(int gx, int gy, int nz) { return dim3(NXCD * ((gx * gy + NXCD - 1) / NXCD) * nz); }
-  // gfx9 buffer descriptor word 3 for raw access
 
 __device__ inline long long to_fix(float v) { return (long long)rint((double)v * FIX_SCALE); }
 __device__ inline void fix_add(unsigned long long* acc, int i, float v) {
