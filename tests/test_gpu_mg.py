@@ -13,6 +13,8 @@ CASES = {
     "pool_boxes_48": lambda: scenes.sphere_pool(48, seed=0),
     "dam_48": lambda: scenes.dam3d(48),
     "dam_spheres_40": lambda: scenes.dam_spheres(40, seed=2),
+    # odd sizes on every level (45, 23, 12): coarse cells on the high faces have one child per axis
+    "pool_boxes_45": lambda: scenes.sphere_pool(45, seed=3),
 }
 
 
