@@ -390,6 +390,6 @@ void efs_learned_free(efs_sim* s);
 
 // event-timed kernels: KT_OTHER times the fine-level multigrid up sweep, KT_MG_DOWN its down sweep
 enum { KT_CG_DIR_SPMV = 0, KT_CG_UPDATE = 1, KT_JACOBI = 2, KT_OTHER = 3, KT_MG_DOWN = 4 };
-void efs_ktimer_begin(efs_sim* s, int k, hipEvent_t* a, hipEvent_t* b);
-void efs_ktimer_end(efs_sim* s, int k, hipEvent_t a, hipEvent_t b);
+void efs_ktimer_pair(efs_sim* s, hipEvent_t* a, hipEvent_t* b);  // for hipExtLaunchKernelGGL's start/stop events
+void efs_ktimer_add(efs_sim* s, int k, hipEvent_t a, hipEvent_t b);
 void efs_trim_timer(TimerSlot& t, size_t keep);  // drops the events of launches past `keep`

@@ -28,6 +28,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "efs_internal.h"
 #include "efs_stencil.h"
 
@@ -651,16 +653,20 @@ int vcycle(efs_sim* s, void*) {
         const Dom d = post_dom(l);
         const dim3 grid(launch_blocks(d));
         const float is = 1.f / l.s;
-        if (k == 0) efs_ktimer_begin(s, KT_MG_DOWN, &ta, &tb);
+        // the fine level's down bracket is timed from the start of its first kernel to the end of its last
+        ta = tb = nullptr;
+        if (k == 0) efs_ktimer_pair(s, &ta, &tb);
         if (EFS_MG_NU == 2) {
-            hipLaunchKernelGGL(k_mg_sweep<SW_DOWN0>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, nullptr, l.x2);
+            hipExtLaunchKernelGGL(k_mg_sweep<SW_DOWN0>, grid, pb, 0, s->stream, ta, nullptr, 0, l, c, is, d, st, (double*)nullptr,
+                                  (const float*)nullptr, l.x2);
             hipLaunchKernelGGL(k_mg_sweep<SW_RB>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, l.x2, l.x);
-            hipLaunchKernelGGL(k_mg_restrict_x, dim3((l.nx + MB - 1) / MB, (l.ny + MBY - 1) / MBY, c.nz), dim3(MB, MBY), 0, s->stream,
-                               l, l.x, c, st);
+            hipExtLaunchKernelGGL(k_mg_restrict_x, dim3((l.nx + MB - 1) / MB, (l.ny + MBY - 1) / MBY, c.nz), dim3(MB, MBY), 0,
+                                  s->stream, nullptr, tb, 0, l, (const float*)l.x, c, st);
         } else {
-            hipLaunchKernelGGL(k_mg_sweep<SW_DOWN>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, nullptr, nullptr);
+            hipExtLaunchKernelGGL(k_mg_sweep<SW_DOWN>, grid, pb, 0, s->stream, ta, tb, 0, l, c, is, d, st, (double*)nullptr,
+                                  (const float*)nullptr, (float*)nullptr);
         }
-        if (k == 0) efs_ktimer_end(s, KT_MG_DOWN, ta, tb);
+        if (k == 0) efs_ktimer_add(s, KT_MG_DOWN, ta, tb);
     }
     hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(MG_CO_NT), 0, s->stream, m->L[nl - 1], 1.f / m->L[nl - 1].s,
                        EFS_MG_COARSE_SWEEPS, st);
@@ -672,20 +678,26 @@ int vcycle(efs_sim* s, void*) {
         const float is = 1.f / l.s;
         const bool rz = k == 0 && m->fused_rz;
         double* prz = rz ? m->p_rz : nullptr;
-        if (k == 0) efs_ktimer_begin(s, KT_OTHER, &ta, &tb);
+        ta = tb = nullptr;
+        if (k == 0) efs_ktimer_pair(s, &ta, &tb);
         if (EFS_MG_NU == 2) {
-            hipLaunchKernelGGL(k_mg_sweep<SW_BR_E>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, l.x, l.x2);
+            hipExtLaunchKernelGGL(k_mg_sweep<SW_BR_E>, grid, pb, 0, s->stream, ta, nullptr, 0, l, c, is, d, st, (double*)nullptr,
+                                  (const float*)l.x, l.x2);
             if (rz)
-                hipLaunchKernelGGL(k_mg_sweep<SW_BR_RZ>, grid, pb, 0, s->stream, l, c, is, d, st, prz, l.x2, l.x);
+                hipExtLaunchKernelGGL(k_mg_sweep<SW_BR_RZ>, grid, pb, 0, s->stream, nullptr, tb, 0, l, c, is, d, st, prz,
+                                      (const float*)l.x2, l.x);
             else
-                hipLaunchKernelGGL(k_mg_sweep<SW_BR>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, l.x2, l.x);
+                hipExtLaunchKernelGGL(k_mg_sweep<SW_BR>, grid, pb, 0, s->stream, nullptr, tb, 0, l, c, is, d, st, (double*)nullptr,
+                                      (const float*)l.x2, l.x);
         } else {
             if (rz)
-                hipLaunchKernelGGL(k_mg_sweep<SW_UP_RZ>, grid, pb, 0, s->stream, l, c, is, d, st, prz, nullptr, l.x);
+                hipExtLaunchKernelGGL(k_mg_sweep<SW_UP_RZ>, grid, pb, 0, s->stream, ta, tb, 0, l, c, is, d, st, prz,
+                                      (const float*)nullptr, l.x);
             else
-                hipLaunchKernelGGL(k_mg_sweep<SW_UP>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, nullptr, l.x);
+                hipExtLaunchKernelGGL(k_mg_sweep<SW_UP>, grid, pb, 0, s->stream, ta, tb, 0, l, c, is, d, st, (double*)nullptr,
+                                      (const float*)nullptr, l.x);
         }
-        if (k == 0) efs_ktimer_end(s, KT_OTHER, ta, tb);
+        if (k == 0) efs_ktimer_add(s, KT_OTHER, ta, tb);
     }
     EFS_HIP(hipGetLastError());
     return 0;
