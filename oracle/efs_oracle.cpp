@@ -486,9 +486,11 @@ template <class T> struct Sim3D {
     }
 
     // ---- CPUSimulator3D::rho_sharpen (CPUSimulator3D.cpp:366-456) ----
-    void rho_sharpen() {
+    // Delta rho of every interior cell (CPUSimulator3D.cpp:366-444), before any is applied.
+    void sharpen_delta(std::vector<T>& del_rho) const {
         const T delT = (T)(P.dt * 3);  // PROPERTY_F(DT) * 3 is a float product
-        std::vector<T> del_rho(N, T(0)), max_inc(N, T(0)), max_dec(N, T(0)), w(N, T(0));
+        del_rho.assign(N, T(0));
+        std::vector<T> max_inc(N, T(0)), max_dec(N, T(0)), w(N, T(0));
         std::vector<V3<T>> dp(N), dm(N);
         for (int z = 1; z < D - 1; z++)
             for (int y = 1; y < H - 1; y++)
@@ -537,6 +539,10 @@ template <class T> struct Sim3D {
                     if (rho[c] + del_rho[c] < 0 || rho[c] < 1e-5) del_rho[c] = -rho[c];
                     else if (rho[c] > T(0.5f)) del_rho[c] = 0;
                 }
+    }
+    void rho_sharpen() {
+        std::vector<T> del_rho;
+        sharpen_delta(del_rho);
         const T rhoD = (T)P.rho_d;
         if (P.sharpen_mode == 0) {
             for (int z = 1; z < D - 1; z++)
@@ -1107,9 +1113,11 @@ template <class T> struct Sim2D {
         }
         gamma[jj] = avg; gamma[ii] = avg;
     }
-    void rho_sharpen() {  // CPUSimulator2D.cpp:320-402
+    // Delta rho of every interior cell (CPUSimulator2D.cpp:320-390), before any is applied.
+    void sharpen_delta(std::vector<T>& del_rho) const {
         const T delT = (T)(P.dt * 3);
-        std::vector<T> del_rho(N, 0), mi(N, 0), md(N, 0), w(N, 0);
+        del_rho.assign(N, T(0));
+        std::vector<T> mi(N, 0), md(N, 0), w(N, 0);
         std::vector<V2<T>> dp(N), dm(N);
         for (int y = 1; y < H - 1; y++)
             for (int x = 1; x < W - 1; x++) {
@@ -1145,6 +1153,10 @@ template <class T> struct Sim2D {
                 if (rho[c] + del_rho[c] < 0 || rho[c] < 1e-5) del_rho[c] = -rho[c];
                 else if (rho[c] > T(0.5f)) del_rho[c] = 0;
             }
+    }
+    void rho_sharpen() {  // CPUSimulator2D.cpp:320-402
+        std::vector<T> del_rho;
+        sharpen_delta(del_rho);
         if (P.sharpen_mode == 0) {
             for (int y = 1; y < H - 1; y++)
                 for (int x = 1; x < W - 1; x++) {
@@ -1281,14 +1293,8 @@ template <class T> struct Sim2D {
                 if (rho[c] < T(1.f)) divergence[c] += std::min(lp * (T(1.f) - rho[c]), eta);
             }
     }
-    int project(ora_stats* st) {  // CPUSimulator2D.cpp:450-524
-        std::fill(pressure.begin(), pressure.end(), T(0));
-        std::fill(divergence.begin(), divergence.end(), T(0));
-        enforce_wall_boundaries();
-        compute_divergence();
-        const T err = (P.solver_type == 3) ? jacobi_solve() : cg_solve();
-        if (st) { st->err = (double)err; st->iterations = solve_steps; }
-        if (err > (T)P.solver_threshold) { paused = true; return 1; }
+    // face-wise gradient (CPUSimulator2D.cpp:491-501): a face touching a solid is set to 0
+    void subtract_pressure_gradient() {
         for (int y = 1; y < H - 1; y++)
             for (int x = 1; x < W - 1; x++) {
                 const size_t c = id(x, y);
@@ -1297,6 +1303,16 @@ template <class T> struct Sim2D {
                 if (boundary[c] != T(1.f) && boundary[id(x, y - 1)] != T(1.f)) vel[c].y -= (pressure[c] - pressure[id(x, y - 1)]);
                 else vel[c].y = 0;
             }
+    }
+    int project(ora_stats* st) {  // CPUSimulator2D.cpp:450-524
+        std::fill(pressure.begin(), pressure.end(), T(0));
+        std::fill(divergence.begin(), divergence.end(), T(0));
+        enforce_wall_boundaries();
+        compute_divergence();
+        const T err = (P.solver_type == 3) ? jacobi_solve() : cg_solve();
+        if (st) { st->err = (double)err; st->iterations = solve_steps; }
+        if (err > (T)P.solver_threshold) { paused = true; return 1; }
+        subtract_pressure_gradient();
         if (P.re_calc_div) compute_divergence();
         enforce_wall_boundaries();
         return std::isfinite((double)err) ? 0 : 2;
@@ -1544,6 +1560,38 @@ size_t ora3_num_entries(void* s, int dbl) {
     return S->P.key64 ? S->A64.size() : S->A32.size();
 }
 
+// Pieces of the step in isolation, for the oracle's own pins.
+// project()'s gradient subtraction (CPUSimulator3D.cpp:550-563) on the current pressure.
+void ora3_gradient(void* s, int dbl) {
+    if (dbl) ((Sim3D<double>*)s)->subtract_pressure_gradient();
+    else ((Sim3D<float>*)s)->subtract_pressure_gradient();
+}
+// rho_sharpen's delta rho (CPUSimulator3D.cpp:366-444) into tmpScal; rho is not changed.
+void ora3_sharpen_delta(void* s, int dbl) {
+    auto run = [&](auto* S) { S->sharpen_delta(S->tmpScal); };
+    if (dbl) run((Sim3D<double>*)s);
+    else run((Sim3D<float>*)s);
+}
+// trace_along_field (CPUSimulator3D.cpp:568-607) from start with the RHO_D parameter, 10 steps.
+void ora3_trace_field(void* s, int dbl, const double* start, double* out) {
+    auto run = [&](auto* S, auto tag) {
+        using T = decltype(tag);
+        const V3<T> r = S->trace_along_field(V3<T>((T)start[0], (T)start[1], (T)start[2]), (T)S->P.rho_d, 10);
+        out[0] = r.x; out[1] = r.y; out[2] = r.z;
+    };
+    if (dbl) run((Sim3D<double>*)s, 0.0);
+    else run((Sim3D<float>*)s, 0.0f);
+}
+// scatter_values (CPUSimulator3D.cpp:609-655) of value at pos into rho.
+void ora3_scatter(void* s, int dbl, const double* pos, double value) {
+    auto run = [&](auto* S, auto tag) {
+        using T = decltype(tag);
+        S->scatter_values(S->rho, V3<T>((T)pos[0], (T)pos[1], (T)pos[2]), (T)value);
+    };
+    if (dbl) run((Sim3D<double>*)s, 0.0);
+    else run((Sim3D<float>*)s, 0.0f);
+}
+
 // ---- 2D ----
 void* ora2_create(int w, int h, int dbl) {
     if (dbl) return new Sim2D<double>(w, h);
@@ -1601,6 +1649,48 @@ void ora2_matmul(void* s, int dbl, const void* x, void* y) {
         std::vector<T> xv((const T*)x, (const T*)x + S->N), yv(S->N);
         S->mat_mul(yv, xv);
         std::memcpy(y, yv.data(), S->N * sizeof(T));
+    };
+    if (dbl) run((Sim2D<double>*)s, 0.0);
+    else run((Sim2D<float>*)s, 0.0f);
+}
+
+void ora2_gradient(void* s, int dbl) {
+    if (dbl) ((Sim2D<double>*)s)->subtract_pressure_gradient();
+    else ((Sim2D<float>*)s)->subtract_pressure_gradient();
+}
+void ora2_enforce_walls(void* s, int dbl) {
+    if (dbl) ((Sim2D<double>*)s)->enforce_wall_boundaries();
+    else ((Sim2D<float>*)s)->enforce_wall_boundaries();
+}
+void ora2_divergence(void* s, int dbl) {
+    auto run = [&](auto* S) {
+        std::fill(S->divergence.begin(), S->divergence.end(), 0);
+        S->enforce_wall_boundaries();
+        S->compute_divergence();
+    };
+    if (dbl) run((Sim2D<double>*)s);
+    else run((Sim2D<float>*)s);
+}
+void ora2_sharpen_delta(void* s, int dbl) {
+    auto run = [&](auto* S) { S->sharpen_delta(S->tmpScal); };
+    if (dbl) run((Sim2D<double>*)s);
+    else run((Sim2D<float>*)s);
+}
+double ora2_sinterp(void* s, int dbl, int field, const double* pos) {
+    auto run = [&](auto* S, auto tag) -> double {
+        using T = decltype(tag);
+        auto* f = (T*)ora2_field(s, dbl, field);
+        std::vector<T> v(f, f + S->N);
+        return (double)S->sinterp(v, V2<T>((T)pos[0], (T)pos[1]));
+    };
+    if (dbl) return run((Sim2D<double>*)s, 0.0);
+    return run((Sim2D<float>*)s, 0.0f);
+}
+void ora2_trace(void* s, int dbl, const double* start, double dt, int substeps, double* out) {
+    auto run = [&](auto* S, auto tag) {
+        using T = decltype(tag);
+        const V2<T> r = S->traceParticle(V2<T>((T)start[0], (T)start[1]), (T)dt, substeps);
+        out[0] = r.x; out[1] = r.y;
     };
     if (dbl) run((Sim2D<double>*)s, 0.0);
     else run((Sim2D<float>*)s, 0.0f);

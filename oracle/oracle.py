@@ -64,6 +64,11 @@ def load():
         "ora2_set_params": (None, [vp, i32, P(OraParams)]), "ora2_field": (vp, [vp, i32, i32]),
         "ora2_step": (i32, [vp, i32, P(OraStats)]), "ora2_stage": (i32, [vp, i32, i32, P(OraStats)]),
         "ora2_solve": (i32, [vp, i32, P(OraStats)]), "ora2_matmul": (None, [vp, i32, vp, vp]),
+        "ora3_gradient": (None, [vp, i32]), "ora3_sharpen_delta": (None, [vp, i32]),
+        "ora3_trace_field": (None, [vp, i32, P(f64), P(f64)]), "ora3_scatter": (None, [vp, i32, P(f64), f64]),
+        "ora2_gradient": (None, [vp, i32]), "ora2_enforce_walls": (None, [vp, i32]),
+        "ora2_divergence": (None, [vp, i32]), "ora2_sharpen_delta": (None, [vp, i32]),
+        "ora2_sinterp": (f64, [vp, i32, i32, P(f64)]), "ora2_trace": (None, [vp, i32, P(f64), f64, i32, P(f64)]),
         "ora_uniform": (f64, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
         "ora_normal": (f64, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
         "ora_rasterise": (None, [P(i32), P(OraPrim), i32, vp, vp]),
@@ -181,6 +186,28 @@ class Oracle3D:
     def num_entries(self):
         return self.lib.ora3_num_entries(self.h, self.dbl)
 
+    def gradient(self):
+        """project()'s pressure-gradient subtraction alone, on the current pressure."""
+        self.lib.ora3_gradient(self.h, self.dbl)
+
+    def enforce_walls(self):
+        self.lib.ora3_enforce_walls(self.h, self.dbl)
+
+    def sharpen_delta(self):
+        """rho_sharpen's per-cell delta rho (before any is applied); rho is unchanged."""
+        self.lib.ora3_sharpen_delta(self.h, self.dbl)
+        return self.field("tmpscal").copy()
+
+    def trace_field(self, start):
+        s = (ctypes.c_double * 3)(*start)
+        o = (ctypes.c_double * 3)()
+        self.lib.ora3_trace_field(self.h, self.dbl, s, o)
+        return np.array(list(o))
+
+    def scatter(self, pos, value):
+        s = (ctypes.c_double * 3)(*pos)
+        self.lib.ora3_scatter(self.h, self.dbl, s, value)
+
     def load_scene(self, scene):
         """Rasterise the scene's primitives and add the seeded face noise (float inputs)."""
         W, H, D = self.dims
@@ -247,6 +274,29 @@ class Oracle2D:
         y = np.empty_like(x)
         self.lib.ora2_matmul(self.h, self.dbl, x.ctypes.data, y.ctypes.data)
         return y
+
+    def gradient(self):
+        self.lib.ora2_gradient(self.h, self.dbl)
+
+    def enforce_walls(self):
+        self.lib.ora2_enforce_walls(self.h, self.dbl)
+
+    def divergence(self):
+        self.lib.ora2_divergence(self.h, self.dbl)
+
+    def sharpen_delta(self):
+        self.lib.ora2_sharpen_delta(self.h, self.dbl)
+        return self.field("tmpscal").copy()
+
+    def sinterp(self, name, pos):
+        s = (ctypes.c_double * 2)(*pos)
+        return self.lib.ora2_sinterp(self.h, self.dbl, FIELDS[name], s)
+
+    def trace(self, start, dt, substeps=4):
+        s = (ctypes.c_double * 2)(*start)
+        o = (ctypes.c_double * 2)()
+        self.lib.ora2_trace(self.h, self.dbl, s, dt, substeps, o)
+        return np.array(list(o))
 
     def load_scene(self, scene):
         W, H = self.dims
