@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(HERE, "libefs_hip.so")
 FIELD_VEL, FIELD_RHO, FIELD_GAMMA, FIELD_BETA, FIELD_DIVERGENCE, FIELD_PRESSURE, FIELD_FORCES, FIELD_BOUNDARY = range(8)
 STAGE_VEL_DECAY, STAGE_VEL_EXTRAPOLATE, STAGE_RHO_SHARPEN, STAGE_RHO_ADVECT, STAGE_VEL_ADVECT, STAGE_ADD_FORCE, STAGE_PROJECT = range(7)
 STAGE_NAMES = ["0 Vel Decay", "1 Vel Extrapolate", "2 Rho Sharpen", "3 Rho Advect", "4 Vel Advect", "5 Add Force", "6 Project"]
-SOLVER_SELF_CG, SOLVER_JACOBI = 2, 3
+SOLVER_SELF_CG, SOLVER_JACOBI, SOLVER_MULTIGRID = 2, 3, 4
 CRIT_REFERENCE, CRIT_RELATIVE = 0, 1
 PRECOND_NONE, PRECOND_LEARNED, PRECOND_MG = 0, 1, 2
 GUESS_DIVERGENCE, GUESS_PREVIOUS = 0, 1
@@ -95,6 +95,7 @@ SIGNATURES = {
     "efs_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "efs_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
     "efs_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
+    "efs_available_solvers": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     "efs_dims": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "efs_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]),
     "efs_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]),

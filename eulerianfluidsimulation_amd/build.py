@@ -33,18 +33,21 @@ FLAGS = [
 
 HOST_DIR = os.path.join(HERE, "host")
 HOST_BIN = os.path.join(HOST_DIR, "efs_run")
+MIRROR_BIN = os.path.join(HOST_DIR, "efs_mirror_check")
 
 
 def build_host(verbose: bool = False) -> str:
-    """C++ host driver over the C++ adapter (HipSimulator3D.h), linked against the library."""
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", os.path.join(HOST_DIR, "efs_run.cpp"), "-o", HOST_BIN,
-           "-L", HERE, "-lefs_hip", "-Wl,-rpath,$ORIGIN/.."]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("g++ failed building the host driver")
+    """C++ host programs over the C++ adapter (HipSimulator3D.h), linked against the library: the
+    driver efs_run and the adapter's mirror self-check efs_mirror_check."""
+    for src, out in (("efs_run.cpp", HOST_BIN), ("efs_mirror_check.cpp", MIRROR_BIN)):
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", os.path.join(HOST_DIR, src), "-o", out,
+               "-L", HERE, "-lefs_hip", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed building {src}")
     return HOST_BIN
 
 

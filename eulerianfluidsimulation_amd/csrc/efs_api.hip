@@ -229,20 +229,55 @@ void stage_event(efs_sim* s, int k) {
     hipEventRecord(s->ev_stage[k], s->stream);
 }
 
+// The pressure solve ISolver3D::solve dispatches to (CPUSimulator3D.cpp:18-40 picks the solver
+// from SOLVER_TYPE): Jacobi, CG, or CG preconditioned by multigrid (the reference's MULTIGRID id,
+// Properties.h:481, which no reference backend implements) or by the learned network.
+int solve_3d(efs_sim* s, efs_stats* out) {
+    if (s->P.solver_type == EFS_SOLVER_JACOBI) return efs_jacobi_solve(s, out);
+    if (s->P.solver_type == EFS_SOLVER_MULTIGRID) return efs_mg_cg_solve(s, out);
+    if (s->P.precond == EFS_PRECOND_LEARNED) return efs_learned_cg_solve(s, out);
+    if (s->P.precond == EFS_PRECOND_MG) return efs_mg_cg_solve(s, out);
+    return efs_cg_solve(s, out);
+}
+
+// The solvers a backend offers, in the order ISimulator::availableSolvers takes them
+// (ISimulator.cpp:5-30: an unavailable SOLVER_TYPE falls back to the first one).
+int available_solvers(int dim, int32_t* out) {
+    out[0] = EFS_SOLVER_SELF_CG;
+    out[1] = EFS_SOLVER_JACOBI;
+    if (dim == 2) return 2;
+    out[2] = EFS_SOLVER_MULTIGRID;
+    return 3;
+}
+
+bool check_params(int dim, const efs_params* p) {
+    int32_t av[4];
+    const int n = available_solvers(dim, av);
+    bool ok = false;
+    for (int k = 0; k < n; k++) ok |= p->solver_type == av[k];
+    if (!ok) {
+        efs_set_error(dim == 2 ? "solver_type must be SELF_CG (2) or JACOBI (3) in 2D"
+                               : "solver_type must be SELF_CG (2), JACOBI (3) or MULTIGRID (4)");
+        return false;
+    }
+    if (dim == 2 && p->precond != EFS_PRECOND_NONE) {
+        efs_set_error("the CG preconditioners are 3D only");
+        return false;
+    }
+    return true;
+}
+
 int run_project(efs_sim* s, efs_stats* st) {
     int rc;
     // pressure.clear(): Jacobi starts from it; the CG solves write every cell of p when they end
     // (efs_cg_finalize), so for them the clear would be overwritten unread
-    const bool cg = s->P.solver_type != EFS_SOLVER_JACOBI && s->P.precond != EFS_PRECOND_LEARNED;
+    const bool cg = s->P.solver_type == EFS_SOLVER_MULTIGRID ||
+                    (s->P.solver_type != EFS_SOLVER_JACOBI && s->P.precond != EFS_PRECOND_LEARNED);
     if (!cg) EFS_HIP(hipMemsetAsync(s->p, 0, (size_t)s->N * 4, s->stream));
     if ((rc = efs_launch_walls_divergence(s))) return rc;
     efs_stats local{};
     efs_stats* out = st ? st : &local;
-    int status;
-    if (s->P.solver_type == EFS_SOLVER_JACOBI) status = efs_jacobi_solve(s, out);
-    else if (s->P.precond == EFS_PRECOND_LEARNED) status = efs_learned_cg_solve(s, out);
-    else if (s->P.precond == EFS_PRECOND_MG) status = efs_mg_cg_solve(s, out);
-    else status = efs_cg_solve(s, out);
+    const int status = solve_3d(s, out);
     if (status < 0) return status;
     if (status == 1) return 1;  // not converged: velocity keeps the pre-solve state
     if ((rc = efs_launch_gradient_walls(s))) return rc;
@@ -331,14 +366,7 @@ static int create(int dim, int32_t width, int32_t height, int32_t depth, const e
         efs_set_error("grid too large for 32-bit cell indexing of the 8-entry advection arrays");
         return -1;
     }
-    if (params && params->solver_type != EFS_SOLVER_SELF_CG && params->solver_type != EFS_SOLVER_JACOBI) {
-        efs_set_error("solver_type must be SELF_CG (2) or JACOBI (3)");
-        return -1;
-    }
-    if (params && dim == 2 && params->precond != EFS_PRECOND_NONE) {
-        efs_set_error("the CG preconditioners are 3D only");
-        return -1;
-    }
+    if (params && !check_params(dim, params)) return -1;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         efs_set_error("no HIP device available");
@@ -384,17 +412,21 @@ int efs_destroy(efs_handle h) {
 
 int efs_set_params(efs_handle h, const efs_params* p) {
     if (!valid_handle(h) || !p) return -1;
-    if (p->solver_type != EFS_SOLVER_SELF_CG && p->solver_type != EFS_SOLVER_JACOBI) {
-        efs_set_error("solver_type must be SELF_CG (2) or JACOBI (3)");
-        return -1;
-    }
-    if (h->dim == 2 && p->precond != EFS_PRECOND_NONE) {
-        efs_set_error("the CG preconditioners are 3D only");
-        return -1;
-    }
+    if (!check_params(h->dim, p)) return -1;
     h->P = *p;
     h->traces_valid = false;
     return 0;
+}
+int efs_available_solvers(int32_t dim, int32_t* solvers, int32_t capacity) {
+    if (dim != 2 && dim != 3) {
+        efs_set_error("dim must be 2 or 3");
+        return -1;
+    }
+    int32_t av[4];
+    const int n = available_solvers(dim, av);
+    if (solvers)
+        for (int k = 0; k < n && k < capacity; k++) solvers[k] = av[k];
+    return n;
 }
 int efs_get_params(efs_handle h, efs_params* p) {
     if (!valid_handle(h) || !p) return -1;
@@ -476,10 +508,7 @@ int efs_solve_pressure(efs_handle h, efs_stats* st) {
     EFS_HIP(hipMemsetAsync(h->p, 0, (size_t)h->N * 4, h->stream));
     int rc;
     if (h->dim == 2) rc = efs2_solve(h, out);
-    else if (h->P.solver_type == EFS_SOLVER_JACOBI) rc = efs_jacobi_solve(h, out);
-    else if (h->P.precond == EFS_PRECOND_LEARNED) rc = efs_learned_cg_solve(h, out);
-    else if (h->P.precond == EFS_PRECOND_MG) rc = efs_mg_cg_solve(h, out);
-    else rc = efs_cg_solve(h, out);
+    else rc = solve_3d(h, out);
     if (rc < 0) return rc;
     EFS_HIP(hipStreamSynchronize(h->stream));
     out->status = rc;

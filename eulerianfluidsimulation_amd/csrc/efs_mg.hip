@@ -730,21 +730,29 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
     pc.writes_rz = m->fused_rz;
     // The recurrence residual stops the iteration; with the true residual enabled and the relative
     // criterion, a solve whose FP64 true residual is still above the tolerance restarts from its
-    // iterate (at most twice).
+    // iterate (at most twice) while iterations remain under max_steps.  A restart never runs past
+    // the cap: once the cap is spent the solve keeps the status of the attempt that met the
+    // recurrence criterion, with its true residual in the stats.
     efs_stats local{};
     efs_stats* o = out ? out : &local;
-    int total = 0;
+    int total = 0, first_rc = 0;
     for (int attempt = 0; attempt < 3; attempt++) {
+        if (attempt > 0 && total >= s->P.max_steps) break;
         const size_t kt0 = s->ktimers[KT_OTHER].start.size(), kt1 = s->ktimers[KT_MG_DOWN].start.size();
         rc = efs_cg_run_pc(s, o, false, std::max(1, s->P.max_steps - total), &pc);
         // launches after convergence return at once: keep the timed fine up sweep of each real V-cycle
         efs_trim_timer(s->ktimers[KT_OTHER], kt0 + ((size_t)s->st_host->iter + 1));
         efs_trim_timer(s->ktimers[KT_MG_DOWN], kt1 + ((size_t)s->st_host->iter + 1));
+        if (rc < 0) return rc;
         total += o->iterations;
-        if (attempt == 0) m->last_its = o->iterations;
+        if (attempt == 0) {
+            m->last_its = o->iterations;
+            first_rc = rc;
+        }
         const bool verify = s->P.true_residual && s->P.criterion == EFS_CRIT_RELATIVE;
         if (rc != 0 || !verify || o->true_rel_residual <= (double)s->P.solver_threshold) break;
     }
     o->iterations = total;
+    if (rc == 1 && first_rc == 0) rc = 0;  // a restart stopped by the cap: the first attempt converged
     return rc;
 }

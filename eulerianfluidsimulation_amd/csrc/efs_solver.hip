@@ -728,7 +728,14 @@ int efs_launch_rowcode(efs_sim* s, int* border_rows) {
 void efs_ktimer_pair(efs_sim* s, hipEvent_t* a, hipEvent_t* b) {
     *a = *b = nullptr;
     if (!s->ktiming) return;
-    if (hipEventCreate(a) != hipSuccess || hipEventCreate(b) != hipSuccess) *a = *b = nullptr;
+    if (hipEventCreate(a) != hipSuccess) {
+        *a = nullptr;
+        return;
+    }
+    if (hipEventCreate(b) != hipSuccess) {
+        hipEventDestroy(*a);
+        *a = *b = nullptr;
+    }
 }
 void efs_ktimer_add(efs_sim* s, int k, hipEvent_t a, hipEvent_t b) {
     if (!a) return;

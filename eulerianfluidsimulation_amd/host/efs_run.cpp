@@ -13,11 +13,11 @@ int run(Sim& sim, int steps, const efs_prim& dam) {
     efs::check(efs_scene_rasterise(sim.handle(), &dam, 1), "efs_scene_rasterise");
     efs::check(efs_scene_face_noise(sim.handle(), 0, 1e-3), "efs_scene_face_noise");
     double mass0 = 0;
-    for (float r : sim.grid().rho()) mass0 += r;
+    for (float r : sim.grid().view(EFS_FIELD_RHO)) mass0 += r;
     for (int s = 0; s < steps; s++) {
         const int rc = sim.update();
         double mass = 0;
-        for (float r : sim.grid().rho()) mass += r;
+        for (float r : sim.grid().view(EFS_FIELD_RHO)) mass += r;
         const efs_stats& st = sim.lastStats();
         std::printf("step %d rc=%d cg_iters=%d rr=%.3e mass=%.6f (%.6f)\n", s, rc, st.iterations, st.rr, mass, mass0);
     }

@@ -50,8 +50,11 @@ enum efs_stage {
     EFS_STAGE_PROJECT = 6
 };
 
-/* Solver ids keep the reference enum values (Properties.h:475-481). */
-enum efs_solver { EFS_SOLVER_SELF_CG = 2, EFS_SOLVER_JACOBI = 3 };
+/* Solver ids keep the reference enum values (Properties.h:475-481).  MULTIGRID, which the reference
+ * lists (Properties.cpp:144) but no reference backend implements, runs CG preconditioned by one
+ * geometric multigrid V-cycle per iteration (3D only; the same solve as SELF_CG with
+ * precond = EFS_PRECOND_MG). */
+enum efs_solver { EFS_SOLVER_SELF_CG = 2, EFS_SOLVER_JACOBI = 3, EFS_SOLVER_MULTIGRID = 4 };
 
 /* CG stop criterion. */
 enum efs_criterion {
@@ -121,6 +124,12 @@ int efs_create_2d(int32_t width, int32_t height, const efs_params* params, efs_h
 int efs_destroy(efs_handle h);
 /* Reference: the PROPERTY_* reads each stage performs (Properties.h:16-32). */
 int efs_set_params(efs_handle h, const efs_params* params);
+/* Reference: ISimulator::availableSolvers (src/simulation/simulator/ISimulator.cpp:5-30, the list
+ * each backend's ctor passes, e.g. CPUSimulator3D.cpp:18-21): writes up to `capacity` solver ids
+ * for a 2D or 3D backend, first the default, and returns how many there are.  A solver_type outside
+ * the list is rejected by efs_create_* and efs_set_params with -1; the reference's fallback to the
+ * first entry is the caller's (the C++ adapter does it). */
+int efs_available_solvers(int32_t dim, int32_t* solvers, int32_t capacity);
 int efs_get_params(efs_handle h, efs_params* params);
 int efs_dims(efs_handle h, int32_t dims[3]);
 

@@ -1487,8 +1487,9 @@ int ora3_step(void* s, int dbl, ora_stats* st) {
     return ((Sim3D<float>*)s)->step(st);
 }
 int ora3_stage(void* s, int dbl, int stage, ora_stats* st) {
-    if (dbl) return ((Sim3D<double>*)s)->stage(stage, st);
-    return ((Sim3D<float>*)s)->stage(stage, st);
+    const int rc = dbl ? ((Sim3D<double>*)s)->stage(stage, st) : ((Sim3D<float>*)s)->stage(stage, st);
+    if (st) st->status = rc;
+    return rc;
 }
 // Pressure solve on the current divergence (pressure cleared first, as project() does).
 int ora3_solve(void* s, int dbl, ora_stats* st) {
@@ -1592,6 +1593,94 @@ void ora3_scatter(void* s, int dbl, const double* pos, double value) {
     else run((Sim3D<float>*)s, 0.0f);
 }
 
+// FP64 solution of A p = divergence for the error statements of the full-size configurations:
+// the same operator as CGSolverCPU3D::impl_mat_mul (CGSolver.cpp:467-497), conjugate gradients in
+// FP64 with OpenMP-parallel loops (reduction order differs from the sequential solve only in the
+// last bits of FP64), stopped at ||b - A p|| / ||b|| <= rel_tol (recomputed residual, checked every
+// 50 iterations and at the end).  The error of the result is set by that final residual, not by
+// the initial guess, so x0 (N doubles, or null for x0 = b) may be any guess.  Writes the
+// FP64 instance's pressure; returns the iterations and the final relative residual.
+int ora3_cg64(void* s, const double* x0, double rel_tol, int max_iter, int nthreads, double* out_rel) {
+    auto* S = (Sim3D<double>*)s;
+    const long long N = (long long)S->N;
+    const int W = S->W, H = S->H, D = S->D;
+    std::vector<int8_t> row((size_t)N);  // -1: identity row; else neighbour count
+    std::vector<uint8_t> nbm((size_t)N);  // bit k: neighbour k counted and fluid (enters the sum)
+    const long long off[6] = {-1, 1, -(long long)W, (long long)W, (long long)W * H, -(long long)W * H};
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long long c = 0; c < N; c++) {
+        const int x = (int)(c % W), y = (int)((c / W) % H), z = (int)(c / ((long long)W * H));
+        if (S->rho[c] <= 0.5 || S->boundary[c] == 1.0) { row[c] = -1; nbm[c] = 0; continue; }
+        const int nx[6] = {x - 1, x + 1, x, x, x, x}, ny[6] = {y, y, y - 1, y + 1, y, y}, nz[6] = {z, z, z, z, z + 1, z - 1};
+        int cnt = 0;
+        uint8_t m = 0;
+        for (int k = 0; k < 6; k++) {
+            if (nx[k] < 0 || nx[k] >= W || ny[k] < 0 || ny[k] >= H || nz[k] < 0 || nz[k] >= D) continue;
+            if (nx[k] == 0 || nx[k] == W - 1 || ny[k] == 0 || ny[k] == H - 1 || nz[k] == 0 || nz[k] == D - 1) continue;
+            const long long q = c + off[k];
+            if (S->boundary[q] > 0.5) continue;
+            cnt++;
+            if (S->rho[q] > 0.5) m |= (uint8_t)(1u << k);
+        }
+        row[c] = (int8_t)cnt;
+        nbm[c] = m;
+    }
+    auto matmul = [&](std::vector<double>& out, const std::vector<double>& v) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (long long c = 0; c < N; c++) {
+            if (row[c] < 0) { out[c] = v[c]; continue; }
+            double sum = 0.0;
+            for (int k = 0; k < 6; k++)
+                if (nbm[c] & (1u << k)) sum += v[c + off[k]];
+            out[c] = sum - (double)row[c] * v[c];
+        }
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+        double d = 0.0;
+#pragma omp parallel for num_threads(nthreads) reduction(+ : d) schedule(static)
+        for (long long c = 0; c < N; c++) d += a[c] * b[c];
+        return d;
+    };
+    const std::vector<double>& b = S->divergence;
+    std::vector<double>& x = S->pressure;
+    if (x0) x.assign(x0, x0 + N);
+    else x = b;
+    std::vector<double> r((size_t)N), d((size_t)N), t((size_t)N);
+    const double bb = dot(b, b);
+    auto residual = [&]() {
+        matmul(t, x);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (long long c = 0; c < N; c++) r[c] = b[c] - t[c];
+        return dot(r, r);
+    };
+    double rr = residual();
+    d = r;
+    int it = 0;
+    const double stop = rel_tol * rel_tol * bb;
+    while (it < max_iter && rr > stop) {
+        matmul(t, d);
+        const double alpha = rr / dot(d, t);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (long long c = 0; c < N; c++) {
+            x[c] += alpha * d[c];
+            r[c] -= alpha * t[c];
+        }
+        it++;
+        double rn = dot(r, r);
+        if (it % 50 == 0 || rn <= stop) {
+            rn = residual();  // recomputed: the recurrence residual drifts in long solves
+            if (rn <= stop) { rr = rn; break; }
+        }
+        const double beta = rn / rr;
+        rr = rn;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (long long c = 0; c < N; c++) d[c] = r[c] + beta * d[c];
+    }
+    rr = residual();
+    if (out_rel) *out_rel = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    return it;
+}
+
 // ---- 2D ----
 void* ora2_create(int w, int h, int dbl) {
     if (dbl) return new Sim2D<double>(w, h);
@@ -1629,8 +1718,9 @@ int ora2_step(void* s, int dbl, ora_stats* st) {
     return ((Sim2D<float>*)s)->step(st);
 }
 int ora2_stage(void* s, int dbl, int stage, ora_stats* st) {
-    if (dbl) return ((Sim2D<double>*)s)->stage(stage, st);
-    return ((Sim2D<float>*)s)->stage(stage, st);
+    const int rc = dbl ? ((Sim2D<double>*)s)->stage(stage, st) : ((Sim2D<float>*)s)->stage(stage, st);
+    if (st) st->status = rc;
+    return rc;
 }
 int ora2_solve(void* s, int dbl, ora_stats* st) {
     auto run = [&](auto* S) {

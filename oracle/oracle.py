@@ -64,6 +64,7 @@ def load():
         "ora2_set_params": (None, [vp, i32, P(OraParams)]), "ora2_field": (vp, [vp, i32, i32]),
         "ora2_step": (i32, [vp, i32, P(OraStats)]), "ora2_stage": (i32, [vp, i32, i32, P(OraStats)]),
         "ora2_solve": (i32, [vp, i32, P(OraStats)]), "ora2_matmul": (None, [vp, i32, vp, vp]),
+        "ora3_cg64": (i32, [vp, vp, f64, i32, i32, P(f64)]),
         "ora3_gradient": (None, [vp, i32]), "ora3_sharpen_delta": (None, [vp, i32]),
         "ora3_trace_field": (None, [vp, i32, P(f64), P(f64)]), "ora3_scatter": (None, [vp, i32, P(f64), f64]),
         "ora2_gradient": (None, [vp, i32]), "ora2_enforce_walls": (None, [vp, i32]),
@@ -185,6 +186,20 @@ class Oracle3D:
 
     def num_entries(self):
         return self.lib.ora3_num_entries(self.h, self.dbl)
+
+    def solve64(self, x0=None, rel_tol=1e-11, max_iter=20000, threads=None):
+        """FP64 CG on the current divergence (FP64 instance only), OpenMP-parallel, to
+        ||b - A p|| / ||b|| <= rel_tol; returns (iterations, final relative residual)."""
+        assert self.dbl, "solve64 runs on the FP64 instance"
+        if threads is None:
+            threads = max(1, min(16, os.cpu_count() or 1))
+        xp = None
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64)
+            xp = x0.ctypes.data
+        rel = ctypes.c_double()
+        it = self.lib.ora3_cg64(self.h, xp, rel_tol, max_iter, threads, ctypes.byref(rel))
+        return it, rel.value
 
     def gradient(self):
         """project()'s pressure-gradient subtraction alone, on the current pressure."""

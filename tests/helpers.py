@@ -16,6 +16,18 @@ def oracle_state(scene, nsteps, **over):
     return {k: o.field(k).copy() for k in FIELDS}
 
 
+def oracle_pre_project_state(scene, nsteps, **over):
+    """The state project() receives in step nsteps + 1: `nsteps` full oracle steps, then the six
+    stages before the projection (teacher forcing for the projection tests)."""
+    o = Oracle3D(*scene.dims, params=params_from_scene(scene, **over))
+    o.load_scene(scene)
+    for _ in range(nsteps):
+        o.step()
+    for s in range(6):
+        o.stage(s)
+    return {k: o.field(k).copy() for k in FIELDS}
+
+
 def to_oracle(state, scene, dbl=False, **over):
     o = Oracle3D(*scene.dims, params=params_from_scene(scene, **over), dbl=dbl)
     for k, v in state.items():

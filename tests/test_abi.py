@@ -61,3 +61,13 @@ def test_host_adapter_builds():
     from eulerianfluidsimulation_amd import build as b
     path = b.build_host()
     assert os.path.exists(path)
+
+
+def test_available_solvers_without_gpu():
+    """ISimulator::availableSolvers (ISimulator.cpp:5-30): SELF_CG first (the fallback), then
+    JACOBI, and in 3D the reference's MULTIGRID id (Properties.h:481)."""
+    lib = _abi.load()
+    buf = (ctypes.c_int32 * 4)()
+    assert lib.efs_available_solvers(3, buf, 4) == 3 and list(buf[:3]) == [2, 3, 4]
+    assert lib.efs_available_solvers(2, buf, 4) == 2 and list(buf[:2]) == [2, 3]
+    assert lib.efs_available_solvers(1, buf, 4) == -1

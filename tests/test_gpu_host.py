@@ -24,3 +24,15 @@ def test_cpp_driver_runs(hiplib, extra):
         m = re.search(r"rc=(\d+) cg_iters=(\d+) .* mass=([\d.]+) \(([\d.]+)\)", l)
         assert m and m.group(1) == "0"
         assert abs(float(m.group(3)) - float(m.group(4))) < 1e-3 * float(m.group(4))
+
+
+def test_cpp_adapter_mirror_protocol(hiplib):
+    """HipSimulator3D.h's host mirrors (CudaGrid3D.cu:137-170's lazy download, edits uploaded
+    before the next step): after steps each mirror equals efs_download; an accessor edit reaches
+    the device and keeps every other cell; markModified before the first read keeps the device
+    values; unedited fields are never uploaded; raw calls through handle() flush edits first."""
+    assert os.path.exists(build.MIRROR_BIN), "mirror self-check not built"
+    out = subprocess.run([build.MIRROR_BIN, "24", "3"], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0
+    assert "MIRROR_CHECK_OK" in out.stdout

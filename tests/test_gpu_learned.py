@@ -96,7 +96,8 @@ def _projection_case(sim, sc):
 
 
 def test_learned_solve_untrained_converges(torch_cuda):
-    """An untrained network still gives a converged pressure (flexible CG, fallback allowed)."""
+    """An untrained network stalls the flexible CG, which falls back to plain CG (the fallback flag
+    is set) and still reaches the FP64-verified tolerance."""
     sc = scenes.sphere_pool(32, seed=0)
     net = learned.UNet(seed=6)
     sim = HipSimulator3D(*sc.dims, params=gpu_params(sc, criterion=1, true_residual=1, max_steps=4000))
@@ -109,6 +110,7 @@ def test_learned_solve_untrained_converges(torch_cuda):
     print(f"[learned untrained] rc={rc} its={st.iterations} fallback={st.fallback} rel={st.rel_residual:.3e} "
           f"true={st.true_rel_residual:.3e}")
     assert rc == 0 and st.true_rel_residual < 1.5e-5
+    assert st.fallback == 1
     sim.close()
 
 
@@ -133,7 +135,11 @@ def test_learned_preconditioner_gain(torch_cuda):
             st = sim.last_stats
             assert rc == 0 and st.true_rel_residual < 1.5e-5, (mode, seed, rc, st.true_rel_residual)
             (plain_its if mode == "plain" else learned_its).append(st.iterations)
+            if mode == "learned":
+                assert st.fallback == 0 and st.warm_started == 1, (seed, st.fallback, st.warm_started)
             print(f"[learned gain] seed {seed} {mode}: its={st.iterations} fallback={st.fallback} true={st.true_rel_residual:.3e}")
             sim.close()
     print(f"[learned gain] plain {plain_its} learned {learned_its}")
-    assert np.mean(learned_its) < np.mean(plain_its)
+    # measured: 13-17 learned against 28-43 plain iterations (-62 %); a gain under 40 % fails
+    assert all(l < p for l, p in zip(learned_its, plain_its))
+    assert np.mean(learned_its) < 0.6 * np.mean(plain_its)

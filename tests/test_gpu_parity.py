@@ -12,9 +12,18 @@ import pytest
 
 from eulerianfluidsimulation_amd import HipSimulator3D, scenes
 from oracle.oracle import Oracle3D, params_from_scene
-from tests.helpers import gpu_params, maxdiff, nbits_equal, oracle_state, to_gpu, to_oracle
+from tests.helpers import gpu_params, maxdiff, nbits_equal, oracle_pre_project_state, oracle_state, to_gpu, to_oracle
 
 pytestmark = pytest.mark.gpu
+
+# Tolerances, about 10x the largest difference measured on the MI355X (in brackets).
+TOL_ADVECT = 1e-5        # rho_advect rho / gamma vs the key64 oracle        [7.2e-7]
+TOL_ADVECT_BETA = 3e-5   # rho_advect beta                                   [2.9e-6]
+TOL_SHARPEN_PAR = 3e-9   # rho_sharpen vs the same (apply-all) order         [2.3e-10]
+TOL_SHARPEN_SEQ = 3e-5   # rho_sharpen vs the reference's sequential order   [2.7e-6]
+TOL_FREE_RHO = 2e-4      # 5 free-running steps, max |rho_gpu - rho_ref|      [1.8e-5]
+TOL_CG_ERR = 1e-4        # pressure error vs FP64 of an r.r < 1e-5 solve      [1.04e-5]
+TOL_CG_VS_REF = 4e-6     # GPU (FP64 dots) vs reference (FP32 dots) pressure  [4.0e-7]
 
 SCENES = {
     "dam": (lambda: scenes.dam3d(22), 6),
@@ -97,31 +106,70 @@ def test_project_jacobi_bitexact(case):
     sim.close()
 
 
-def test_project_cg(case):
-    """CG: divergence bit-exact; pressure within the solver tolerance of the FP64 solution."""
-    name, sc, state = case
+PROJECT_CASES = {
+    "dam22": (lambda: scenes.dam3d(22), 6),
+    "pool_boxes22": (lambda: scenes.sphere_pool(22, seed=1), 5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PROJECT_CASES))
+def test_project_cg_teacher_forced(hiplib, name):
+    """The projection of a step, on the state the six earlier stages of that step left (teacher
+    forcing), with the reference criterion r.r < 1e-5: the divergence is bit-exact, the iteration
+    count matches the FP32 reference's (>= 50 iterations here), and both pressures are compared
+    with each other and with the FP64 solution of the same system."""
+    make, nsteps = PROJECT_CASES[name]
+    sc = make()
+    state = oracle_pre_project_state(sc, nsteps)
     sim, o = fresh(sc, state)
     rc = sim.project()
     st = sim.last_stats
     ost = o.stage(6)
     g = sim.grid()
     assert nbits_equal(g.divergence(), o.field("divergence"))
-    b = o.field("divergence").astype(np.float64)
     o64 = to_oracle({k: o.field(k) for k in ["vel", "rho", "gamma", "beta", "divergence", "pressure", "forces", "boundary"]},
-                    sc, dbl=True, solver_threshold=1e-24, max_steps=5000)
-    o64.field("divergence")[...] = b
-    o64.solve()
+                    sc, dbl=True)
+    o64.field("divergence")[...] = o.field("divergence").astype(np.float64)
+    it64, rel64 = o64.solve64(rel_tol=1e-12)
     p64 = o64.field("pressure").copy()
     pg = g.pressure().astype(np.float64)
     po = o.field("pressure").astype(np.float64)
-    eg = np.linalg.norm(pg - p64) / np.linalg.norm(p64)
-    eo = np.linalg.norm(po - p64) / np.linalg.norm(p64)
-    print(f"[{name}] cg: rc={rc} it_gpu={st.iterations} it_ref={ost.iterations} rr_gpu={st.rr:.3e} "
-          f"rr_ref={ost.err:.3e} |p-p64|/|p64| gpu={eg:.3e} ref={eo:.3e}")
-    assert rc == 0
+    n64 = np.linalg.norm(p64)
+    eg, eo = np.linalg.norm(pg - p64) / n64, np.linalg.norm(po - p64) / n64
+    dgo = np.linalg.norm(pg - po) / np.linalg.norm(po)
+    print(f"[{name}] cg: rc={rc} it_gpu={st.iterations} it_ref={ost.iterations} rr_gpu={st.rr:.3e} rr_ref={ost.err:.3e} "
+          f"|pg-p64|/|p64|={eg:.3e} |pref-p64|/|p64|={eo:.3e} |pg-pref|/|pref|={dgo:.3e} (p64: {it64} its, rel {rel64:.1e})",
+          flush=True)
+    assert rc == 0 and ost.status == 0
     assert st.rr < 1e-5
+    assert ost.iterations >= 50
     assert abs(st.iterations - ost.iterations) <= max(3, ost.iterations // 10)
-    assert eg <= max(3 * eo, 1e-5)
+    assert eg <= TOL_CG_ERR and eo <= TOL_CG_ERR
+    assert eg <= 1.05 * eo  # FP64 dots are at least as accurate at the same iteration count
+    assert dgo <= TOL_CG_VS_REF
+    sim.close()
+
+
+def test_project_cg_where_the_fp32_reference_stalls(hiplib):
+    """At 48^3 (dam break, step 11) the reference's FP32 CG does not reach r.r < 1e-5 within its
+    500 iterations (its sequential FP32 dot products stall near r.r = 2e-2), so the reference CPU
+    path would pause (CPUSimulator3D.cpp:541-547).  The library's CG, with FP64 dots and iterate,
+    meets the same criterion in about 140 iterations, closer to the FP64 solution."""
+    sc = scenes.dam3d(46)
+    state = oracle_pre_project_state(sc, 10)
+    sim, o = fresh(sc, state)
+    rc = sim.project()
+    st = sim.last_stats
+    ost = o.stage(6)
+    o64 = to_oracle({k: o.field(k) for k in ["rho", "boundary", "divergence"]}, sc, dbl=True)
+    it64, rel64 = o64.solve64(rel_tol=1e-12)
+    p64 = o64.field("pressure")
+    eg = float(np.linalg.norm(sim.grid().pressure() - p64) / np.linalg.norm(p64))
+    print(f"[dam46 stall] gpu rc={rc} its={st.iterations} rr={st.rr:.2e} |p-p64|/|p64|={eg:.2e}; reference rc={ost.status} "
+          f"its={ost.iterations} rr={ost.err:.2e}", flush=True)
+    assert ost.status == 1 and ost.err > 1e-5  # the reference pauses
+    assert rc == 0 and st.rr < 1e-5 and st.iterations < 300
+    assert eg < 1e-5
     sim.close()
 
 
@@ -136,8 +184,8 @@ def test_rho_advect(case):
     dr, dg, db = maxdiff(g.rho(), o.field("rho")), maxdiff(g.gamma(), o.field("gamma")), maxdiff(g.beta(), o.field("beta"))
     m0, m1 = float(state["rho"].astype(np.float64).sum()), float(g.rho().astype(np.float64).sum())
     print(f"[{name}] rho_advect max diff rho {dr:.3e} gamma {dg:.3e} beta {db:.3e}; mass {m0} -> {m1}")
-    assert dr < 1e-4 and dg < 1e-4 and db < 1e-4
-    assert abs(m1 - m0) <= 1e-4 * max(1.0, m0)
+    assert dr < TOL_ADVECT and dg < TOL_ADVECT and db < TOL_ADVECT_BETA
+    assert abs(m1 - m0) <= 1e-6 * max(1.0, m0)
     sim.close()
 
 
@@ -165,8 +213,8 @@ def test_rho_advect_scattered_flow(hiplib, dims):
     dr, dg, db = maxdiff(g.rho(), o.field("rho")), maxdiff(g.gamma(), o.field("gamma")), maxdiff(g.beta(), o.field("beta"))
     m0, m1 = float(state["rho"].astype(np.float64).sum()), float(g.rho().astype(np.float64).sum())
     print(f"[scatter {dims}] rho_advect max diff rho {dr:.3e} gamma {dg:.3e} beta {db:.3e}; mass {m0} -> {m1}", flush=True)
-    assert dr < 1e-4 and dg < 1e-4 and db < 1e-4
-    assert abs(m1 - m0) <= 1e-4 * max(1.0, m0)
+    assert dr < TOL_ADVECT and dg < TOL_ADVECT and db < TOL_ADVECT_BETA
+    assert abs(m1 - m0) <= 1e-6 * max(1.0, m0)
     sim.close()
 
 
@@ -183,30 +231,42 @@ def test_rho_sharpen(case):
     d_par, d_seq = maxdiff(r, o.field("rho")), maxdiff(r, o_seq.field("rho"))
     m0, m1 = float(state["rho"].astype(np.float64).sum()), float(r.astype(np.float64).sum())
     print(f"[{name}] sharpen max diff vs parallel-order oracle {d_par:.3e}, vs sequential reference {d_seq:.3e}; mass {m0} -> {m1}")
-    assert d_par < 1e-5
-    assert d_seq < 5e-2
-    assert abs(m1 - m0) <= 1e-4 * max(1.0, m0)
+    assert d_par < TOL_SHARPEN_PAR
+    assert d_seq < TOL_SHARPEN_SEQ
+    assert abs(m1 - m0) <= 1e-6 * max(1.0, m0)
     sim.close()
 
 
 def test_free_running_steps(case):
+    """Five free-running steps of the GPU and the FP32 reference from the same state.  The density
+    follows within TOL_FREE_RHO.  Almost every face velocity follows within 1e-4 of the velocity
+    scale (measured 2.4e-5 at the 99.9th percentile); a few faces differ more (measured up to
+    0.14 of 36), where a cell's density sits within rounding of a class threshold (0.5 for decay,
+    extrapolation and the divergence mask) and the two runs put it on different sides."""
     name, sc, state = case
     sim, o = fresh(sc, state)
     o.params.key64 = 1
     o.set_params(o.params)
+    worst_rho = 0.0
     for k in range(5):
         rc = sim.update()
         ost = o.step()
         g = sim.grid()
-        print(f"[{name}] step {k}: rc={rc} it={sim.last_stats.iterations}/{ost.iterations} "
-              f"rho {maxdiff(g.rho(), o.field('rho')):.3e} vel {maxdiff(g.vel(), o.field('vel')):.3e}")
-        assert rc == 0
+        dr = maxdiff(g.rho(), o.field("rho"))
+        worst_rho = max(worst_rho, dr)
+        vo = o.field("vel")
+        dv = np.abs(g.vel().astype(np.float64) - vo)
+        vscale = float(np.abs(vo).max())
+        print(f"[{name}] step {k}: rc={rc} it={sim.last_stats.iterations}/{ost.iterations} rho {dr:.3e} "
+              f"vel max {dv.max():.3e} p99.9 {np.quantile(dv, 0.999):.3e} (scale {vscale:.1f})", flush=True)
+        assert rc == 0 and ost.status == 0
     g = sim.grid()
-    vscale = float(np.abs(o.field("vel")).max())
-    assert maxdiff(g.rho(), o.field("rho")) < 5e-2
-    assert maxdiff(g.vel(), o.field("vel")) < 5e-2 * max(1.0, vscale)
+    assert worst_rho < TOL_FREE_RHO
+    # 99.9 % of the face velocities within 1e-4 of the scale; the worst within 1e-2 of it
+    assert np.quantile(dv, 0.999) <= 1e-4 * max(1.0, vscale)
+    assert dv.max() <= 1e-2 * max(1.0, vscale)
     m_gpu, m_ref = float(g.rho().astype(np.float64).sum()), float(o.field("rho").astype(np.float64).sum())
-    assert abs(m_gpu - m_ref) <= 1e-3 * max(1.0, m_ref)
+    assert abs(m_gpu - m_ref) <= 1e-5 * max(1.0, m_ref)
     sim.close()
 
 

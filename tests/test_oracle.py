@@ -146,6 +146,20 @@ def test_cg_fp64_matches_dense_solve():
     np.testing.assert_array_equal(np.delete(p, idx), np.delete(b.ravel(), idx))
 
 
+def test_cg64_parallel_matches_dense_solve():
+    """The OpenMP FP64 solve used for the full-size error statements: dense-solve solution, from
+    x0 = b and from an arbitrary guess; identity rows keep x0."""
+    o, idx, p_ref, b = _poisson_case(True, 1e-26)
+    it, rel = o.solve64(rel_tol=1e-13, threads=4)
+    p = o.field("pressure").ravel()
+    assert rel <= 1e-13 and it > 5
+    np.testing.assert_allclose(p[idx], p_ref, rtol=0, atol=1e-11)
+    np.testing.assert_array_equal(np.delete(p, idx), np.delete(b.ravel(), idx))
+    guess = np.random.default_rng(3).standard_normal(b.shape)
+    it2, rel2 = o.solve64(x0=guess, rel_tol=1e-13, threads=3)
+    np.testing.assert_allclose(o.field("pressure").ravel()[idx], p_ref, rtol=0, atol=1e-11)
+
+
 def test_cg_fp32_reference_criterion():
     o, idx, p_ref, b = _poisson_case(False, 1e-5)
     st = o.solve()
