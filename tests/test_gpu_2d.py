@@ -19,6 +19,15 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ["vel", "rho", "gamma", "beta", "divergence", "pressure", "forces", "boundary"]
 
+# Tolerances, about 10x the largest difference measured on the MI355X (in brackets).
+TOL2_ADVECT = 1e-5       # rho_advect rho / gamma / beta                      [6.0e-7]
+TOL2_SHARPEN_PAR = 1e-9  # rho_sharpen vs the same apply-all order            [0]
+TOL2_SHARPEN_SEQ = 6e-6  # rho_sharpen vs the in-place sequential order       [5.4e-7]
+TOL2_CG_ERR = 3e-4       # teacher-forced CG pressure vs FP64                  [2.8e-5]
+TOL2_CG_VS_REF = 7e-6    # GPU vs reference pressure                           [7.1e-7]
+TOL2_CG_VEL = 1.5e-5     # projected velocity vs the reference, of the scale   [1.4e-6]
+TOL2_C1_RHO = 5e-3       # config 1, 100 steps, max |rho_gpu - rho_ref|        [4.7e-4]
+
 SCENES = {
     "dam2d": (lambda: scenes.dam2d(40), 6),
     "pool2d": (lambda: scenes.pool2d(40, seed=1), 5),
@@ -44,6 +53,17 @@ def to_gpu2(sim, state):
     g = sim.grid()
     for k in ["boundary", "vel", "rho", "gamma", "beta", "divergence", "pressure", "forces"]:
         g.set(k, state[k])
+
+
+def oracle_pre_project2(sc, nsteps, **over):
+    """The state project() receives in step nsteps + 1 (teacher forcing)."""
+    o = Oracle2D(sc.dims[0], sc.dims[1], params=params_from_scene(sc, **over))
+    o.load_scene(sc)
+    for _ in range(nsteps):
+        o.step()
+    for s in range(6):
+        o.stage(s)
+    return {k: o.field(k).copy() for k in FIELDS}
 
 
 @pytest.fixture(scope="module", params=sorted(SCENES))
@@ -121,10 +141,12 @@ def test_project_jacobi_bitexact_2d(case2):
 
 @pytest.mark.parametrize("recalc", [0, 1])
 def test_project_cg_2d(case2, recalc):
-    """CG 2D with the reference criterion (r.r < 1e-5): divergence bit-exact before the solve,
-    iteration count and velocity close to the reference's; then the relative criterion at 1e-7
-    resolves the pressure to the FP64 oracle solution."""
-    name, sc, state = case2
+    """CG 2D with the reference criterion (r.r < 1e-5) on the state the six earlier stages of a
+    step left (teacher forcing, so the solve has work to do): divergence bit-exact before the
+    solve, the same iteration count as the FP32 reference, pressures close to each other and to
+    the FP64 solution; then the relative criterion at 1e-7 resolves the FP64 solution."""
+    name, sc, _ = case2
+    state = oracle_pre_project2(sc, SCENES[name][1])
     sim, o = fresh2(sc, state, re_calc_div=recalc)
     rc = sim.project()
     st = sim.last_stats
@@ -142,14 +164,19 @@ def test_project_cg_2d(case2, recalc):
     po = o.field("pressure").astype(np.float64)
     eg = np.linalg.norm(pg - p64) / np.linalg.norm(p64)
     eo = np.linalg.norm(po - p64) / np.linalg.norm(p64)
-    print(f"[{name}] cg 2D recalc={recalc}: rc={rc} it_gpu={st.iterations} it_ref={ost.iterations} rr_gpu={st.rr:.3e} "
-          f"rr_ref={ost.err:.3e} |p-p64|/|p64| gpu={eg:.3e} ref={eo:.3e} vel {maxdiff(g.vel(), o.field('vel')):.3e}")
-    assert rc == 0
-    assert st.rr < 1e-5
-    assert abs(st.iterations - ost.iterations) <= max(3, ost.iterations // 10)
-    assert eg <= max(3 * eo, 1e-5)
+    dgo = np.linalg.norm(pg - po) / np.linalg.norm(po)
+    dv = maxdiff(g.vel(), o.field("vel"))
     vscale = max(1.0, float(np.abs(o.field("vel")).max()))
-    assert maxdiff(g.vel(), o.field("vel")) <= 1e-3 * vscale
+    print(f"[{name}] cg 2D recalc={recalc}: rc={rc} it_gpu={st.iterations} it_ref={ost.iterations} rr_gpu={st.rr:.3e} "
+          f"rr_ref={ost.err:.3e} |p-p64|/|p64| gpu={eg:.3e} ref={eo:.3e} |pg-pref|/|pref|={dgo:.3e} vel {dv:.3e} "
+          f"(scale {vscale:.1f})", flush=True)
+    assert rc == 0 and ost.status == 0
+    assert st.rr < 1e-5
+    assert ost.iterations >= 20
+    assert abs(st.iterations - ost.iterations) <= max(3, ost.iterations // 10)
+    assert eg <= TOL2_CG_ERR and eg <= 1.05 * eo
+    assert dgo <= TOL2_CG_VS_REF
+    assert dv <= TOL2_CG_VEL * vscale
     if not recalc:
         assert nbits_equal(g.divergence(), o.field("divergence"))
     else:
@@ -177,8 +204,8 @@ def test_rho_advect_2d(case2):
     dr, dg, db = maxdiff(g.rho(), o.field("rho")), maxdiff(g.gamma(), o.field("gamma")), maxdiff(g.beta(), o.field("beta"))
     m0, m1 = float(state["rho"].astype(np.float64).sum()), float(g.rho().astype(np.float64).sum())
     print(f"[{name}] rho_advect 2D max diff rho {dr:.3e} gamma {dg:.3e} beta {db:.3e}; mass {m0} -> {m1}")
-    assert dr < 1e-4 and dg < 1e-4 and db < 1e-4
-    assert abs(m1 - m0) <= 1e-4 * max(1.0, m0)
+    assert dr < TOL2_ADVECT and dg < TOL2_ADVECT and db < TOL2_ADVECT
+    assert abs(m1 - m0) <= 1e-6 * max(1.0, m0)
     sim.close()
 
 
@@ -195,9 +222,9 @@ def test_rho_sharpen_2d(case2):
     d_par, d_seq = maxdiff(r, o.field("rho")), maxdiff(r, o_seq.field("rho"))
     m0, m1 = float(state["rho"].astype(np.float64).sum()), float(r.astype(np.float64).sum())
     print(f"[{name}] sharpen 2D vs parallel-order oracle {d_par:.3e}, vs sequential {d_seq:.3e}; mass {m0} -> {m1}")
-    assert d_par < 1e-5
-    assert d_seq < 5e-2
-    assert abs(m1 - m0) <= 1e-4 * max(1.0, m0)
+    assert d_par < TOL2_SHARPEN_PAR
+    assert d_seq < TOL2_SHARPEN_SEQ
+    assert abs(m1 - m0) <= 1e-6 * max(1.0, m0)
     sim.close()
 
 
@@ -228,8 +255,9 @@ def test_dam_break_128_free_running(hiplib, solver):
     m_gpu, m_ref = float(g.rho().astype(np.float64).sum()), float(o.field("rho").astype(np.float64).sum())
     print(f"[dam2d {solver}] mass {m0:.4f} -> gpu {m_gpu:.4f} ref {m_ref:.4f}; vel scale {vscale:.2f}")
     assert nbits_equal(g.boundary(), o.field("boundary"))
-    assert maxdiff(g.rho(), o.field("rho")) < 0.1
-    assert abs(m_gpu - m_ref) <= 2e-3 * max(1.0, m_ref)
+    assert worst < TOL2_C1_RHO
+    assert abs(m_gpu - m_ref) <= 1e-6 * max(1.0, m_ref)
+    assert abs(m_gpu - m0) <= 1e-6 * max(1.0, m0)
     sim.close()
 
 
