@@ -121,6 +121,8 @@ SIGNATURES = {
                                               ctypes.POINTER(ctypes.c_int32)]),
     "efs_query_box": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
                                      ctypes.POINTER(QueryResult)]),
+    "efs_query_box_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+                                     ctypes.POINTER(QueryResult)]),
     "efs_unet_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     "efs_unet_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "efs_unet_num_params": (ctypes.c_int64, [ctypes.c_void_p]),

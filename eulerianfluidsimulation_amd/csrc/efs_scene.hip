@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "efs_internal.h"
@@ -165,6 +166,84 @@ __global__ __launch_bounds__(QY_NT) void k_query_box(DevGrid g, int3 lo, int3 hi
     }
 }
 
+// Sums the block partials in block order into *out, the same additions in the same order as the
+// host loop of efs_query_box (one thread: at most QY_MAXB partials).
+__global__ void k_query_fold(const QueryPart* __restrict__ parts, int nb, QueryPart* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    QueryPart f;
+    for (int k = 0; k < 7; k++) f.sum[k] = 0.0;
+    long long cnt = 0;
+    for (int k = 0; k < 3; k++) { f.mn[k] = INT_MAX; f.mx[k] = INT_MIN; }
+    for (int q = 0; q < nb; q++) {
+        for (int k = 0; k < 7; k++) f.sum[k] += parts[q].sum[k];
+        cnt += parts[q].cnt;
+        for (int k = 0; k < 3; k++) { f.mn[k] = min(f.mn[k], parts[q].mn[k]); f.mx[k] = max(f.mx[k], parts[q].mx[k]); }
+    }
+    f.cnt = (int)cnt;
+    f.pad = 0;
+    *out = f;
+}
+
+}  // namespace
+
+// Scratch of efs_query_box_noalloc: QY_MAXB block partials and the folded result, one copy per
+// GPU, held until the process ends.  Host code never passes this name to a kernel: it is only a
+// host-side handle of the symbol, resolved per device by hipGetSymbolAddress below.
+static __device__ __align__(256) QueryPart scratch[QY_MAXB + 1];
+
+namespace {
+
+constexpr int QY_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+QueryPart* g_scratch_ptr[QY_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+int query_scratch(QueryPart** out) {
+    int dev = 0;
+    EFS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= QY_MAX_DEVICES) {
+        efs_set_error("efs_query_box_noalloc: device index out of range");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return efs_hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = (QueryPart*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// Clips the box to [max(0, lo), min(n - 1, hi)) per axis as the reference's loops do; returns the
+// cell count.
+long long query_clip(const efs_sim* h, const int32_t lo_in[3], const int32_t hi_in[3], int lo[3], int hi[3]) {
+    const int dims[3] = {h->W, h->H, h->D};
+    for (int k = 0; k < 3; k++) {
+        lo[k] = std::max(0, lo_in[k]);
+        hi[k] = std::max(lo[k], std::min(dims[k] - 1, hi_in[k]));
+    }
+    if (h->dim == 2) {  // the 2D grid is one plane
+        lo[2] = 0;
+        hi[2] = 1;
+    }
+    return (long long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+}
+
+// Means from the totals (QueryNode.cpp:82-84: with no cell every value stays 0).
+void query_finish(const double sum[7], long long cnt, const int mn[3], const int mx[3], efs_query_result* out) {
+    out->count = (int32_t)cnt;
+    if (cnt == 0) return;
+    for (int k = 0; k < 3; k++) {
+        out->max_pos[k] = (float)mx[k];
+        out->min_pos[k] = (float)mn[k];
+        out->avg_pos[k] = (float)(sum[k] / (double)cnt);
+        out->vel[k] = (float)(sum[3 + k] / (double)cnt);
+    }
+    out->pressure = (float)(sum[6] / (double)cnt);
+}
+
 }  // namespace
 
 namespace {
@@ -261,18 +340,9 @@ extern "C" int efs_query_box(efs_handle h, const int32_t lo_in[3], const int32_t
         efs_set_error("efs_query_box: invalid argument");
         return -1;
     }
-    const int dims[3] = {h->W, h->H, h->D};
     int lo[3], hi[3];
-    for (int k = 0; k < 3; k++) {  // [max(0, lo), min(n - 1, hi)) as the reference's loops
-        lo[k] = std::max(0, lo_in[k]);
-        hi[k] = std::max(lo[k], std::min(dims[k] - 1, hi_in[k]));
-    }
-    if (h->dim == 2) {  // the 2D grid is one plane
-        lo[2] = 0;
-        hi[2] = 1;
-    }
+    const long long n = query_clip(h, lo_in, hi_in, lo, hi);
     std::memset(out, 0, sizeof(*out));
-    const long long n = (long long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
     if (n == 0) return 0;
     const int nb = (int)std::min<long long>(QY_MAXB, (n + QY_NT - 1) / QY_NT);
     QueryPart* dparts = nullptr;
@@ -293,14 +363,45 @@ extern "C" int efs_query_box(efs_handle h, const int32_t lo_in[3], const int32_t
         cnt += q.cnt;
         for (int k = 0; k < 3; k++) { mn[k] = std::min(mn[k], q.mn[k]); mx[k] = std::max(mx[k], q.mx[k]); }
     }
-    out->count = (int32_t)cnt;
-    if (cnt == 0) return 0;  // positions stay 0 (QueryNode.cpp:82-84)
-    for (int k = 0; k < 3; k++) {
-        out->max_pos[k] = (float)mx[k];
-        out->min_pos[k] = (float)mn[k];
-        out->avg_pos[k] = (float)(sum[k] / (double)cnt);
-        out->vel[k] = (float)(sum[3 + k] / (double)cnt);
+    query_finish(sum, cnt, mn, mx, out);
+    return 0;
+}
+
+// efs_query_box without allocation: the block partials and their fold live in the library's
+// static device scratch (QY_MAXB + 1 partials, 45 KB per GPU).  The grid-stride kernel needs at
+// most QY_MAXB blocks for any box, so only boxes whose cell count does not fit the int32 count are
+// refused.  Calls that overlap on one GPU share the scratch and race.
+extern "C" int efs_query_box_noalloc(efs_handle h, const int32_t lo_in[3], const int32_t hi_in[3], int32_t only_fluid,
+                                     efs_query_result* out) {
+    if (!h || !lo_in || !hi_in || !out) {
+        efs_set_error("efs_query_box_noalloc: invalid argument");
+        return -1;
     }
-    out->pressure = (float)(sum[6] / (double)cnt);
+    int lo[3], hi[3];
+    const long long n = query_clip(h, lo_in, hi_in, lo, hi);
+    std::memset(out, 0, sizeof(*out));
+    if (n > (long long)INT_MAX) {
+        efs_set_error("efs_query_box_noalloc: the box holds more than 2^31 - 1 cells");
+        return -1;
+    }
+    if (n == 0) return 0;
+    const int nb = (int)std::min<long long>(QY_MAXB, (n + QY_NT - 1) / QY_NT);
+    if (nb > QY_MAXB) {  // cannot happen with the clamp above; the scratch holds QY_MAXB partials
+        efs_set_error("efs_query_box_noalloc: more block partials than the scratch holds");
+        return -1;
+    }
+    QueryPart* ptr = nullptr;
+    if (int rc = query_scratch(&ptr)) return rc;
+    // every launched block writes its own partial and the fold overwrites the last slot, so
+    // nothing left by an earlier call is read
+    hipLaunchKernelGGL(k_query_box, dim3(nb), dim3(QY_NT), 0, h->stream, h->grid(), make_int3(lo[0], lo[1], lo[2]),
+                       make_int3(hi[0], hi[1], hi[2]), only_fluid, ptr);
+    hipLaunchKernelGGL(k_query_fold, dim3(1), dim3(1), 0, h->stream, ptr, nb, ptr + QY_MAXB);
+    QueryPart fin;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&fin, ptr + QY_MAXB, sizeof(QueryPart), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return efs_hip_fail(e, "efs_query_box_noalloc");
+    query_finish(fin.sum, (long long)fin.cnt, fin.mn, fin.mx, out);
     return 0;
 }

@@ -212,6 +212,13 @@ typedef struct {
     float pressure;
 } efs_query_result;
 int efs_query_box(efs_handle h, const int32_t lo[3], const int32_t hi[3], int32_t only_fluid, efs_query_result* out);
+/* No counterpart in the reference.  efs_query_box for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * statistics, bit-identical, with the block partials in the library's static device scratch (512
+ * partials and their fold, 45 KB per GPU, held until the process ends) instead of a per-call
+ * allocation.  Any box fits the scratch; a box of more than 2^31 - 1 cells is refused (-1).  Calls
+ * that overlap on one GPU share the scratch and race: serialise them. */
+int efs_query_box_noalloc(efs_handle h, const int32_t lo[3], const int32_t hi[3], int32_t only_fluid, efs_query_result* out);
 
 /* ---- Learned preconditioner (no counterpart in the reference: it accelerates the solve that
  * CGSolverCPU3D::solve performs, CGSolver.cpp:416-465) ------------------------------------
