@@ -15,7 +15,7 @@ SOLVER_SELF_CG, SOLVER_JACOBI, SOLVER_MULTIGRID = 2, 3, 4
 CRIT_REFERENCE, CRIT_RELATIVE = 0, 1
 PRECOND_NONE, PRECOND_LEARNED, PRECOND_MG = 0, 1, 2
 GUESS_DIVERGENCE, GUESS_PREVIOUS = 0, 1
-KT_CG_DIR_SPMV, KT_CG_UPDATE, KT_JACOBI, KT_OTHER, KT_MG_DOWN = range(5)
+KT_CG_DIR_SPMV, KT_CG_UPDATE, KT_JACOBI, KT_OTHER, KT_MG_DOWN, KT_ADV_BWD, KT_ADV_FWD, KT_WALLS_DIV, KT_GRADIENT = range(9)
 
 
 class Params(ctypes.Structure):

@@ -566,7 +566,7 @@ int efs_set_kernel_timing(efs_handle h, int32_t on) {
     return 0;
 }
 int efs_kernel_time(efs_handle h, int32_t k, int64_t* launches, double* total_ms) {
-    if (!valid_handle(h) || k < 0 || k > 4) return -1;
+    if (!valid_handle(h) || k < 0 || k >= KT_COUNT) return -1;
     EFS_HIP(hipStreamSynchronize(h->stream));
     TimerSlot& t = h->ktimers[k];
     for (size_t i = 0; i < t.start.size(); i++) {

@@ -308,7 +308,7 @@ struct efs_sim {
     // timing
     bool timing = false;
     bool ktiming = false;
-    TimerSlot ktimers[5];
+    TimerSlot ktimers[9];
     hipEvent_t ev_stage[8] = {};
     // learned preconditioner state (owned by the preconditioner module)
     void* learned = nullptr;
@@ -388,8 +388,12 @@ void efs_mg_free(efs_sim* s);
 int efs_learned_cg_solve(efs_sim* s, efs_stats* st);
 void efs_learned_free(efs_sim* s);
 
-// event-timed kernels: KT_OTHER times the fine-level multigrid up sweep, KT_MG_DOWN its down sweep
-enum { KT_CG_DIR_SPMV = 0, KT_CG_UPDATE = 1, KT_JACOBI = 2, KT_OTHER = 3, KT_MG_DOWN = 4 };
+// event-timed kernels: KT_OTHER times the fine-level multigrid up sweep, KT_MG_DOWN its down sweep;
+// the advection traces, the walls + divergence pass and the gradient pass of the step follow
+enum {
+    KT_CG_DIR_SPMV = 0, KT_CG_UPDATE = 1, KT_JACOBI = 2, KT_OTHER = 3, KT_MG_DOWN = 4,
+    KT_ADV_BWD = 5, KT_ADV_FWD = 6, KT_WALLS_DIV = 7, KT_GRADIENT = 8, KT_COUNT = 9
+};
 void efs_ktimer_pair(efs_sim* s, hipEvent_t* a, hipEvent_t* b);  // for hipExtLaunchKernelGGL's start/stop events
 void efs_ktimer_add(efs_sim* s, int k, hipEvent_t a, hipEvent_t b);
 void efs_trim_timer(TimerSlot& t, size_t keep);  // drops the events of launches past `keep`

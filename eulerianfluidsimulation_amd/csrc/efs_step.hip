@@ -13,6 +13,7 @@
 // independent across lines, so one thread per line reproduces them exactly (lines along x are
 // transposed through LDS so that the loads stay coalesced).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 
@@ -986,9 +987,15 @@ int efs_launch_rho_advect(efs_sim* s) {
     const dim3 g1(blocks1d(s->N)), b1(256);
     const float dt = s->P.dt;
     // steps 1-4
-    hipLaunchKernelGGL(k_adv_bwd, ai, ablk, 0, s->stream, g, dt, s->s1, s->bwd_p0, s->bwd_w, s->acc0, s->tu, s->tv, s->tw);
+    hipEvent_t ta, tb;
+    efs_ktimer_pair(s, &ta, &tb);
+    hipExtLaunchKernelGGL(k_adv_bwd, ai, ablk, 0, s->stream, ta, tb, 0, g, dt, s->s1, s->bwd_p0, s->bwd_w, s->acc0, s->tu, s->tv,
+                          s->tw);
+    efs_ktimer_add(s, KT_ADV_BWD, ta, tb);
     // step 5 (+ forward half of step 6)
-    hipLaunchKernelGGL(k_adv_fwd, ai, ablk, 0, s->stream, g, dt, s->acc0, s->fwd_p0, s->fwd_w, s->acc1);
+    efs_ktimer_pair(s, &ta, &tb);
+    hipExtLaunchKernelGGL(k_adv_fwd, ai, ablk, 0, s->stream, ta, tb, 0, g, dt, s->acc0, s->fwd_p0, s->fwd_w, s->acc1);
+    efs_ktimer_add(s, KT_ADV_FWD, ta, tb);
     hipLaunchKernelGGL(k_fill_u64, g1, b1, 0, s->stream, s->N, s->acc0);
     // steps 6-8: gamma6 -> s1 (the advected gamma of step 1 is no longer needed), new beta
     hipLaunchKernelGGL(k_adv_rows, af, ablk, 0, s->stream, g, s->bwd_p0, s->bwd_w, s->acc1, s->s1, s->acc0);
@@ -1034,15 +1041,21 @@ int efs_launch_add_force(efs_sim* s) {
 }
 
 int efs_launch_walls_divergence(efs_sim* s) {
-    hipLaunchKernelGGL(k_walls_div, full_grid(s), dim3(BX, BY), 0, s->stream, s->grid(), s->P.liquid_threshold,
-                       s->P.lambda_plus, s->P.lambda_minus, s->P.eta);
+    hipEvent_t ta, tb;
+    efs_ktimer_pair(s, &ta, &tb);
+    hipExtLaunchKernelGGL(k_walls_div, full_grid(s), dim3(BX, BY), 0, s->stream, ta, tb, 0, s->grid(), s->P.liquid_threshold,
+                          s->P.lambda_plus, s->P.lambda_minus, s->P.eta);
+    efs_ktimer_add(s, KT_WALLS_DIV, ta, tb);
     EFS_HIP(hipGetLastError());
     return 0;
 }
 
 int efs_launch_gradient_walls(efs_sim* s) {
-    hipLaunchKernelGGL(k_gradient_walls, full_grid_xcd(s), dim3(BX, BY), 0, s->stream, s->grid(),
-                       s->p64_valid ? (const double*)s->cg_x64 : nullptr);
+    hipEvent_t ta, tb;
+    efs_ktimer_pair(s, &ta, &tb);
+    hipExtLaunchKernelGGL(k_gradient_walls, full_grid_xcd(s), dim3(BX, BY), 0, s->stream, ta, tb, 0, s->grid(),
+                          s->p64_valid ? (const double*)s->cg_x64 : nullptr);
+    efs_ktimer_add(s, KT_GRADIENT, ta, tb);
     EFS_HIP(hipGetLastError());
     return 0;
 }

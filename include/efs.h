@@ -161,10 +161,11 @@ void* efs_get_stream(efs_handle h);
 int efs_synchronize(efs_handle h);
 /* 1 = record device time per stage into efs_stats.stage_ms (adds event pairs per stage). */
 int efs_set_timing(efs_handle h, int32_t on);
-/* Per-kernel event timing of the dominant solver kernels: 1 = bracket each launch of the tracked
- * kernels with events; efs_kernel_time reads the accumulated count and milliseconds of kernel
- * 0 (CG direction + SpMV), 1 (CG update), 2 (Jacobi), 3 (multigrid fine up sweep) or 4
- * (multigrid fine down sweep with restriction). */
+/* Per-kernel event timing of the dominant kernels: 1 = record start and end events on each launch
+ * of the tracked kernels; efs_kernel_time reads the accumulated count and milliseconds of kernel
+ * 0 (CG direction + SpMV), 1 (CG update), 2 (Jacobi), 3 (multigrid fine up sweep), 4 (multigrid
+ * fine down sweep with restriction), 5 (rho_advect backward trace), 6 (rho_advect forward trace),
+ * 7 (project's wall enforcement + divergence) or 8 (project's gradient + wall enforcement). */
 int efs_set_kernel_timing(efs_handle h, int32_t on);
 int efs_kernel_time(efs_handle h, int32_t kernel, int64_t* launches, double* total_ms);
 
