@@ -877,19 +877,27 @@ __device__ inline float wall_face(float f, bool self_solid, bool lower_solid) {
 
 // enforce_wall_boundaries + divergence with density correction; divergence cleared elsewhere.
 // The divergence of cell c reads the +x/+y/+z faces, enforced on the fly (enforcement is
-// idempotent, so reading a face another thread has already enforced gives the same value).
+// idempotent, so reading a face another thread has already enforced gives the same value).  A
+// face is stored only where the enforcement changed it (a face pointing into a solid), which
+// few are: storing every face wrote 16 of the kernel's 18 bytes per cell.  The blocks walk the
+// grid in XCD order (xcd_block), so the +y and +z neighbour rows are read through the L2 that
+// read them as their own block's rows.
 __global__ __launch_bounds__(256) void k_walls_div(DevGrid g, float liq, float lp, float lm, float eta) {
-    FULL_XYZ(g);
+    FULL_XYZ_XCD(g);
     const int sy = g.W, sz = g.W * g.H;
     if (x == 0 || y == 0 || z == 0) {
         g.div[c] = 0.f;
         return;
     }
     const bool s0 = g.bf[c] & BF_EQ1;
-    const float u0 = wall_face(g.u[c], s0, g.bf[c - 1] & BF_EQ1);
-    const float v0 = wall_face(g.v[c], s0, g.bf[c - sy] & BF_EQ1);
-    const float w0 = wall_face(g.w[c], s0, g.bf[c - sz] & BF_EQ1);
-    g.u[c] = u0; g.v[c] = v0; g.w[c] = w0;
+    const float uc = g.u[c], vc = g.v[c], wc = g.w[c];
+    const float u0 = wall_face(uc, s0, g.bf[c - 1] & BF_EQ1);
+    const float v0 = wall_face(vc, s0, g.bf[c - sy] & BF_EQ1);
+    const float w0 = wall_face(wc, s0, g.bf[c - sz] & BF_EQ1);
+    // (a NaN face compares unequal to itself and is stored back unchanged)
+    if (u0 != uc) g.u[c] = u0;
+    if (v0 != vc) g.v[c] = v0;
+    if (w0 != wc) g.w[c] = w0;
     float dv = 0.f;
     const bool interior = x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
     const float r = g.rho[c];
@@ -1043,7 +1051,7 @@ int efs_launch_add_force(efs_sim* s) {
 int efs_launch_walls_divergence(efs_sim* s) {
     hipEvent_t ta, tb;
     efs_ktimer_pair(s, &ta, &tb);
-    hipExtLaunchKernelGGL(k_walls_div, full_grid(s), dim3(BX, BY), 0, s->stream, ta, tb, 0, s->grid(), s->P.liquid_threshold,
+    hipExtLaunchKernelGGL(k_walls_div, full_grid_xcd(s), dim3(BX, BY), 0, s->stream, ta, tb, 0, s->grid(), s->P.liquid_threshold,
                           s->P.lambda_plus, s->P.lambda_minus, s->P.eta);
     efs_ktimer_add(s, KT_WALLS_DIV, ta, tb);
     EFS_HIP(hipGetLastError());
