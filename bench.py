@@ -89,6 +89,16 @@ def parse():
                     help="SELF_CG (with --precond) or JACOBI (max_steps / 2 passes, the reference's Jacobi solve)")
     ap.add_argument("--plain-steps", type=int, default=None,
                     help="timed steps of each plain-CG comparison run (default: --steps; 0 skips them)")
+    ap.add_argument("--learned-weights", default=None, metavar="NPZ",
+                    help="with --precond learned: the trained network (eulerianfluidsimulation_amd.learned --out)")
+    ap.add_argument("--learned-every", type=int, default=1, help="with --precond learned: apply the network every k-th "
+                    "flexible CG iteration")
+    ap.add_argument("--fp64-check", action="store_true",
+                    help="after the timed steps, one more step whose projection is compared with the FP64 oracle "
+                         "(divergence, and pressure against an FP64 CG solve to relative residual 1e-11)")
+    ap.add_argument("--unet", type=int, default=0, metavar="N",
+                    help="instead of the step: time the learned preconditioner's U-Net (widths 16-64) at N^3 "
+                         "(inference forward and one training step) against the BF16 MFMA peak")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=80, help="interior resolution of the CPU sample's stages and solve")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -188,6 +198,9 @@ def run(sc, params, warmup, steps, timed_params=None, kernel_timing=False, keep_
     import numpy as np
     from eulerianfluidsimulation_amd import HipSimulator3D
     sim = HipSimulator3D(*sc.dims, params=params)
+    if _NET is not None and params.precond == _abi_precond_learned():
+        from eulerianfluidsimulation_amd import learned
+        learned.attach(sim, _NET[0], learned.default_opts(_NET[1], every=ARGS.learned_every))
     t0 = time.perf_counter()
     sim.load_scene(sc)
     print(f"[bench{tag}] scene {sc.name} dims {sc.dims} built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
@@ -200,12 +213,15 @@ def run(sc, params, warmup, steps, timed_params=None, kernel_timing=False, keep_
     if kernel_timing:
         lib.efs_set_kernel_timing(sim.h, 1)
     sim.synchronize()
-    iters, true_res, rcs = [], [], []
+    iters, true_res, rcs, fallbacks = [], [], [], []
     t0 = time.perf_counter()
     for i in range(steps):
         rcs.append(sim.update())
         iters.append(sim.last_stats.iterations)
         true_res.append(sim.last_stats.true_rel_residual)
+        fallbacks.append(sim.last_stats.fallback)
+        if steps >= 100 and i % 50 == 49:
+            print(f"[bench{tag}] step {i + 1}: its {iters[-1]} ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     print(f"[bench{tag}] {steps} steps in {elapsed:.3f}s, its/step {np.mean(iters):.2f}", file=sys.stderr, flush=True)
@@ -222,8 +238,44 @@ def run(sc, params, warmup, steps, timed_params=None, kernel_timing=False, keep_
     if keep_state:
         g = sim.grid()
         state = {k: getattr(g, k)() for k in ("rho", "boundary", "divergence")}
+    fp64 = fp64_check(sc, sim) if (ARGS.fp64_check and not tag) else None
     sim.close()
-    return {"elapsed": elapsed, "iters": iters, "true_res": true_res, "rcs": rcs, "kt": kt, "state": state}
+    return {"elapsed": elapsed, "iters": iters, "true_res": true_res, "rcs": rcs, "kt": kt, "state": state,
+            "fallbacks": fallbacks, "fp64": fp64}
+
+
+def fp64_check(sc, sim):
+    """One more step: the six stages, then the projection with the run's solver; its divergence and
+    pressure against the FP64 oracle on the same pre-projection state."""
+    import numpy as np
+    from oracle.oracle import Oracle3D, params_from_scene
+    for s in range(6):
+        sim._stage(s)
+    g = sim.grid()
+    state = {k: getattr(g, k)() for k in ("vel", "rho", "boundary", "gamma", "beta", "forces")}
+    rc = sim.project()
+    div_g, p_g = g.divergence().astype(np.float64), g.pressure().astype(np.float64)
+    o = Oracle3D(*sc.dims, params=params_from_scene(sc), dbl=True)
+    for k, v in state.items():
+        o.field(k)[...] = v
+    o.divergence()
+    div64 = o.field("divergence")
+    e_div = float(np.linalg.norm(div_g - div64) / max(1e-300, np.linalg.norm(div64)))
+    t0 = time.perf_counter()
+    it64, rel64 = o.solve64(x0=p_g, rel_tol=1e-11)
+    p64 = o.field("pressure")
+    e_p = float(np.linalg.norm(p_g - p64) / max(1e-300, np.linalg.norm(p64)))
+    print(f"[bench] FP64 check: |div-div64|/|div64| {e_div:.2e} |p-p64|/|p64| {e_p:.2e} ({it64} FP64 iterations, "
+          f"{time.perf_counter() - t0:.0f}s)", file=sys.stderr, flush=True)
+    return {"project_rc": rc, "div_rel_err": e_div, "p_rel_err": e_p, "fp64_cg_iters": it64, "fp64_rel_residual": rel64}
+
+
+_NET = None  # (UNet, out_scale) with --precond learned
+
+
+def _abi_precond_learned():
+    from eulerianfluidsimulation_amd import _abi
+    return _abi.PRECOND_LEARNED
 
 
 def roofline(name, launches, total_ms):
@@ -238,9 +290,60 @@ def roofline(name, launches, total_ms):
             "algorithmic_bytes_per_cell": per_cell, "algorithmic_bytes_per_launch": b}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def unet_flops(n, widths=(16, 32, 48, 64), inference=True):
+    """FLOPs of one U-Net forward at n^3: 2 * 27 * cin * cout per voxel for each 3x3x3 layer; the
+    output layer counts its one used channel at inference (the library computes only that one)
+    and all 16 in training."""
+    c = widths
+    layers = [(16, c[0], 0), (c[0], c[0], 0), (c[0], c[1], 1), (c[1], c[1], 1), (c[1], c[2], 2), (c[2], c[2], 2),
+              (c[2], c[3], 3), (c[3], c[3], 3), (c[3] + c[2], c[2], 2), (c[2] + c[1], c[1], 1), (c[1] + c[0], c[0], 0),
+              (c[0], 1 if inference else 16, 0)]
+    return sum(2 * 27 * ci * co * (n ** 3) / 8 ** lvl for ci, co, lvl in layers)
+
+
+def unet_bench():
+    import torch
+    from eulerianfluidsimulation_amd import learned
+    n = ARGS.unet
+    net = learned.UNet(seed=1)
+    x = torch.randn((n, n, n, 16), dtype=torch.float32, device="cuda")
+    t = torch.randn((n, n, n), dtype=torch.float32, device="cuda")
+    for _ in range(max(1, ARGS.warmup)):
+        net.forward(x)
+        net.loss_grad(x, t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(ARGS.steps):
+        net.forward(x)
+    torch.cuda.synchronize()
+    fwd_ms = 1e3 * (time.perf_counter() - t0) / ARGS.steps
+    t0 = time.perf_counter()
+    for _ in range(ARGS.steps):
+        net.loss_grad(x, t)
+        net.adam_step(1e-4)
+    torch.cuda.synchronize()
+    train_ms = 1e3 * (time.perf_counter() - t0) / ARGS.steps
+    f_inf = unet_flops(n)
+    f_train = 3 * unet_flops(n, inference=False)  # forward, backward data, backward weights
+    tf = f_inf / (fwd_ms * 1e-3) / 1e12
+    out = {"metric": "U-Net inference and training step at N^3 (learned preconditioner network)", "n": n,
+           "steps": ARGS.steps, "forward_ms": fwd_ms, "train_step_ms": train_ms,
+           "forward_gflop": f_inf / 1e9, "train_gflop": f_train / 1e9,
+           "roofline": {"bound": "mfma", "achieved": tf, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": tf / MFMA_BF16_PEAK_TFLOPS, "traffic": None, "kernel": "U-Net forward (all layers)"},
+           "train_tflops": f_train / (train_ms * 1e-3) / 1e12, "dtype": "bf16 operands, fp32 accumulation"}
+    print(json.dumps(out))
+
+
 def main():
     global ARGS
     ARGS = parse()
+    if ARGS.unet:
+        unet_bench()
+        return
     if ARGS.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {ARGS.gpus} is not supported (the path would shard "
               "by z-slabs across GPUs, see DESIGN.md)", file=sys.stderr)
@@ -268,6 +371,15 @@ def main():
         torch.cuda.synchronize()
         print(f"[bench] copy probe: {ARGS.copy_probe} MiB read + {ARGS.copy_probe} MiB written", file=sys.stderr, flush=True)
         del src, dst
+    if ARGS.precond == "learned":
+        global _NET
+        if not ARGS.learned_weights:
+            print("--precond learned needs --learned-weights", file=sys.stderr)
+            sys.exit(2)
+        from eulerianfluidsimulation_amd import learned
+        net = learned.UNet(seed=0)
+        out_scale = net.load(ARGS.learned_weights)
+        _NET = (net, out_scale)
     main_run = run(sc, p, ARGS.warmup, ARGS.steps, kernel_timing=True, keep_state=not ARGS.no_cpu_baseline)
     steps_per_s = ARGS.steps / main_run["elapsed"]
     iters, true_res, kt = main_run["iters"], main_run["true_res"], main_run["kt"]
@@ -318,8 +430,10 @@ def main():
         "cg_iters_per_step": its,
         "cg_iters_each_step": iters,
         "all_steps_converged": all(rc == 0 for rc in main_run["rcs"]),
+        "learned_fallback_steps": int(sum(main_run["fallbacks"])) if ARGS.precond == "learned" else None,
         "true_rel_residual_max": float(max(true_res)) if true_res else None,
         "plain_cg": plain,
+        "fp64_check": main_run["fp64"],
         "kernel_events": {k: {"launches": v[0], "avg_us": 1e3 * v[1] / max(1, v[0]), "total_ms": v[1]} for k, v in kt.items()},
         "roofline": rooflines[dom],
         "rooflines": timed,

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "efs_internal.h"
 
@@ -70,6 +71,47 @@ __global__ void k_fill_f32(int n, float* a, float v) {
 }
 
 inline int blocks1d(int n) { return (n + 255) / 256; }
+
+// Seeded primitives at interior cell centres ((i - 0.5) h per axis, in double), in order: a solid
+// primitive sets boundary 1 and rho 0, a fluid one rho 1 on non-solid cells; the border is solid
+// with rho 0 (Grid3D.cpp:28-42, Editor.cpp:246-257).  One thread per cell.
+constexpr int RAST_BATCH = 32;
+struct PrimBatch {
+    int n;
+    efs_prim p[RAST_BATCH];
+};
+__global__ void k_rasterise(int W, int H, int D, int is3, double hx, double hy, double hz, PrimBatch pb, float* __restrict__ rho,
+                            float* __restrict__ bnd) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= W * H * D) return;
+    const int x = c % W, y = (c / W) % H, z = c / (W * H);
+    float r = rho[c], b = bnd[c];
+    if (x == 0 || y == 0 || x == W - 1 || y == H - 1 || (is3 && (z == 0 || z == D - 1))) {
+        b = 1.f;
+        r = 0.f;
+    } else {
+        const double cx = (x - 0.5) * hx, cy = (y - 0.5) * hy, cz = is3 ? (z - 0.5) * hz : 0.0;
+        for (int k = 0; k < pb.n; k++) {
+            const efs_prim& P = pb.p[k];
+            bool in;
+            if (P.kind == 0) {
+                in = cx >= P.a[0] && cx < P.b[0] && cy >= P.a[1] && cy < P.b[1] && (!is3 || (cz >= P.a[2] && cz < P.b[2]));
+            } else {
+                const double dx = cx - P.a[0], dy = cy - P.a[1], dz = is3 ? cz - P.a[2] : 0.0;
+                in = dx * dx + dy * dy + dz * dz <= P.b[0] * P.b[0];
+            }
+            if (!in) continue;
+            if (P.solid) {
+                b = 1.f;
+                r = 0.f;
+            } else if (b != 1.f) {
+                r = 1.f;
+            }
+        }
+    }
+    rho[c] = r;
+    bnd[c] = b;
+}
 
 bool valid_handle(efs_sim* s) {
     if (!s) {
@@ -584,49 +626,24 @@ int efs_kernel_time(efs_handle h, int32_t k, int64_t* launches, double* total_ms
 }
 
 int efs_scene_rasterise(efs_handle h, const efs_prim* prims, int32_t nprims) {
-    if (!valid_handle(h) || (nprims > 0 && !prims)) return -1;
-    const int W = h->W, H = h->H, D = h->D;
-    std::vector<float> rho((size_t)h->N), bnd((size_t)h->N);
-    EFS_HIP(hipMemcpyAsync(rho.data(), h->rho, rho.size() * 4, hipMemcpyDeviceToHost, h->stream));
-    EFS_HIP(hipMemcpyAsync(bnd.data(), h->bndf, bnd.size() * 4, hipMemcpyDeviceToHost, h->stream));
-    EFS_HIP(hipStreamSynchronize(h->stream));
+    if (!valid_handle(h) || nprims < 0 || (nprims > 0 && !prims)) return -1;
+    h->traces_valid = false;
     const bool is3 = h->dim == 3;
-    const double hx = 1.0 / (W - 2), hy = 1.0 / (H - 2), hz = is3 ? 1.0 / (D - 2) : 1.0;
-    for (int z = 0; z < D; z++)
-        for (int y = 0; y < H; y++)
-            for (int x = 0; x < W; x++) {
-                const size_t c = ((size_t)z * H + y) * W + x;
-                float r = rho[c], b = bnd[c];
-                if (x == 0 || y == 0 || x == W - 1 || y == H - 1 || (is3 && (z == 0 || z == D - 1))) {
-                    b = 1.f;
-                    r = 0.f;
-                } else {
-                    const double cx = (x - 0.5) * hx, cy = (y - 0.5) * hy, cz = is3 ? (z - 0.5) * hz : 0.0;
-                    for (int k = 0; k < nprims; k++) {
-                        const efs_prim& P = prims[k];
-                        bool in;
-                        if (P.kind == 0) {
-                            in = cx >= P.a[0] && cx < P.b[0] && cy >= P.a[1] && cy < P.b[1] &&
-                                 (!is3 || (cz >= P.a[2] && cz < P.b[2]));
-                        } else {
-                            const double dx = cx - P.a[0], dy = cy - P.a[1], dz = is3 ? cz - P.a[2] : 0.0;
-                            in = dx * dx + dy * dy + dz * dz <= P.b[0] * P.b[0];
-                        }
-                        if (!in) continue;
-                        if (P.solid) {
-                            b = 1.f;
-                            r = 0.f;
-                        } else if (b != 1.f) {
-                            r = 1.f;
-                        }
-                    }
-                }
-                rho[c] = r;
-                bnd[c] = b;
-            }
-    int rc = efs_upload(h, EFS_FIELD_RHO, rho.data(), rho.size() * 4);
+    const double hx = 1.0 / (h->W - 2), hy = 1.0 / (h->H - 2), hz = is3 ? 1.0 / (h->D - 2) : 1.0;
+    // primitives go by value in batches; later batches see the cells earlier ones wrote, which is
+    // the host order of one loop over all primitives
+    for (int k0 = 0; k0 == 0 || k0 < nprims; k0 += RAST_BATCH) {
+        PrimBatch pb{};
+        pb.n = std::min(RAST_BATCH, nprims - k0);
+        for (int k = 0; k < pb.n; k++) pb.p[k] = prims[k0 + k];
+        hipLaunchKernelGGL(k_rasterise, dim3(blocks1d(h->N)), dim3(256), 0, h->stream, h->W, h->H, h->D, is3 ? 1 : 0, hx, hy, hz,
+                           pb, h->rho, h->bndf);
+        EFS_HIP(hipGetLastError());
+    }
+    int rc = efs_launch_boundary_flags(h);
     if (rc) return rc;
-    return efs_upload(h, EFS_FIELD_BOUNDARY, bnd.data(), bnd.size() * 4);
+    EFS_HIP(hipStreamSynchronize(h->stream));
+    return 0;
 }
 
 int efs_scene_face_noise(efs_handle h, uint64_t seed, double sigma) {
@@ -638,18 +655,34 @@ int efs_scene_face_noise(efs_handle h, uint64_t seed, double sigma) {
     EFS_HIP(hipStreamSynchronize(h->stream));
     int rc = efs_download(h, EFS_FIELD_VEL, vel.data(), vel.size() * 4);
     if (rc) return rc;
-    const int z0 = nc == 3 ? 1 : 0, z1 = nc == 3 ? D - 1 : 1;
-    for (int z = z0; z < z1; z++)
-        for (int y = 1; y < H - 1; y++)
-            for (int x = 1; x < W - 1; x++) {
-                const size_t c = ((size_t)z * H + y) * W + x;
-                if (bnd[c] == 1.f) continue;
-                for (int k = 0; k < nc; k++) {
-                    const size_t nb = k == 0 ? c - 1 : (k == 1 ? c - (size_t)W : c - (size_t)W * H);
-                    if (bnd[nb] == 1.f) continue;
-                    vel[c * nc + k] += (float)(sigma * normal01(seed, 1, c * nc + k));
-                }
-            }
+    // every face is independent (counter = cell * nc + component), so host threads split the
+    // planes (in 2D the rows) between them; the values do not depend on the split
+    const bool three = nc == 3;
+    const int lo = 1, hi = three ? D - 1 : H - 1;
+    auto work = [&](int a0, int a1) {
+        for (int a = a0; a < a1; a++) {
+            const int zb = three ? a : 0, ze = three ? a + 1 : 1, yb = three ? 1 : a, ye = three ? H - 1 : a + 1;
+            for (int z = zb; z < ze; z++)
+                for (int y = yb; y < ye; y++)
+                    for (int x = 1; x < W - 1; x++) {
+                        const size_t c = ((size_t)z * H + y) * W + x;
+                        if (bnd[c] == 1.f) continue;
+                        for (int k = 0; k < nc; k++) {
+                            const size_t nb = k == 0 ? c - 1 : (k == 1 ? c - (size_t)W : c - (size_t)W * H);
+                            if (bnd[nb] == 1.f) continue;
+                            vel[c * nc + k] += (float)(sigma * normal01(seed, 1, c * nc + k));
+                        }
+                    }
+        }
+    };
+    const int span = std::max(0, hi - lo);
+    const int nt = (int)std::max(1u, std::min(16u, std::min(std::thread::hardware_concurrency(), (unsigned)std::max(1, span / 8))));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++) {
+        const int a0 = lo + (int)((long long)span * t / nt), a1 = lo + (int)((long long)span * (t + 1) / nt);
+        pool.emplace_back(work, a0, a1);
+    }
+    for (auto& th : pool) th.join();
     return efs_upload(h, EFS_FIELD_VEL, vel.data(), vel.size() * 4);
 }
 

@@ -431,8 +431,11 @@ int efs_learned_cg_solve(efs_sim* s, efs_stats* out) {
     std::memcpy(s->st_host, &init, sizeof(CGState));
     EFS_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(CGState), hipMemcpyHostToDevice, s->stream));
     EFS_HIP(hipMemsetAsync(L->q, 0, (size_t)s->N * 4, s->stream));
+    // with the previous pressure as the initial guess (time stepping) the solve starts from it and
+    // its first flexible iteration applies the network to that residual; otherwise x0 = M(b)
+    const bool from_previous = s->P.initial_guess == EFS_GUESS_PREVIOUS && s->x64_has_previous;
     if ((rc = efs_cg_setup_x0(s))) return rc;
-    if (L->opts.warm_start) {
+    if (L->opts.warm_start && !from_previous) {
         // x0 = M(b) on the fluid rows (identity rows keep x0 = b)
         hipLaunchKernelGGL(k_sumsq, dim3(nbt), bt, 0, s->stream, dt, s->div, s->p_bb);
         if ((rc = apply_M(s, L, dt, s->div, s->p_bb, 1))) return rc;
@@ -440,7 +443,7 @@ int efs_learned_cg_solve(efs_sim* s, efs_stats* out) {
     if (L->opts.every <= 0) {
         // warm start only: the plain solver continues from x0 = M(b)
         rc = efs_cg_run(s, out, false, s->P.max_steps);
-        if (out) out->warm_started = L->opts.warm_start;
+        if (out) out->warm_started = L->opts.warm_start && !from_previous;
         return rc;
     }
     // r0 = b - A x0, d0 unused (the flexible solve forms d from z)
@@ -484,7 +487,7 @@ int efs_learned_cg_solve(efs_sim* s, efs_stats* out) {
             *out = plain;
             out->iterations += fcg_iters;
             out->fallback = 1;
-            out->warm_started = L->opts.warm_start;
+            out->warm_started = L->opts.warm_start && !from_previous;
         }
         return rc;
     }
@@ -496,7 +499,7 @@ int efs_learned_cg_solve(efs_sim* s, efs_stats* out) {
         out->rel_residual = fin.bnorm2 > 0 ? std::sqrt(fin.gamma / fin.bnorm2) : 0.0;
         out->true_rel_residual = -1.0;
         out->fallback = 0;
-        out->warm_started = L->opts.warm_start;
+        out->warm_started = L->opts.warm_start && !from_previous;
     }
     if (s->P.true_residual && out) {
         double num = 0, den = 0;

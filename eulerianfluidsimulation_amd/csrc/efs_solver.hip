@@ -678,7 +678,11 @@ int efs_solver_alloc(efs_sim* s) {
     }
     EFS_HIP(hipMalloc(&s->cg_x64, N * 8));
     EFS_HIP(hipMemset(s->cg_x64, 0, N * 8));
-    s->max_blocks = std::max(launch_blocks(dom_simple(s, true)), launch_blocks(dom_tiled(s, true))) + 64;
+    // the interior launch can have more blocks than the full one (its z chunks round shorter:
+    // 2304 against 2200 at 194^3), so both are counted
+    s->max_blocks = 64;
+    for (bool full : {true, false})
+        s->max_blocks = std::max(s->max_blocks, std::max(launch_blocks(dom_simple(s, full)), launch_blocks(dom_tiled(s, full))) + 64);
     EFS_HIP(hipMalloc(&s->partials, (size_t)s->max_blocks * 2 * sizeof(double)));
     double** pp[] = {&s->p_rr, &s->p_bb, &s->p_dad};
     for (double** p : pp) {

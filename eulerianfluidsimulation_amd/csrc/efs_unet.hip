@@ -1,10 +1,9 @@
 // 3D U-Net of the learned pressure preconditioner: forward, backward and Adam on gfx950.
 //
 // Layout: activations channels-last (NDHWC), channels a multiple of 16.  Convolutions are
-// implicit GEMMs on the matrix cores: v_mfma_f32_16x16x16_bf16 with BF16 operands and FP32
-// accumulation.  A wavefront computes 16 consecutive voxels of one row x every output channel;
-// per tap and per 16-channel input chunk it loads one A fragment (16 voxels x 16 channels) and
-// one B fragment (16 channels x 16 outputs) per output tile.
+// implicit GEMMs on the matrix cores with BF16 operands and FP32 accumulation: the forward and
+// backward-data convolutions stage each block's input halo in LDS and run gfx950's
+// v_mfma_f32_16x16x32_bf16 (k_conv3); the weight gradients use v_mfma_f32_16x16x16_bf16.
 //   forward        y[v][co] = b[co] + sum_tap sum_ci x[v + off(tap)][ci] * W[tap][co][ci]
 //   backward data  dx = the same kernel on dy with W flipped in space and transposed in channels
 //   backward W     dW[tap][co][ci] = sum_v dy[v][co] * x[v + off(tap)][ci]: per-block partials,
@@ -12,6 +11,7 @@
 // Master weights, gradients and Adam moments are FP32; BF16 copies feed the matrix cores.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -32,82 +32,147 @@ __device__ inline uint16_t f2bf(float f) {  // round to nearest even
 }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-template <typename T> __device__ inline bf16x4 load4(const T* p);
-template <> __device__ inline bf16x4 load4<uint16_t>(const uint16_t* p) {
-    const uint2 v = *reinterpret_cast<const uint2*>(p);
-    bf16x4 r;
-    r[0] = (short)(v.x & 0xffff); r[1] = (short)(v.x >> 16);
-    r[2] = (short)(v.y & 0xffff); r[3] = (short)(v.y >> 16);
-    return r;
-}
-template <> __device__ inline bf16x4 load4<float>(const float* p) {
-    const float4 v = *reinterpret_cast<const float4*>(p);
-    bf16x4 r;
-    r[0] = (short)f2bf(v.x); r[1] = (short)f2bf(v.y); r[2] = (short)f2bf(v.z); r[3] = (short)f2bf(v.w);
-    return r;
+constexpr int MAXNT = 8;  // up to 128 output channels per wavefront (weight gradients)
+
+// ---- 3x3x3 convolution (zero padding) as an implicit GEMM on v_mfma_f32_16x16x32_bf16 --------
+// A 256-thread block computes a CT_X x CT_Y x CT_Z tile of output voxels for NT 16-channel output
+// tiles.  Per 16-channel input chunk the block stages the tile's input halo (HT_X x HT_Y x HT_Z
+// voxels x 16 channels, 23 KB) in LDS once and reuses it for all 27 taps: the GEMM's K runs over
+// (tap, channel) in 14 steps of 32 (two taps of 16 channels; the 28th tap has zero weights).  Wave
+// w owns output rows y0 + 4 (w & 1) + r (r = 0..3) of plane z0 + (w >> 1): four 16-voxel M tiles.
+// Operand layout of the 16x16x32 BF16 MFMA: lane l supplies A row l % 16 (a voxel) and B column
+// l % 16 (an output channel) for K elements 8 (l / 16) .. 8 (l / 16) + 7, here 8 consecutive
+// channels of one tap, a 16-byte LDS (A) or global (B) load; lane l receives D rows
+// 4 (l / 16) + i, i = 0..3, of column l % 16.
+constexpr int CT_X = 16, CT_Y = 8, CT_Z = 2;
+constexpr int HT_X = CT_X + 2, HT_Y = CT_Y + 2, HT_Z = CT_Z + 2, HT_N = HT_X * HT_Y * HT_Z;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Stages the halo of the block's tile for channels [c0, c0 + 16) of a source with Csrc channels
+// (zero outside the D x H x W volume).  With `up`, the source is at half resolution and each fine
+// voxel reads its parent (nearest upsampling).
+__device__ inline void stage_halo(uint4* tile, const uint16_t* __restrict__ src, int Csrc, int c0, int x0, int y0, int z0,
+                                  int D, int H, int W, bool up) {
+    const int sH = up ? H >> 1 : H, sW = up ? W >> 1 : W, sh = up ? 1 : 0;
+    for (int i = threadIdx.x; i < HT_N * 2; i += 256) {
+        const int v = i >> 1, half = i & 1;
+        const int hx = v % HT_X, hy = (v / HT_X) % HT_Y, hz = v / (HT_X * HT_Y);
+        const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+        uint4 val = make_uint4(0u, 0u, 0u, 0u);
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D)
+            val = *reinterpret_cast<const uint4*>(src + (((size_t)(gz >> sh) * sH + (gy >> sh)) * sW + (gx >> sh)) * Csrc + c0 +
+                                                  half * 8);
+        tile[i] = val;
+    }
 }
 
-constexpr int MAXNT = 8;  // up to 128 output channels per wavefront
-
-// Convolution (kernel size 3 with zero padding, or 1).  One wavefront per 16-voxel row tile.
-template <typename TI>
-__global__ __launch_bounds__(256) void k_conv_fwd(const TI* __restrict__ in, int Cin, const uint16_t* __restrict__ w,
-                                                  const float* __restrict__ bias, int Cout, int D, int H, int W, int ks,
-                                                  int relu, int ch0, float* __restrict__ out_f, uint16_t* __restrict__ out_b) {
-    const int lane = threadIdx.x & 63;
-    const int tiles_x = (W + 15) >> 4;
-    const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const long long ntiles = (long long)tiles_x * H * D;
-    if (t >= ntiles) return;
-    const int tx = (int)(t % tiles_x);
-    const long long rest = t / tiles_x;
-    const int y = (int)(rest % H), z = (int)(rest / H);
-    const int x0 = tx * 16;
-    const int row = lane & 15, kq = lane >> 4;
-    const int nt = Cout >> 4, nc = Cin >> 4;
-    floatx4 acc[MAXNT];
+// out[v][co] = bias[co] + sum_tap sum_ci in[v + off(tap)][ci] * w[tap][co][ci], then ReLU (relu),
+// written as FP32 (out_f) and / or BF16 (out_b).  With `up` (Cup channels at half resolution) the
+// input is the decoder's concatenation, channels [0, Cup) the upsampled `up` and the rest `in`
+// (Cin - Cup channels), read in place.  blockIdx.z = z tile * ngroups + output-channel group; a
+// group covers NT 16-channel tiles (tiles past Cout are skipped).  Cin, Cup, Cout: multiples of 16;
+// D, H, W: any (partial tiles are masked; with `up`, even).
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, int Cup, const uint16_t* __restrict__ in, int Cin,
+                                               const uint16_t* __restrict__ w, const float* __restrict__ bias, int Cout, int D,
+                                               int H, int W, int relu, int ngroups, float* __restrict__ out_f,
+                                               uint16_t* __restrict__ out_b) {
+    __shared__ uint4 tile[HT_N * 2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const int x0 = blockIdx.x * CT_X, y0 = blockIdx.y * CT_Y;
+    const int z0 = (blockIdx.z / ngroups) * CT_Z, nt0 = (blockIdx.z % ngroups) * NT;
+    const int ry = (wave & 1) * 4, rz = wave >> 1;
+    floatx4 acc[4][NT];
 #pragma unroll
-    for (int i = 0; i < MAXNT; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int ntap = ks == 3 ? 27 : 1;
-    for (int tap = 0; tap < ntap; tap++) {
-        const int dz = ks == 3 ? tap / 9 - 1 : 0, dy = ks == 3 ? (tap / 3) % 3 - 1 : 0, dx = ks == 3 ? tap % 3 - 1 : 0;
-        const int xx = x0 + row + dx, yy = y + dy, zz = z + dz;
-        const bool ok = xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D;
-        const long long base = (((long long)zz * H + yy) * W + xx) * Cin + kq * 4;
-        const uint16_t* wt = w + ((size_t)tap * Cout) * Cin + (size_t)row * Cin + kq * 4;
-        for (int cc = 0; cc < nc; cc++) {
-            bf16x4 a = ok ? load4<TI>(in + base + cc * 16) : bf16x4{0, 0, 0, 0};
+    for (int r = 0; r < 4; r++)
 #pragma unroll
-            for (int n = 0; n < MAXNT; n++) {
-                if (n < nt) {
-                    const bf16x4 b = load4<uint16_t>(wt + (size_t)n * 16 * Cin + cc * 16);
-                    acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
-                }
+        for (int n = 0; n < NT; n++) acc[r][n] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int nchunks = Cin >> 4;
+    for (int ch = 0; ch < nchunks; ch++) {
+        if (ch) __syncthreads();
+        if (up && ch * 16 < Cup) stage_halo(tile, up, Cup, ch * 16, x0, y0, z0, D, H, W, true);
+        else stage_halo(tile, in, Cin - Cup, ch * 16 - Cup, x0, y0, z0, D, H, W, false);
+        __syncthreads();
+#pragma unroll 2
+        for (int s = 0; s < 14; s++) {
+            const int tap = 2 * s + (g >> 1);
+            const int tc = tap < 27 ? tap : 26;  // the padding tap reads valid LDS; its weights are 0
+            const int dz = tc / 9, dy = (tc / 3) % 3, dx = tc % 3;
+            bf16x8 b[NT];
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const int co = (nt0 + n) * 16 + m;
+                uint4 bv = make_uint4(0u, 0u, 0u, 0u);
+                if (tap < 27 && co < Cout)
+                    bv = *reinterpret_cast<const uint4*>(w + ((size_t)tap * Cout + co) * Cin + ch * 16 + (g & 1) * 8);
+                b[n] = __builtin_bit_cast(bf16x8, bv);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int hv = ((rz + dz) * HT_Y + (ry + r + dy)) * HT_X + (m + dx);
+                const bf16x8 a = __builtin_bit_cast(bf16x8, tile[hv * 2 + (g & 1)]);
+#pragma unroll
+                for (int n = 0; n < NT; n++) acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[n], acc[r][n], 0, 0, 0);
             }
         }
     }
-    // D[i][j]: lane j + 16 * (i / 4), element i % 4  (i = voxel row, j = output channel)
+    const int z = z0 + rz;
+    if (z >= D) return;
 #pragma unroll
-    for (int n = 0; n < MAXNT; n++) {
-        if (n >= nt) break;
-        const int co = n * 16 + row;
+    for (int n = 0; n < NT; n++) {
+        const int co = (nt0 + n) * 16 + m;
+        if (co >= Cout) continue;
         const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int x = x0 + kq * 4 + e;
-            if (x >= W) continue;
-            float v = acc[n][e] + bv;
-            if (relu) v = v > 0.f ? v : 0.f;
-            const long long vox = ((long long)z * H + y) * W + x;
-            const long long o = vox * Cout + co;
-            if (ch0) {  // compact output of channel 0 only
-                if (co == 0) out_f[vox] = v;
-                continue;
+        for (int r = 0; r < 4; r++) {
+            const int y = y0 + ry + r;
+            if (y >= H) continue;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int x = x0 + 4 * g + e;
+                if (x >= W) continue;
+                float v = acc[r][n][e] + bv;
+                if (relu) v = v > 0.f ? v : 0.f;
+                const size_t o = (((size_t)z * H + y) * W + x) * Cout + co;
+                if (out_f) out_f[o] = v;
+                if (out_b) out_b[o] = f2bf(v);
             }
-            if (out_f) out_f[o] = v;
-            if (out_b) out_b[o] = f2bf(v);
         }
     }
+}
+
+// Output channel 0 only of a 16 -> Cout 3x3x3 convolution without activation (the U-Net's last
+// layer at inference, whose other output channels are unused): one voxel per thread, the 27 x 16
+// products from the LDS halo and weights, summed in FP32.
+__global__ __launch_bounds__(256) void k_conv3_ch0(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w, int Cout,
+                                                   const float* __restrict__ bias, int D, int H, int W, float* __restrict__ out0) {
+    __shared__ uint4 tile[HT_N * 2];
+    __shared__ __align__(16) float wsh[27 * 16];
+    const int x0 = blockIdx.x * CT_X, y0 = blockIdx.y * CT_Y, z0 = blockIdx.z * CT_Z;
+    stage_halo(tile, in, 16, 0, x0, y0, z0, D, H, W, false);
+    for (int i = threadIdx.x; i < 27 * 16; i += 256) wsh[i] = bf2f(w[((size_t)(i >> 4) * Cout) * 16 + (i & 15)]);
+    __syncthreads();
+    const int tx = threadIdx.x & 15, ty = (threadIdx.x >> 4) & 7, tz = threadIdx.x >> 7;
+    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+    if (x >= W || y >= H || z >= D) return;
+    float sum = 0.f;
+    for (int tap = 0; tap < 27; tap++) {
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+        const int hv = ((tz + dz) * HT_Y + (ty + dy)) * HT_X + (tx + dx);
+        const uint4 lo = tile[hv * 2], hi = tile[hv * 2 + 1];  // the voxel's 16 channels, two 16-byte reads
+        const unsigned wv[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        const float4* wt4 = reinterpret_cast<const float4*>(wsh + tap * 16);  // the tap's 16 weights, four 16-byte reads
+        const float4 w4[4] = {wt4[0], wt4[1], wt4[2], wt4[3]};
+        const float wf[16] = {w4[0].x, w4[0].y, w4[0].z, w4[0].w, w4[1].x, w4[1].y, w4[1].z, w4[1].w,
+                              w4[2].x, w4[2].y, w4[2].z, w4[2].w, w4[3].x, w4[3].y, w4[3].z, w4[3].w};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            sum += bf2f((uint16_t)(wv[k] & 0xffffu)) * wf[2 * k];
+            sum += bf2f((uint16_t)(wv[k] >> 16)) * wf[2 * k + 1];
+        }
+    }
+    out0[((size_t)z * H + y) * W + x] = sum + (bias ? bias[0] : 0.f);
 }
 
 // Weight gradient partials: block (tap, co tile, chunk) sums over its voxel rows.
@@ -182,19 +247,26 @@ __global__ void k_reduce_chunks(int n, int nchunks, const float* __restrict__ pa
     out[i] = s;
 }
 
-// bias gradient: sum over voxels of dy[v][co] (fixed-order per channel)
-__global__ void k_bias_grad(const uint16_t* __restrict__ dy, int C, long long nvox, float* __restrict__ g) {
-    const int co = blockIdx.x;
-    float s = 0.f;
-    for (long long v = threadIdx.x; v < nvox; v += blockDim.x) s += bf2f(dy[v * C + co]);
+// Bias gradient g[co] = sum_v dy[v][co] in two deterministic passes: block b sums voxels
+// [b * per, (b + 1) * per) for every channel (thread t: channel t % C, every R-th voxel, R =
+// 256 / C; coalesced), the R sums of a channel added in order into part[b][co]; then
+// k_reduce_chunks adds the blocks' partials in block order.
+constexpr int BG_BLOCKS = 512;
+__global__ __launch_bounds__(256) void k_bias_part(const uint16_t* __restrict__ dy, int C, long long nvox, long long per,
+                                                   float* __restrict__ part) {
     __shared__ float red[256];
-    red[threadIdx.x] = s;
+    const int R = 256 / C, t = threadIdx.x, c = t % C, r = t / C;
+    const long long v0 = (long long)blockIdx.x * per, v1 = min(nvox, v0 + per);
+    float s = 0.f;
+    if (r < R)
+        for (long long v = v0 + r; v < v1; v += R) s += bf2f(dy[v * C + c]);
+    red[t] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+    if (t < C) {
+        float a = 0.f;
+        for (int k = 0; k < R; k++) a += red[k * C + t];
+        part[(size_t)blockIdx.x * C + t] = a;
     }
-    if (threadIdx.x == 0) g[co] = red[0];
 }
 
 // dpre = dy * (y > 0) (y is the post-ReLU activation), as BF16 MFMA operand
@@ -471,11 +543,36 @@ int prep_weights(efs_unet_s* u) {
     return 0;
 }
 
+// A 3x3x3 convolution of BF16 channels-last input with `cin` channels into `cout` channels
+// (k_conv3: NT output tiles per block, the rest in further block groups).
+void conv3(hipStream_t st, const uint16_t* in, int cin, const uint16_t* w, const float* bias, int cout, int D, int H, int W,
+           int relu, float* of, uint16_t* ob, const uint16_t* up = nullptr, int cup = 0) {
+    const int ntiles = cout / 16;
+    const int ntile_blocks = ((W + CT_X - 1) / CT_X) * ((H + CT_Y - 1) / CT_Y) * ((D + CT_Z - 1) / CT_Z);
+    // few voxel tiles (the coarse levels): one output tile per block, so that more blocks share
+    // the work; otherwise up to 4 per block, which reuses each staged halo for more outputs
+    int NT = ntiles <= 4 ? ntiles : (ntiles % 4 == 0 ? 4 : (ntiles % 3 == 0 ? 3 : 4));
+    if (ntile_blocks * ((ntiles + NT - 1) / NT) < 1024) NT = 1;
+    const int ngroups = (ntiles + NT - 1) / NT;
+    const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, ((D + CT_Z - 1) / CT_Z) * ngroups);
+    switch (NT) {
+        case 1: hipLaunchKernelGGL(k_conv3<1>, grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, ob); break;
+        case 2: hipLaunchKernelGGL(k_conv3<2>, grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, ob); break;
+        case 3: hipLaunchKernelGGL(k_conv3<3>, grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, ob); break;
+        default: hipLaunchKernelGGL(k_conv3<4>, grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, ob); break;
+    }
+}
+
+// `up` (cup channels at half resolution): the layer's input is up(up) ++ in, read in place.
 template <typename TI>
-void conv_fwd(efs_unet_s* u, const Conv& l, const TI* in, int D, int H, int W, float* of, uint16_t* ob, int ch0 = 0) {
-    const long long tiles = (long long)((W + 15) / 16) * H * D;
-    hipLaunchKernelGGL(k_conv_fwd<TI>, dim3(nb(tiles, 4)), dim3(256), 0, u->stream, in, l.cin, u->wb + l.woff, u->w + l.boff,
-                       l.cout, D, H, W, l.ks, l.relu, ch0, of, ob);
+void conv_fwd(efs_unet_s* u, const Conv& l, const TI* in, int D, int H, int W, float* of, uint16_t* ob, int ch0 = 0,
+              const uint16_t* up = nullptr, int cup = 0) {
+    if (ch0) {  // channel 0 only (inference output); the layer has 16 input channels
+        const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, (D + CT_Z - 1) / CT_Z);
+        hipLaunchKernelGGL(k_conv3_ch0, grid, dim3(256), 0, u->stream, in, u->wb + l.woff, l.cout, u->w + l.boff, D, H, W, of);
+        return;
+    }
+    conv3(u->stream, in, l.cin, u->wb + l.woff, u->w + l.boff, l.cout, D, H, W, l.relu, of, ob, up, cup);
 }
 
 // forward from the BF16 input already in u->inb; compact = write only channel 0 into u->out0
@@ -499,15 +596,19 @@ int forward(efs_unet_s* u, bool compact = false) {
                        D / 8, H / 8, W / 8, u->p3);
     conv_fwd<uint16_t>(u, L[6], u->p3, D / 8, H / 8, W / 8, nullptr, u->act[6]);
     conv_fwd<uint16_t>(u, L[7], u->act[6], D / 8, H / 8, W / 8, nullptr, u->act[7]);
-    hipLaunchKernelGGL(k_up_concat, dim3(nb((long long)D / 4 * H / 4 * W / 4 * (c[3] + c[2]))), dim3(256), 0, u->stream,
-                       u->act[7], c[3], u->act[5], c[2], D / 4, H / 4, W / 4, u->cat2);
-    conv_fwd<uint16_t>(u, L[8], u->cat2, D / 4, H / 4, W / 4, nullptr, u->act[8]);
-    hipLaunchKernelGGL(k_up_concat, dim3(nb((long long)D / 2 * H / 2 * W / 2 * (c[2] + c[1]))), dim3(256), 0, u->stream,
-                       u->act[8], c[2], u->act[3], c[1], D / 2, H / 2, W / 2, u->cat1);
-    conv_fwd<uint16_t>(u, L[9], u->cat1, D / 2, H / 2, W / 2, nullptr, u->act[9]);
-    hipLaunchKernelGGL(k_up_concat, dim3(nb((long long)D * H * W * (c[1] + c[0]))), dim3(256), 0, u->stream, u->act[9], c[1],
-                       u->act[1], c[0], D, H, W, u->cat0);
-    conv_fwd<uint16_t>(u, L[10], u->cat0, D, H, W, nullptr, u->act[10]);
+    // decoder: each layer reads up(coarser) ++ skip in place; training also materialises the
+    // concatenations, which the weight gradients read
+    conv_fwd<uint16_t>(u, L[8], u->act[5], D / 4, H / 4, W / 4, nullptr, u->act[8], 0, u->act[7], c[3]);
+    conv_fwd<uint16_t>(u, L[9], u->act[3], D / 2, H / 2, W / 2, nullptr, u->act[9], 0, u->act[8], c[2]);
+    conv_fwd<uint16_t>(u, L[10], u->act[1], D, H, W, nullptr, u->act[10], 0, u->act[9], c[1]);
+    if (!compact) {
+        hipLaunchKernelGGL(k_up_concat, dim3(nb((long long)D / 4 * H / 4 * W / 4 * (c[3] + c[2]))), dim3(256), 0, u->stream,
+                           u->act[7], c[3], u->act[5], c[2], D / 4, H / 4, W / 4, u->cat2);
+        hipLaunchKernelGGL(k_up_concat, dim3(nb((long long)D / 2 * H / 2 * W / 2 * (c[2] + c[1]))), dim3(256), 0, u->stream,
+                           u->act[8], c[2], u->act[3], c[1], D / 2, H / 2, W / 2, u->cat1);
+        hipLaunchKernelGGL(k_up_concat, dim3(nb((long long)D * H * W * (c[1] + c[0]))), dim3(256), 0, u->stream, u->act[9], c[1],
+                           u->act[1], c[0], D, H, W, u->cat0);
+    }
     if (compact) conv_fwd<uint16_t>(u, L[11], u->act[10], D, H, W, u->out0, nullptr, 1);
     else conv_fwd<uint16_t>(u, L[11], u->act[10], D, H, W, u->out, nullptr);
     UHIP(hipGetLastError());
@@ -523,7 +624,8 @@ int conv_bwd(efs_unet_s* u, const Conv& l, const TX* x, const uint16_t* y_act, c
     const int ntap = l.ks == 3 ? 27 : 1;
     const long long tiles = (long long)((W + 15) / 16) * H * D;
     const int nchunks = (int)std::max<long long>(1, std::min<long long>(64, tiles / 8));
-    const size_t need = (size_t)nchunks * l.nw();
+    // the weight partials, then (after their reduction, in stream order) the bias partials
+    const size_t need = std::max((size_t)nchunks * l.nw(), (size_t)BG_BLOCKS * l.cout);
     if (need > u->wpart_n) {
         if (u->wpart) hipFree(u->wpart);
         UHIP(hipMalloc(&u->wpart, need * 4));
@@ -532,12 +634,11 @@ int conv_bwd(efs_unet_s* u, const Conv& l, const TX* x, const uint16_t* y_act, c
     hipLaunchKernelGGL(k_conv_wgrad<TX>, dim3(ntap * (l.cout / 16) * nchunks), dim3(64), 0, u->stream, u->dpre, l.cout, x, l.cin,
                        D, H, W, l.ks, nchunks, u->wpart);
     hipLaunchKernelGGL(k_reduce_chunks, dim3(nb(l.nw())), dim3(256), 0, u->stream, (int)l.nw(), nchunks, u->wpart, u->g + l.woff);
-    hipLaunchKernelGGL(k_bias_grad, dim3(l.cout), dim3(256), 0, u->stream, u->dpre, l.cout, nvox, u->g + l.boff);
-    if (gx) {
-        // dx = conv(dpre, W flipped / transposed), no bias, no activation
-        hipLaunchKernelGGL(k_conv_fwd<uint16_t>, dim3(nb(tiles, 4)), dim3(256), 0, u->stream, u->dpre, l.cout, u->wt + l.woff,
-                           (const float*)nullptr, l.cin, D, H, W, l.ks, 0, 0, gx, (uint16_t*)nullptr);
-    }
+    const long long per = (nvox + BG_BLOCKS - 1) / BG_BLOCKS;
+    hipLaunchKernelGGL(k_bias_part, dim3(BG_BLOCKS), dim3(256), 0, u->stream, u->dpre, l.cout, nvox, per, u->wpart);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(nb(l.cout)), dim3(256), 0, u->stream, l.cout, BG_BLOCKS, u->wpart, u->g + l.boff);
+    if (gx)  // dx = conv(dpre, W flipped / transposed), no bias, no activation
+        conv3(u->stream, u->dpre, l.cout, u->wt + l.woff, nullptr, l.cin, D, H, W, 0, gx, nullptr);
     UHIP(hipGetLastError());
     return 0;
 }

@@ -302,3 +302,19 @@ def test_vel_advect_reuses_rho_advect_traces_bitexact(case):
     print(f"[{name}] rho_advect + vel_advect max|dv| = {maxdiff(v, o.field('vel'))}")
     assert nbits_equal(v, o.field("vel"))
     sim.close()
+
+
+@pytest.mark.parametrize("n", [126, 192])
+def test_cg_at_sizes_whose_interior_launch_has_more_blocks(hiplib, n):
+    """At these sizes the CG's interior launch has more blocks than the full-grid one (its z chunks
+    round shorter), which the partial-sum buffers must hold: the solve runs and converges."""
+    sc = scenes.sphere_pool(n, seed=0)
+    sim = HipSimulator3D(*sc.dims, params=gpu_params(sc, criterion=1, true_residual=1, max_steps=4000))
+    sim.load_scene(sc)
+    for s in range(6):
+        sim._stage(s)
+    rc = sim.project()
+    st = sim.last_stats
+    print(f"[launch size n={n}] rc={rc} its={st.iterations} true={st.true_rel_residual:.2e}", flush=True)
+    assert rc == 0 and st.true_rel_residual < 1.5e-5
+    sim.close()
