@@ -112,3 +112,21 @@ def test_reference_bucket_mesh_is_closed():
     p = rng.uniform(-1.0, 1.0, (500, 3))
     w = voxel_ref.winding(p, bnd.reshape(-1, 3, 3)) / (4 * np.pi)
     assert np.all(np.minimum(np.abs(w), np.abs(np.abs(w) - 1.0)) < 1e-6)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "assets", "scenes")
+
+
+def test_golden_reference_scenes_parse_and_resolve_meshes():
+    """The reference scene files kept as fixtures parse; their meshes resolve to the copied OBJ
+    files and load closed (winding 0 or +-4 pi everywhere away from the surface)."""
+    for name, dims, nmesh in (("Dam32", (162, 34, 34), 0), ("Pressure32Depth", (18, 50, 18), 0),
+                              ("Bucket32", (66, 68, 18), 1), ("LIShape32", (64, 116, 16), 1)):
+        spec = scene_json.parse(os.path.join(GOLDEN, name + ".json"))
+        assert spec.resolution == dims and len(spec.meshes) == nmesh, name
+        bnd, flu = scene_json.mesh_triangles(spec)
+        if nmesh:
+            assert len(bnd) > 0, name
+            p = np.random.default_rng(2).uniform(-2.0, 2.0, (300, 3))
+            w = voxel_ref.winding(p, bnd.reshape(-1, 3, 3)) / (4 * np.pi)
+            assert np.all(np.minimum(np.abs(w), np.abs(np.abs(w) - 1.0)) < 1e-6), name
