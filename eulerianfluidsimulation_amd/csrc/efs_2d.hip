@@ -8,12 +8,12 @@
 // sparse advection matrix held as 4 backward entries per row and 4 forward entries per
 // column, with fixed-point scatters).
 //
-// Solvers.  A 2D grid of the sizes the reference runs (130^2 = 16,900 cells) is far too small
-// to fill the chip, and a CG iteration would be a handful of launches of a few microseconds
-// each.  The whole solve therefore runs in one workgroup of 1024 threads that loops on the
-// device: the vectors stay in the L2, each iteration costs three workgroup barriers, and the
-// host waits once per solve.  Dots, alpha and beta are FP64, the iterate is FP64, the operator
-// FP32 -- the same mixed precision as the 3D CG.
+// Solvers.  The CG runs as two multi-block kernels per iteration and Jacobi as one kernel per
+// pass (k2m_*).  Dots, alpha and beta are FP64, the iterate is FP64, the operator FP32 -- the
+// same mixed precision as the 3D CG.  (A whole solve in one 1024-thread workgroup, which avoids
+// the launches, measured slower at every size from 130^2 -- 21.6 against 12.6 us per CG
+// iteration -- and grows with the grid on its one CU: 1.59 ms against 21 us per iteration at
+// 1026^2.)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,7 +25,6 @@
 namespace {
 
 constexpr int BX = 64, BY = 4;
-constexpr int S2_NT = 1024;  // threads of the single-workgroup solvers
 
 inline dim3 grid2_interior(const efs_sim* s) { return dim3((s->W - 2 + BX - 1) / BX, (s->H - 2 + BY - 1) / BY); }
 inline dim3 grid2_full(const efs_sim* s) { return dim3((s->W + BX - 1) / BX, (s->H + BY - 1) / BY); }
@@ -587,19 +586,6 @@ __device__ inline double row2d(unsigned rc, const double* __restrict__ v, int c,
     return sum - (double)((rc >> RC_CNT_SHIFT) & 7u) * xc;
 }
 
-// Sum over the workgroup, in a fixed order; every thread gets the same value.
-__device__ inline double block_sum(double v, double* red) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < S2_NT / 64; w++) t += red[w];
-    __syncthreads();
-    return t;
-}
-
 struct Cg2Args {
     int N, W;
     int max_it;
@@ -608,120 +594,6 @@ struct Cg2Args {
     int true_residual;
     double thr;
 };
-
-// The whole CG solve (CGSolver.cpp:32-83) in one workgroup.  Writes iter, gamma (= r.r),
-// bnorm2, converged, done and true_num into st.
-__global__ __launch_bounds__(S2_NT) void k2_cg(Cg2Args a, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
-                                              double* __restrict__ x64, float* __restrict__ r, float* __restrict__ d,
-                                              float* __restrict__ t, float* __restrict__ p, CGState* st) {
-    __shared__ double red[S2_NT / 64];
-    const int tid = threadIdx.x;
-    const int N = a.N, W = a.W;
-    for (int i = tid; i < N; i += S2_NT)
-        if (!(a.prev && (rcode[i] & RC_CG_ROW))) x64[i] = (double)b[i];
-    __syncthreads();
-    double vr = 0.0, vb = 0.0;
-    for (int i = tid; i < N; i += S2_NT) {
-        const double bi = (double)b[i];
-        const float ri = (float)(bi - row2d(rcode[i], x64, i, W));
-        r[i] = ri;
-        d[i] = ri;
-        vr += (double)ri * (double)ri;
-        vb += bi * bi;
-    }
-    double rr = block_sum(vr, red);
-    const double bb = block_sum(vb, red);
-    const double thr = a.relative ? a.thr * a.thr * bb : a.thr;
-    int iter = 0;
-    bool conv = rr < thr || rr == 0.0;
-    while (!conv && iter < a.max_it && isfinite(rr)) {
-        // t = A d, d.t
-        double vd = 0.0;
-        for (int i = tid; i < N; i += S2_NT) {
-            const float ti = row2(rcode[i], d, i, W);
-            t[i] = ti;
-            vd += (double)d[i] * (double)ti;
-        }
-        const double alpha = rr / block_sum(vd, red);
-        // x += alpha d; r -= alpha t, or r = b - A x on replacement iterations (i % 64 == 0)
-        const bool replace = (iter & 0x3F) == 0;
-        double vn = 0.0;
-        for (int i = tid; i < N; i += S2_NT) {
-            x64[i] = x64[i] + alpha * (double)d[i];
-            if (!replace) {
-                const float rn = (float)((double)r[i] - alpha * (double)t[i]);
-                r[i] = rn;
-                vn += (double)rn * (double)rn;
-            }
-        }
-        if (replace) {
-            __syncthreads();
-            for (int i = tid; i < N; i += S2_NT) {
-                const float rn = (float)((double)b[i] - row2d(rcode[i], x64, i, W));
-                r[i] = rn;
-                vn += (double)rn * (double)rn;
-            }
-        }
-        const double rr_new = block_sum(vn, red);
-        iter++;
-        conv = rr_new < thr || rr_new == 0.0;
-        const double beta = rr_new / rr;
-        rr = rr_new;
-        if (conv) break;
-        for (int i = tid; i < N; i += S2_NT) d[i] = (float)((double)d[i] * beta + (double)r[i]);
-        __syncthreads();
-    }
-    for (int i = tid; i < N; i += S2_NT) p[i] = (float)x64[i];
-    double tn = 0.0;
-    if (a.true_residual) {
-        __syncthreads();
-        for (int i = tid; i < N; i += S2_NT) {
-            const double e = (double)b[i] - row2d(rcode[i], x64, i, W);
-            tn += e * e;
-        }
-        tn = block_sum(tn, red);
-    }
-    if (tid == 0) {
-        st->iter = iter;
-        st->gamma = rr;
-        st->bnorm2 = bb;
-        st->thr = thr;
-        st->converged = conv ? 1 : 0;
-        st->done = 1;
-        st->true_num = tn;
-    }
-}
-
-// Jacobi passes (JacobiSolver.cpp:18-70), one workgroup, ping-pong between p and tmp; the
-// result ends in p.
-__global__ __launch_bounds__(S2_NT) void k2_jacobi(int N, int W, int iters, const uint16_t* __restrict__ rcode,
-                                                  const float* __restrict__ b, float* p, float* tmp) {
-    const int tid = threadIdx.x;
-    float* src = p;
-    float* dst = tmp;
-    for (int it = 0; it < iters; it++) {
-        for (int i = tid; i < N; i += S2_NT) {
-            const unsigned rc = rcode[i];
-            float out = src[i];
-            if (rc & RC_JAC_ROW) {
-                float sum = 0.0f;
-                if (rc & RC_NB_XM) sum += src[i - 1];
-                if (rc & RC_NB_XP) sum += src[i + 1];
-                if (rc & RC_NB_YM) sum += src[i - W];
-                if (rc & RC_NB_YP) sum += src[i + W];
-                const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
-                out = cnt > 0 ? (sum - b[i]) / (float)cnt : 0.0f;
-            }
-            dst[i] = out;
-        }
-        __syncthreads();
-        float* sw = src;
-        src = dst;
-        dst = sw;
-    }
-    if (src != p)
-        for (int i = tid; i < N; i += S2_NT) p[i] = src[i];
-}
 
 __global__ __launch_bounds__(256) void k2_spmv(int N, int W, const uint16_t* __restrict__ rcode, const float* __restrict__ xv,
                                                float* __restrict__ yv) {
@@ -733,6 +605,214 @@ __global__ __launch_bounds__(256) void k2_spmv(int N, int W, const uint16_t* __r
 __global__ void k2_fill_u64(int n, unsigned long long* a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = 0ull;
+}
+
+// ---- solvers (CGSolver.cpp:32-111, JacobiSolver.cpp:18-70) ---------------------------------
+// The CG runs as two kernels per iteration over a fixed grid of at most MB_MAXB blocks
+// (grid-stride loops), and Jacobi as one kernel per pass.  Each CG kernel ends with the per-block
+// partial sums of its dot product; every block of the next kernel adds all partials in the same
+// order, so all blocks hold the same r.r, d.t, alpha and beta and take the same convergence
+// decision without a grid-wide barrier.
+constexpr int MB_NT = 256, MB_MAXB = 1024;
+
+#define MB_CELLS(i, N) for (int i = blockIdx.x * MB_NT + threadIdx.x; i < (N); i += gridDim.x * MB_NT)
+
+__device__ inline double mb_wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// this block's total of v into p[blockIdx.x] (waves in order)
+__device__ inline void mb_partial(double v, double* __restrict__ p, double* red) {
+    v = mb_wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < MB_NT / 64; w++) t += red[w];
+        p[blockIdx.x] = t;
+    }
+}
+
+// the sum of n partials, in a fixed order: the same value in every block
+__device__ inline double mb_total(const double* __restrict__ p, int n, double* red) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += MB_NT) s += p[i];
+    s = mb_wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < MB_NT / 64; w++) t += red[w];
+    __syncthreads();
+    return t;
+}
+
+// x_0: b (CGSolver.cpp:47), or the previous FP64 iterate on CG rows
+__global__ __launch_bounds__(MB_NT) void k2m_init_x(int N, int prev, const uint16_t* __restrict__ rcode,
+                                                   const float* __restrict__ b, double* __restrict__ x64) {
+    MB_CELLS(i, N) {
+        if (!(prev && (rcode[i] & RC_CG_ROW))) x64[i] = (double)b[i];
+    }
+}
+
+// r = b - A x (FP64 operator on the FP64 iterate, rounded to FP32) with the partials of r.r and,
+// for the initial residual, of b.b.  Also the residual replacement of iterations i % 64 == 0.
+__global__ __launch_bounds__(MB_NT) void k2m_residual(int N, int W, const uint16_t* __restrict__ rcode,
+                                                     const float* __restrict__ b, const double* __restrict__ x64,
+                                                     float* __restrict__ r, const CGState* st, double* __restrict__ p_rr,
+                                                     double* __restrict__ p_bb) {
+    __shared__ double red[MB_NT / 64];
+    if (st->done) return;
+    double vr = 0.0, vb = 0.0;
+    MB_CELLS(i, N) {
+        const double bi = (double)b[i];
+        const float ri = (float)(bi - row2d(rcode[i], x64, i, W));
+        r[i] = ri;
+        vr += (double)ri * (double)ri;
+        vb += bi * bi;
+    }
+    mb_partial(vr, p_rr, red);
+    if (p_bb) {
+        __syncthreads();
+        mb_partial(vb, p_bb, red);
+    }
+}
+
+// b.b and the stopping threshold (relative criterion: thr^2 b.b), once before the iterations
+__global__ __launch_bounds__(MB_NT) void k2m_setup(int nb, double thr_in, int relative, const double* __restrict__ p_bb,
+                                                  CGState* st) {
+    __shared__ double red[MB_NT / 64];
+    const double bb = mb_total(p_bb, nb, red);
+    if (threadIdx.x == 0) {
+        st->bnorm2 = bb;
+        st->thr = relative ? thr_in * thr_in * bb : thr_in;
+    }
+}
+
+// Iteration i, first kernel: r_i . r_i from the partials and the convergence test (which ends
+// the solve: later launches return at once); beta_i = rr_i / rr_{i-1} and the direction
+// d_i = d_{i-1} beta_i + r_i (d_0 = r_0), recomputed at each cell's neighbours from r and
+// d_{i-1} so that no block waits for another; t = A d_i and the partials of d_i . t.
+__global__ __launch_bounds__(MB_NT) void k2m_dir_spmv(int N, int W, int nb, int i, const uint16_t* __restrict__ rcode,
+                                                     const float* __restrict__ r, const float* __restrict__ dprev,
+                                                     float* __restrict__ dnew, float* __restrict__ t, CGState* st,
+                                                     const double* __restrict__ p_rr, double* __restrict__ p_dt) {
+    __shared__ double red[MB_NT / 64];
+    if (st->done) return;
+    const double rr = mb_total(p_rr, nb, red);
+    const bool conv = rr < st->thr || rr == 0.0;
+    if (conv || !isfinite(rr)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->iter = i;
+            st->gamma = rr;
+            st->converged = conv ? 1 : 0;
+            st->done = 1;
+        }
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->gam[i & 1] = rr;  // other blocks read the other slot
+    const double beta = i == 0 ? 0.0 : rr / st->gam[(i - 1) & 1];
+    auto dn = [&](int j) -> float { return i == 0 ? r[j] : (float)((double)dprev[j] * beta + (double)r[j]); };
+    double vd = 0.0;
+    MB_CELLS(c, N) {
+        const unsigned rc = rcode[c];
+        const float dc = dn(c);
+        float tc = dc;
+        if (rc & RC_CG_ROW) {
+            float sum = 0.0f;
+            if (rc & RC_NB_XM) sum += dn(c - 1);
+            if (rc & RC_NB_XP) sum += dn(c + 1);
+            if (rc & RC_NB_YM) sum += dn(c - W);
+            if (rc & RC_NB_YP) sum += dn(c + W);
+            tc = sum - (float)((rc >> RC_CNT_SHIFT) & 7u) * dc;
+        }
+        dnew[c] = dc;
+        t[c] = tc;
+        vd += (double)dc * (double)tc;
+    }
+    mb_partial(vd, p_dt, red);
+}
+
+// Iteration i, second kernel: alpha = rr_i / d.t; x += alpha d; r -= alpha t with the partials
+// of r.r.  On replacement iterations only x is updated: k2m_residual then sets r = b - A x.
+__global__ __launch_bounds__(MB_NT) void k2m_update(int N, int nb, int i, int replace, const float* __restrict__ d,
+                                                   const float* __restrict__ t, double* __restrict__ x64,
+                                                   float* __restrict__ r, const CGState* st, const double* __restrict__ p_dt,
+                                                   double* __restrict__ p_rr) {
+    __shared__ double red[MB_NT / 64];
+    if (st->done) return;
+    const double alpha = st->gam[i & 1] / mb_total(p_dt, nb, red);
+    double vn = 0.0;
+    MB_CELLS(c, N) {
+        x64[c] = x64[c] + alpha * (double)d[c];
+        if (!replace) {
+            const float rn = (float)((double)r[c] - alpha * (double)t[c]);
+            r[c] = rn;
+            vn += (double)rn * (double)rn;
+        }
+    }
+    if (!replace) mb_partial(vn, p_rr, red);
+}
+
+// After `launched` iterations: the convergence test of r_launched; the solve also ends at max_it.
+__global__ __launch_bounds__(MB_NT) void k2m_check(int nb, int launched, int max_it, CGState* st,
+                                                  const double* __restrict__ p_rr) {
+    __shared__ double red[MB_NT / 64];
+    if (st->done) return;
+    const double rr = mb_total(p_rr, nb, red);
+    const bool conv = rr < st->thr || rr == 0.0;
+    if (threadIdx.x == 0) {
+        st->iter = launched;
+        st->gamma = rr;
+        st->converged = conv ? 1 : 0;
+        if (conv || !isfinite(rr) || launched >= max_it) st->done = 1;
+    }
+}
+
+// pressure = (float) x and, optionally, the partials of ||b - A x||^2 (FP64)
+__global__ __launch_bounds__(MB_NT) void k2m_final(int N, int W, int true_res, const uint16_t* __restrict__ rcode,
+                                                  const float* __restrict__ b, const double* __restrict__ x64,
+                                                  float* __restrict__ p, double* __restrict__ p_tn) {
+    __shared__ double red[MB_NT / 64];
+    double tn = 0.0;
+    MB_CELLS(c, N) {
+        p[c] = (float)x64[c];
+        if (true_res) {
+            const double e = (double)b[c] - row2d(rcode[c], x64, c, W);
+            tn += e * e;
+        }
+    }
+    if (true_res) mb_partial(tn, p_tn, red);
+}
+
+__global__ __launch_bounds__(MB_NT) void k2m_true_total(int nb, CGState* st, const double* __restrict__ p_tn) {
+    __shared__ double red[MB_NT / 64];
+    const double t = mb_total(p_tn, nb, red);
+    if (threadIdx.x == 0) st->true_num = t;
+}
+
+// One Jacobi pass (JacobiSolver.cpp:18-70) from src into dst: on Jacobi rows (sum of the
+// contributing neighbours - b) / nCount, 0 without neighbours; other cells keep their value.
+__global__ __launch_bounds__(MB_NT) void k2m_jacobi_pass(int N, int W, const uint16_t* __restrict__ rcode,
+                                                        const float* __restrict__ b, const float* __restrict__ src,
+                                                        float* __restrict__ dst) {
+    MB_CELLS(i, N) {
+        const unsigned rc = rcode[i];
+        float out = src[i];
+        if (rc & RC_JAC_ROW) {
+            float sum = 0.0f;
+            if (rc & RC_NB_XM) sum += src[i - 1];
+            if (rc & RC_NB_XP) sum += src[i + 1];
+            if (rc & RC_NB_YM) sum += src[i - W];
+            if (rc & RC_NB_YP) sum += src[i + W];
+            const unsigned cnt = (rc >> RC_CNT_SHIFT) & 7u;
+            out = cnt > 0 ? (sum - b[i]) / (float)cnt : 0.0f;
+        }
+        dst[i] = out;
+    }
 }
 
 // ---- stage launchers -----------------------------------------------------------------------
@@ -835,6 +915,56 @@ int efs2_walls_divergence(efs_sim* s) {
     return launch_divergence(s);
 }
 
+// The CG solve (CGSolverCPU2D::solve, CGSolver.cpp:32-83) from x0 = b or the previous iterate.
+// The host launches the iterations in batches of 32 and reads the state between batches;
+// launches past convergence return at once.
+int cg2_solve(efs_sim* s, const Cg2Args& a) {
+    const int nb = std::min(MB_MAXB, (a.N + MB_NT - 1) / MB_NT);
+    if (nb > s->max_blocks) {
+        efs_set_error("2D solve: launch exceeds the partial-sum buffers");
+        return -1;
+    }
+    CGState init{};
+    std::memcpy(s->st_host, &init, sizeof(CGState));
+    EFS_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(CGState), hipMemcpyHostToDevice, s->stream));
+    const dim3 grd(nb), blk(MB_NT);
+    hipLaunchKernelGGL(k2m_init_x, grd, blk, 0, s->stream, a.N, a.prev, s->rowcode, s->div, s->cg_x64);
+    hipLaunchKernelGGL(k2m_residual, grd, blk, 0, s->stream, a.N, a.W, s->rowcode, s->div, s->cg_x64, s->cg_r, s->st, s->p_rr,
+                       s->p_bb);
+    hipLaunchKernelGGL(k2m_setup, dim3(1), blk, 0, s->stream, nb, a.thr, a.relative, s->p_bb, s->st);
+    constexpr int BATCH = 32;
+    int launched = 0;
+    while (true) {
+        const int n = std::min(BATCH, a.max_it - launched);
+        for (int k = 0; k < n; k++) {
+            const int i = launched + k;
+            const float* dprev = (i & 1) ? s->cg_d0 : s->cg_d1;
+            float* dnew = (i & 1) ? s->cg_d1 : s->cg_d0;
+            hipLaunchKernelGGL(k2m_dir_spmv, grd, blk, 0, s->stream, a.N, a.W, nb, i, s->rowcode, s->cg_r, dprev, dnew, s->cg_x1,
+                               s->st, s->p_rr, s->p_dad);
+            const int replace = (i & 0x3F) == 0;  // residual replacement every 64 iterations (CGSolver.cpp:62-69)
+            hipLaunchKernelGGL(k2m_update, grd, blk, 0, s->stream, a.N, nb, i, replace, (const float*)dnew, (const float*)s->cg_x1,
+                               s->cg_x64, s->cg_r, s->st, s->p_dad, s->p_rr);
+            if (replace)
+                hipLaunchKernelGGL(k2m_residual, grd, blk, 0, s->stream, a.N, a.W, s->rowcode, s->div, s->cg_x64, s->cg_r, s->st,
+                                   s->p_rr, (double*)nullptr);
+        }
+        launched += n;
+        hipLaunchKernelGGL(k2m_check, dim3(1), blk, 0, s->stream, nb, launched, a.max_it, s->st, s->p_rr);
+        EFS_HIP(hipGetLastError());
+        EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
+        EFS_HIP(hipStreamSynchronize(s->stream));
+        if (s->st_host->done) break;
+    }
+    hipLaunchKernelGGL(k2m_final, grd, blk, 0, s->stream, a.N, a.W, a.true_residual, s->rowcode, s->div, s->cg_x64, s->p,
+                       s->p_dad);
+    if (a.true_residual) hipLaunchKernelGGL(k2m_true_total, dim3(1), blk, 0, s->stream, nb, s->st, s->p_dad);
+    EFS_HIP(hipGetLastError());
+    EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
+    EFS_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
 // Pressure solve on the current divergence (p cleared by the caller, as project() does).
 int efs2_solve(efs_sim* s, efs_stats* out) {
     int rc = launch_rowcode(s);
@@ -843,7 +973,15 @@ int efs2_solve(efs_sim* s, efs_stats* out) {
         s->p64_valid = false;
         s->x64_has_previous = false;
         const int iters = s->P.max_steps / 2;
-        hipLaunchKernelGGL(k2_jacobi, dim3(1), dim3(S2_NT), 0, s->stream, s->N, s->W, iters, s->rowcode, s->div, s->p, s->cg_x1);
+        // ping-pong between p and a scratch vector; the result ends in p
+        float* src = s->p;
+        float* dst = s->cg_x1;
+        for (int it = 0; it < iters; it++) {
+            hipLaunchKernelGGL(k2m_jacobi_pass, dim3((s->N + MB_NT - 1) / MB_NT), dim3(MB_NT), 0, s->stream, s->N, s->W,
+                               s->rowcode, s->div, (const float*)src, dst);
+            std::swap(src, dst);
+        }
+        if (src != s->p) EFS_HIP(hipMemcpyAsync(s->p, src, (size_t)s->N * 4, hipMemcpyDeviceToDevice, s->stream));
         EFS_HIP(hipGetLastError());
         EFS_HIP(hipStreamSynchronize(s->stream));
         if (out) {
@@ -862,11 +1000,7 @@ int efs2_solve(efs_sim* s, efs_stats* out) {
     a.relative = s->P.criterion == EFS_CRIT_RELATIVE;
     a.true_residual = s->P.true_residual != 0;
     a.thr = (double)s->P.solver_threshold;
-    hipLaunchKernelGGL(k2_cg, dim3(1), dim3(S2_NT), 0, s->stream, a, s->rowcode, s->div, s->cg_x64, s->cg_r, s->cg_d0, s->cg_d1,
-                       s->p, s->st);
-    EFS_HIP(hipGetLastError());
-    EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
-    EFS_HIP(hipStreamSynchronize(s->stream));
+    if ((rc = cg2_solve(s, a))) return rc;
     const CGState fin = *s->st_host;
     s->p64_valid = true;
     s->x64_has_previous = true;
@@ -951,6 +1085,13 @@ int efs2_alloc(efs_sim* s) {
     EFS_HIP(hipMemset(s->cg_x64, 0, N * 8));
     EFS_HIP(hipMalloc(&s->st, sizeof(CGState)));
     EFS_HIP(hipMemset(s->st, 0, sizeof(CGState)));
+    // per-block partial sums of the CG kernels
+    s->max_blocks = MB_MAXB;
+    double** pp[] = {&s->p_rr, &s->p_bb, &s->p_dad};
+    for (double** p : pp) {
+        EFS_HIP(hipMalloc(p, MB_MAXB * sizeof(double)));
+        EFS_HIP(hipMemset(*p, 0, MB_MAXB * sizeof(double)));
+    }
     EFS_HIP(hipHostMalloc(&s->st_host, sizeof(CGState), hipHostMallocDefault));
     std::memset(s->st_host, 0, sizeof(CGState));
     return 0;

@@ -4,7 +4,8 @@ Bit-exact: scene inputs, vel_decay, vel_extrapolate, vel_advect, add_force (face
 divergence, the 5-point operator, Jacobi and a Jacobi projection.  Within a stated tolerance:
 the CG pressure (FP64 dots and iterate vs the reference's sequential FP32 dots), rho_advect
 (fixed-point sums vs std::map-order FP32 sums), rho_sharpen (apply-all-then-trace vs the
-in-place loop) and free-running trajectories of the 128^2 dam break (config 1).
+in-place loop) and free-running trajectories of the 128^2 dam break (config 1).  The solvers
+also on a 514^2 grid, larger than one CG launch covers.
 """
 import ctypes
 
@@ -27,6 +28,8 @@ TOL2_CG_ERR = 3e-4       # teacher-forced CG pressure vs FP64                  [
 TOL2_CG_VS_REF = 7e-6    # GPU vs reference pressure                           [7.1e-7]
 TOL2_CG_VEL = 1.5e-5     # projected velocity vs the reference, of the scale   [1.4e-6]
 TOL2_C1_RHO = 5e-3       # config 1, 100 steps, max |rho_gpu - rho_ref|        [4.7e-4]
+TOL2_LARGE_CG_ERR = 3e-7  # 514^2, relative 1e-7 CG pressure vs FP64          [2.9e-8]
+TOL2_LARGE_CG_RES = 1.5   # 514^2, CPU residual of the FP32 pressure / that of the FP64 solution rounded to FP32
 
 SCENES = {
     "dam2d": (lambda: scenes.dam2d(40), 6),
@@ -225,6 +228,63 @@ def test_rho_sharpen_2d(case2):
     assert d_par < TOL2_SHARPEN_PAR
     assert d_seq < TOL2_SHARPEN_SEQ
     assert abs(m1 - m0) <= 1e-6 * max(1.0, m0)
+    sim.close()
+
+
+def test_solvers_on_a_grid_larger_than_one_launch(hiplib):
+    """514^2 dam break, the state project() receives in step 2: 264,196 cells, more than the
+    1024 blocks x 256 threads of a CG launch, so the grid-stride loops take a second cell in some
+    threads.  Jacobi is bit-exact; the CG to relative 1e-7 matches an FP64 solve to 1e-10 and
+    passes an FP64 residual check on the CPU; with the reference criterion (r.r < 1e-5) the FP32
+    reference CG cannot converge at this size (it pauses after MAX_STEPS), the FP64-dot CG does."""
+    sc = scenes.dam2d(512)
+    state = oracle_pre_project2(sc, 1)
+    # Jacobi, 200 passes
+    sim, o = fresh2(sc, state, solver_type=3, max_steps=400)
+    sim.project()
+    o.stage(6)
+    g = sim.grid()
+    print(f"[dam2d 512] jacobi: p {maxdiff(g.pressure(), o.field('pressure'))} vel {maxdiff(g.vel(), o.field('vel'))}",
+          flush=True)
+    assert sim.last_stats.iterations == 200
+    assert nbits_equal(g.pressure(), o.field("pressure"))
+    assert nbits_equal(g.vel(), o.field("vel"))
+    sim.close()
+    # CG, relative criterion 1e-7, against FP64 to 1e-10
+    ob = to_oracle2(state, sc, solver_type=3, max_steps=0)
+    ob.stage(6)
+    b = ob.field("divergence").astype(np.float64)
+    bb = float((b * b).sum())
+    o64 = to_oracle2(state, sc, dbl=True, solver_threshold=1e-20 * bb, max_steps=20000)
+    o64.field("divergence")[...] = b
+    assert o64.solve().status == 0
+    p64 = o64.field("pressure").copy()
+    sim = HipSimulator2D(sc.dims[0], sc.dims[1], params=gpu_params(sc, criterion=1, solver_threshold=1e-7, true_residual=1))
+    to_gpu2(sim, state)
+    rc = sim.project()
+    st = sim.last_stats
+    pg = sim.grid().pressure().astype(np.float64)
+    e = np.linalg.norm(pg - p64) / np.linalg.norm(p64)
+    # an FP32 pressure cannot have a smaller residual than the FP64 solution rounded to FP32
+    r_cpu = np.linalg.norm(b - o64.matmul(pg)) / np.sqrt(bb)
+    r_floor = np.linalg.norm(b - o64.matmul(p64.astype(np.float32).astype(np.float64))) / np.sqrt(bb)
+    print(f"[dam2d 512] cg relative 1e-7: rc={rc} its={st.iterations} true={st.true_rel_residual:.2e} "
+          f"|p-p64|/|p64|={e:.2e}; CPU FP64 residual of the FP32 pressure {r_cpu:.3e} (of the rounded FP64 "
+          f"solution {r_floor:.3e})", flush=True)
+    assert rc == 0 and st.true_rel_residual < 2e-7
+    assert e < TOL2_LARGE_CG_ERR
+    assert r_cpu < TOL2_LARGE_CG_RES * r_floor
+    sim.close()
+    # the reference criterion
+    sim, o = fresh2(sc, state)
+    rc = sim.project()
+    st = sim.last_stats
+    ost = o.stage(6)
+    e = np.linalg.norm(sim.grid().pressure().astype(np.float64) - p64) / np.linalg.norm(p64)
+    print(f"[dam2d 512] reference criterion: gpu rc={rc} its={st.iterations} rr={st.rr:.2e} |p-p64|/|p64|={e:.2e}; "
+          f"FP32 reference status={ost.status} its={ost.iterations} rr={ost.err:.3g}", flush=True)
+    assert rc == 0 and st.rr < 1e-5 and e < 1e-4
+    assert ost.status == 1
     sim.close()
 
 
