@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "efs_internal.h"
@@ -44,15 +45,35 @@ constexpr int MAXNT = 8;  // up to 128 output channels per wavefront (weight gra
 // l % 16 (an output channel) for K elements 8 (l / 16) .. 8 (l / 16) + 7, here 8 consecutive
 // channels of one tap, a 16-byte LDS (A) or global (B) load; lane l receives D rows
 // 4 (l / 16) + i, i = 0..3, of column l % 16.
+// The halo is held as two planes of 16-byte halves, channels 0-7 of every voxel, then channels
+// 8-15: the 16 lanes that read one half for 16 consecutive voxels then read 256 contiguous bytes,
+// which spreads them over all LDS banks (voxel-major, 32 bytes apart, every bank took two).
+// With PRE a chunk's B operands for all 14 K steps are loaded into registers before the halo is
+// staged, so the weight loads overlap the staging and the MFMA chain never waits on them: that
+// pays on the coarse levels, whose few blocks leave the latency exposed (16^3: 23 -> 15 us), and
+// costs occupancy where thousands of blocks hide it (128^3: 79 -> 102 us), so PRE is chosen per
+// launch.
 constexpr int CT_X = 16, CT_Y = 8, CT_Z = 2;
 constexpr int HT_X = CT_X + 2, HT_Y = CT_Y + 2, HT_Z = CT_Z + 2, HT_N = HT_X * HT_Y * HT_Z;
+constexpr int KSTEPS = 14;  // 27 taps x 16 channels in steps of 32, the 28th tap zero
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// 8 channels starting at p as 8 BF16 values in a uint4: BF16 sources are copied, FP32 sources
+// (the network input of the C entry point) are rounded to nearest even.
+__device__ inline uint4 load8(const uint16_t* p) { return *reinterpret_cast<const uint4*>(p); }
+__device__ inline uint4 load8(const float* p) {
+    const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+    auto pk = [](float lo, float hi) { return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16); };
+    return make_uint4(pk(a.x, a.y), pk(a.z, a.w), pk(b.x, b.y), pk(b.z, b.w));
+}
 
 // Stages the halo of the block's tile for channels [c0, c0 + 16) of a source with Csrc channels
-// (zero outside the D x H x W volume).  With `up`, the source is at half resolution and each fine
-// voxel reads its parent (nearest upsampling).
-__device__ inline void stage_halo(uint4* tile, const uint16_t* __restrict__ src, int Csrc, int c0, int x0, int y0, int z0,
-                                  int D, int H, int W, bool up) {
+// (zero outside the D x H x W volume) into tile[half * HT_N + voxel].  With `up`, the source is at
+// half resolution and each fine voxel reads its parent (nearest upsampling).
+template <typename TS>
+__device__ inline void stage_halo(uint4* tile, const TS* __restrict__ src, int Csrc, int c0, int x0, int y0, int z0, int D,
+                                  int H, int W, bool up) {
     const int sH = up ? H >> 1 : H, sW = up ? W >> 1 : W, sh = up ? 1 : 0;
     for (int i = threadIdx.x; i < HT_N * 2; i += 256) {
         const int v = i >> 1, half = i & 1;
@@ -60,9 +81,8 @@ __device__ inline void stage_halo(uint4* tile, const uint16_t* __restrict__ src,
         const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
         uint4 val = make_uint4(0u, 0u, 0u, 0u);
         if (gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D)
-            val = *reinterpret_cast<const uint4*>(src + (((size_t)(gz >> sh) * sH + (gy >> sh)) * sW + (gx >> sh)) * Csrc + c0 +
-                                                  half * 8);
-        tile[i] = val;
+            val = load8(src + (((size_t)(gz >> sh) * sH + (gy >> sh)) * sW + (gx >> sh)) * Csrc + c0 + half * 8);
+        tile[half * HT_N + v] = val;
     }
 }
 
@@ -71,12 +91,14 @@ __device__ inline void stage_halo(uint4* tile, const uint16_t* __restrict__ src,
 // input is the decoder's concatenation, channels [0, Cup) the upsampled `up` and the rest `in`
 // (Cin - Cup channels), read in place.  blockIdx.z = z tile * ngroups + output-channel group; a
 // group covers NT 16-channel tiles (tiles past Cout are skipped).  Cin, Cup, Cout: multiples of 16;
-// D, H, W: any (partial tiles are masked; with `up`, even).
-template <int NT>
-__global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, int Cup, const uint16_t* __restrict__ in, int Cin,
+// D, H, W: any (partial tiles are masked; with `up`, even).  `in` is BF16, or FP32 rounded to BF16
+// as it is staged (TIN = float).  With out_pool (D, H, W even) the block also writes the 2x2x2
+// average of its BF16 outputs, the next encoder level's input.
+template <int NT, bool PRE, typename TIN>
+__global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, int Cup, const TIN* __restrict__ in, int Cin,
                                                const uint16_t* __restrict__ w, const float* __restrict__ bias, int Cout, int D,
                                                int H, int W, int relu, int ngroups, float* __restrict__ out_f,
-                                               uint16_t* __restrict__ out_b) {
+                                               uint16_t* __restrict__ out_b, uint16_t* __restrict__ out_pool) {
     __shared__ uint4 tile[HT_N * 2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, g = lane >> 4;
@@ -88,89 +110,152 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, 
     for (int r = 0; r < 4; r++)
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[r][n] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // the lane's LDS offsets of the K steps' taps (two taps per step, selected by g >> 1)
+    int hoff[KSTEPS];
+#pragma unroll
+    for (int s = 0; s < KSTEPS; s++) {
+        const int tap = 2 * s + (g >> 1);
+        const int tc = tap < 27 ? tap : 26;  // the padding tap reads valid LDS; its weights are 0
+        hoff[s] = (g & 1) * HT_N + ((rz + tc / 9) * HT_Y + (ry + (tc / 3) % 3)) * HT_X + (m + tc % 3);
+    }
+    // the B operand of K step s, output tile n, input chunk ch
+    auto load_b = [&](int s, int n, int ch) -> bf16x8 {
+        const int tap = 2 * s + (g >> 1), co = (nt0 + n) * 16 + m;
+        uint4 bv = make_uint4(0u, 0u, 0u, 0u);
+        if (tap < 27 && co < Cout)
+            bv = *reinterpret_cast<const uint4*>(w + ((size_t)tap * Cout + co) * Cin + ch * 16 + (g & 1) * 8);
+        return __builtin_bit_cast(bf16x8, bv);
+    };
+    auto mfma_step = [&](int s, const bf16x8 (&b)[NT]) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const bf16x8 a = __builtin_bit_cast(bf16x8, tile[hoff[s] + r * HT_X]);
+#pragma unroll
+            for (int n = 0; n < NT; n++) acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[n], acc[r][n], 0, 0, 0);
+        }
+    };
     const int nchunks = Cin >> 4;
     for (int ch = 0; ch < nchunks; ch++) {
-        if (ch) __syncthreads();
-        if (up && ch * 16 < Cup) stage_halo(tile, up, Cup, ch * 16, x0, y0, z0, D, H, W, true);
-        else stage_halo(tile, in, Cin - Cup, ch * 16 - Cup, x0, y0, z0, D, H, W, false);
-        __syncthreads();
+        if constexpr (PRE) {
+            bf16x8 b[KSTEPS][NT];
+#pragma unroll
+            for (int s = 0; s < KSTEPS; s++)
+#pragma unroll
+                for (int n = 0; n < NT; n++) b[s][n] = load_b(s, n, ch);
+            if (ch) __syncthreads();
+            if (up && ch * 16 < Cup) stage_halo(tile, up, Cup, ch * 16, x0, y0, z0, D, H, W, true);
+            else stage_halo(tile, in, Cin - Cup, ch * 16 - Cup, x0, y0, z0, D, H, W, false);
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < KSTEPS; s++) mfma_step(s, b[s]);
+        } else {
+            if (ch) __syncthreads();
+            if (up && ch * 16 < Cup) stage_halo(tile, up, Cup, ch * 16, x0, y0, z0, D, H, W, true);
+            else stage_halo(tile, in, Cin - Cup, ch * 16 - Cup, x0, y0, z0, D, H, W, false);
+            __syncthreads();
 #pragma unroll 2
-        for (int s = 0; s < 14; s++) {
-            const int tap = 2 * s + (g >> 1);
-            const int tc = tap < 27 ? tap : 26;  // the padding tap reads valid LDS; its weights are 0
-            const int dz = tc / 9, dy = (tc / 3) % 3, dx = tc % 3;
-            bf16x8 b[NT];
+            for (int s = 0; s < KSTEPS; s++) {
+                bf16x8 b[NT];
 #pragma unroll
-            for (int n = 0; n < NT; n++) {
-                const int co = (nt0 + n) * 16 + m;
-                uint4 bv = make_uint4(0u, 0u, 0u, 0u);
-                if (tap < 27 && co < Cout)
-                    bv = *reinterpret_cast<const uint4*>(w + ((size_t)tap * Cout + co) * Cin + ch * 16 + (g & 1) * 8);
-                b[n] = __builtin_bit_cast(bf16x8, bv);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int hv = ((rz + dz) * HT_Y + (ry + r + dy)) * HT_X + (m + dx);
-                const bf16x8 a = __builtin_bit_cast(bf16x8, tile[hv * 2 + (g & 1)]);
-#pragma unroll
-                for (int n = 0; n < NT; n++) acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[n], acc[r][n], 0, 0, 0);
+                for (int n = 0; n < NT; n++) b[n] = load_b(s, n, ch);
+                mfma_step(s, b);
             }
         }
     }
     const int z = z0 + rz;
-    if (z >= D) return;
+    float bv[NT];
 #pragma unroll
     for (int n = 0; n < NT; n++) {
         const int co = (nt0 + n) * 16 + m;
-        if (co >= Cout) continue;
-        const float bv = bias ? bias[co] : 0.f;
+        bv[n] = bias && co < Cout ? bias[co] : 0.f;
+    }
+    auto value = [&](int n, int r, int e) {  // the output before rounding
+        const float v = acc[r][n][e] + bv[n];
+        return relu ? (v > 0.f ? v : 0.f) : v;
+    };
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        const int co = (nt0 + n) * 16 + m;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int y = y0 + ry + r;
-            if (y >= H) continue;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 const int x = x0 + 4 * g + e;
-                if (x >= W) continue;
-                float v = acc[r][n][e] + bv;
-                if (relu) v = v > 0.f ? v : 0.f;
+                if (co >= Cout || z >= D || y >= H || x >= W) continue;
+                const float v = value(n, r, e);
                 const size_t o = (((size_t)z * H + y) * W + x) * Cout + co;
                 if (out_f) out_f[o] = v;
                 if (out_b) out_b[o] = f2bf(v);
             }
         }
     }
+    if (!out_pool) return;
+    // A pooled cell (the lane's x pair k & 1, y pair k >> 1) averages its 8 children's BF16 values,
+    // summed z-major, then y, then x: the plane z0 wave (rz = 0) adds its four from 0 and hands
+    // the partial through LDS to the plane z0 + 1 wave with the same rows (rz = 1), which adds its
+    // four.
+    auto add4 = [&](float acc4, int n, int k) {
+        const int r = 2 * (k >> 1), e = 2 * (k & 1);
+        acc4 = acc4 + bf2f(f2bf(value(n, r, e)));
+        acc4 = acc4 + bf2f(f2bf(value(n, r, e + 1)));
+        acc4 = acc4 + bf2f(f2bf(value(n, r + 1, e)));
+        return acc4 + bf2f(f2bf(value(n, r + 1, e + 1)));
+    };
+    float* part = reinterpret_cast<float*>(tile);  // NT x 2 (ry) x 4 cells x 64 lanes
+    __syncthreads();                               // every wave is done with the halo
+    if (rz == 0) {
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) part[((n * 2 + (wave & 1)) * 4 + k) * 64 + lane] = add4(0.f, n, k);
+    }
+    __syncthreads();
+    if (rz == 1) {
+        const int D2 = D >> 1, H2 = H >> 1, W2 = W >> 1, pz = z0 >> 1;
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const int co = (nt0 + n) * 16 + m;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int px = (x0 >> 1) + 2 * g + (k & 1), py = ((y0 + ry) >> 1) + (k >> 1);
+                const float sum = add4(part[((n * 2 + (wave & 1)) * 4 + k) * 64 + lane], n, k);
+                if (co < Cout && pz < D2 && py < H2 && px < W2)
+                    out_pool[(((size_t)pz * H2 + py) * W2 + px) * Cout + co] = f2bf(sum * 0.125f);
+            }
+        }
+    }
 }
 
 // Output channel 0 only of a 16 -> Cout 3x3x3 convolution without activation (the U-Net's last
-// layer at inference, whose other output channels are unused): one voxel per thread, the 27 x 16
-// products from the LDS halo and weights, summed in FP32.
+// layer at inference, whose other output channels are unused): one voxel per thread, two BF16
+// products per v_dot2 with FP32 accumulation.  The weights are uniform across the block and come
+// through the scalar path, so the LDS carries only the voxels' channels.
 __global__ __launch_bounds__(256) void k_conv3_ch0(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w, int Cout,
                                                    const float* __restrict__ bias, int D, int H, int W, float* __restrict__ out0) {
     __shared__ uint4 tile[HT_N * 2];
-    __shared__ __align__(16) float wsh[27 * 16];
     const int x0 = blockIdx.x * CT_X, y0 = blockIdx.y * CT_Y, z0 = blockIdx.z * CT_Z;
     stage_halo(tile, in, 16, 0, x0, y0, z0, D, H, W, false);
-    for (int i = threadIdx.x; i < 27 * 16; i += 256) wsh[i] = bf2f(w[((size_t)(i >> 4) * Cout) * 16 + (i & 15)]);
     __syncthreads();
     const int tx = threadIdx.x & 15, ty = (threadIdx.x >> 4) & 7, tz = threadIdx.x >> 7;
     const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
     if (x >= W || y >= H || z >= D) return;
     float sum = 0.f;
-    for (int tap = 0; tap < 27; tap++) {
-        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-        const int hv = ((tz + dz) * HT_Y + (ty + dy)) * HT_X + (tx + dx);
-        const uint4 lo = tile[hv * 2], hi = tile[hv * 2 + 1];  // the voxel's 16 channels, two 16-byte reads
-        const unsigned wv[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-        const float4* wt4 = reinterpret_cast<const float4*>(wsh + tap * 16);  // the tap's 16 weights, four 16-byte reads
-        const float4 w4[4] = {wt4[0], wt4[1], wt4[2], wt4[3]};
-        const float wf[16] = {w4[0].x, w4[0].y, w4[0].z, w4[0].w, w4[1].x, w4[1].y, w4[1].z, w4[1].w,
-                              w4[2].x, w4[2].y, w4[2].z, w4[2].w, w4[3].x, w4[3].y, w4[3].z, w4[3].w};
+    // one plane of 9 taps per iteration: 72 SGPRs of weights in flight
+#pragma unroll 1
+    for (int dz = 0; dz < 3; dz++)
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            sum += bf2f((uint16_t)(wv[k] & 0xffffu)) * wf[2 * k];
-            sum += bf2f((uint16_t)(wv[k] >> 16)) * wf[2 * k + 1];
-        }
+    for (int t9 = 0; t9 < 9; t9++) {
+        const int tap = dz * 9 + t9, dy = t9 / 3, dx = t9 % 3;
+        const int hv = ((tz + dz) * HT_Y + (ty + dy)) * HT_X + (tx + dx);
+        const uint4 lo = tile[hv], hi = tile[HT_N + hv];  // the voxel's channels 0-7 and 8-15
+        const uint4* wt = reinterpret_cast<const uint4*>(w + (size_t)tap * Cout * 16);  // co 0: 16 channels
+        const uint4 wl = wt[0], wh = wt[1];
+        const unsigned xv[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        const unsigned wv[8] = {wl.x, wl.y, wl.z, wl.w, wh.x, wh.y, wh.z, wh.w};
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            sum = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, xv[k]), __builtin_bit_cast(bf16x2, wv[k]), sum, false);
     }
     out0[((size_t)z * H + y) * W + x] = sum + (bias ? bias[0] : 0.f);
 }
@@ -278,25 +363,8 @@ __global__ void k_relu_bwd(long long n, const float* __restrict__ dy, const uint
     dpre[i] = f2bf((!relu || bf2f(y[i]) > 0.f) ? g : 0.f);
 }
 
-// 2x2x2 average pool (dims of the output), channels-last
-__global__ void k_pool_fwd(const uint16_t* __restrict__ in, int C, int D, int H, int W, uint16_t* __restrict__ out) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long n = (long long)D * H * W * C;
-    if (i >= n) return;
-    const int c = (int)(i % C);
-    long long v = i / C;
-    const int x = (int)(v % W); v /= W;
-    const int y = (int)(v % H);
-    const int z = (int)(v / H);
-    const int W2 = W * 2, H2 = H * 2;
-    float s = 0.f;
-    for (int k = 0; k < 8; k++) {
-        const long long q = (((long long)(2 * z + (k >> 2)) * H2 + (2 * y + ((k >> 1) & 1))) * W2 + (2 * x + (k & 1))) * C + c;
-        s += bf2f(in[q]);
-    }
-    out[i] = f2bf(s * 0.125f);
-}
-// backward: d_in[child] += d_out[parent] / 8   (dims of the input / fine level)
+// Backward of the 2x2x2 average pool (the forward is fused into k_conv3), channels-last:
+// d_in[child] += d_out[parent] / 8   (dims of the input / fine level)
 __global__ void k_pool_bwd(const float* __restrict__ dout, int C, int D, int H, int W, float* __restrict__ din) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long n = (long long)D * H * W * C;
@@ -545,55 +613,60 @@ int prep_weights(efs_unet_s* u) {
 
 // A 3x3x3 convolution of BF16 channels-last input with `cin` channels into `cout` channels
 // (k_conv3: NT output tiles per block, the rest in further block groups).
-void conv3(hipStream_t st, const uint16_t* in, int cin, const uint16_t* w, const float* bias, int cout, int D, int H, int W,
-           int relu, float* of, uint16_t* ob, const uint16_t* up = nullptr, int cup = 0) {
+template <typename TIN>
+void conv3(hipStream_t st, const TIN* in, int cin, const uint16_t* w, const float* bias, int cout, int D, int H, int W,
+           int relu, float* of, uint16_t* ob, const uint16_t* up = nullptr, int cup = 0, uint16_t* opool = nullptr) {
     const int ntiles = cout / 16;
     const int ntile_blocks = ((W + CT_X - 1) / CT_X) * ((H + CT_Y - 1) / CT_Y) * ((D + CT_Z - 1) / CT_Z);
     // few voxel tiles (the coarse levels): one output tile per block, so that more blocks share
-    // the work; otherwise up to 4 per block, which reuses each staged halo for more outputs
+    // the work, with its B operands preloaded; otherwise up to 4 per block, which reuses each
+    // staged halo for more outputs
     int NT = ntiles <= 4 ? ntiles : (ntiles % 4 == 0 ? 4 : (ntiles % 3 == 0 ? 3 : 4));
-    if (ntile_blocks * ((ntiles + NT - 1) / NT) < 1024) NT = 1;
+    const bool few = ntile_blocks * ((ntiles + NT - 1) / NT) < 1024;
+    if (few) NT = 1;
     const int ngroups = (ntiles + NT - 1) / NT;
     const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, ((D + CT_Z - 1) / CT_Z) * ngroups);
-    switch (NT) {
-        case 1: hipLaunchKernelGGL(k_conv3<1>, grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, ob); break;
-        case 2: hipLaunchKernelGGL(k_conv3<2>, grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, ob); break;
-        case 3: hipLaunchKernelGGL(k_conv3<3>, grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, ob); break;
-        default: hipLaunchKernelGGL(k_conv3<4>, grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, ob); break;
-    }
+#define EFS_CONV3(nt, pre) \
+    hipLaunchKernelGGL((k_conv3<nt, pre, TIN>), grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, \
+                       ob, opool)
+    if (few) EFS_CONV3(1, true);  // B operands preloaded in registers (56 VGPRs)
+    else if (NT == 1) EFS_CONV3(1, false);
+    else if (NT == 2) EFS_CONV3(2, false);
+    else if (NT == 3) EFS_CONV3(3, false);
+    else EFS_CONV3(4, false);
+#undef EFS_CONV3
 }
 
 // `up` (cup channels at half resolution): the layer's input is up(up) ++ in, read in place.
 template <typename TI>
 void conv_fwd(efs_unet_s* u, const Conv& l, const TI* in, int D, int H, int W, float* of, uint16_t* ob, int ch0 = 0,
-              const uint16_t* up = nullptr, int cup = 0) {
-    if (ch0) {  // channel 0 only (inference output); the layer has 16 input channels
-        const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, (D + CT_Z - 1) / CT_Z);
-        hipLaunchKernelGGL(k_conv3_ch0, grid, dim3(256), 0, u->stream, in, u->wb + l.woff, l.cout, u->w + l.boff, D, H, W, of);
-        return;
+              const uint16_t* up = nullptr, int cup = 0, uint16_t* opool = nullptr) {
+    if constexpr (std::is_same<TI, uint16_t>::value) {
+        if (ch0) {  // channel 0 only (inference output); the layer has 16 input channels
+            const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, (D + CT_Z - 1) / CT_Z);
+            hipLaunchKernelGGL(k_conv3_ch0, grid, dim3(256), 0, u->stream, in, u->wb + l.woff, l.cout, u->w + l.boff, D, H, W, of);
+            return;
+        }
     }
-    conv3(u->stream, in, l.cin, u->wb + l.woff, u->w + l.boff, l.cout, D, H, W, l.relu, of, ob, up, cup);
+    conv3(u->stream, in, l.cin, u->wb + l.woff, u->w + l.boff, l.cout, D, H, W, l.relu, of, ob, up, cup, opool);
 }
 
-// forward from the BF16 input already in u->inb; compact = write only channel 0 into u->out0
-int forward(efs_unet_s* u, bool compact = false) {
+// forward from the BF16 input already in u->inb, or from the FP32 input in32 (rounded to BF16
+// as the first layer stages it); compact = write only channel 0, into out0 (default u->out0)
+int forward(efs_unet_s* u, bool compact = false, const float* in32 = nullptr, float* out0 = nullptr) {
     int rc = prep_weights(u);
     if (rc) return rc;
     const int D = u->D, H = u->H, W = u->W;
     const int* c = u->widths;
     const auto& L = u->layers;
-    conv_fwd<uint16_t>(u, L[0], u->inb, D, H, W, nullptr, u->act[0]);
-    conv_fwd<uint16_t>(u, L[1], u->act[0], D, H, W, nullptr, u->act[1]);
-    hipLaunchKernelGGL(k_pool_fwd, dim3(nb((long long)D / 2 * H / 2 * W / 2 * c[0])), dim3(256), 0, u->stream, u->act[1], c[0],
-                       D / 2, H / 2, W / 2, u->p1);
+    if (in32) conv_fwd<float>(u, L[0], in32, D, H, W, nullptr, u->act[0]);
+    else conv_fwd<uint16_t>(u, L[0], u->inb, D, H, W, nullptr, u->act[0]);
+    // the encoder's second convolutions also write the pooled input of the next level
+    conv_fwd<uint16_t>(u, L[1], u->act[0], D, H, W, nullptr, u->act[1], 0, nullptr, 0, u->p1);
     conv_fwd<uint16_t>(u, L[2], u->p1, D / 2, H / 2, W / 2, nullptr, u->act[2]);
-    conv_fwd<uint16_t>(u, L[3], u->act[2], D / 2, H / 2, W / 2, nullptr, u->act[3]);
-    hipLaunchKernelGGL(k_pool_fwd, dim3(nb((long long)D / 4 * H / 4 * W / 4 * c[1])), dim3(256), 0, u->stream, u->act[3], c[1],
-                       D / 4, H / 4, W / 4, u->p2);
+    conv_fwd<uint16_t>(u, L[3], u->act[2], D / 2, H / 2, W / 2, nullptr, u->act[3], 0, nullptr, 0, u->p2);
     conv_fwd<uint16_t>(u, L[4], u->p2, D / 4, H / 4, W / 4, nullptr, u->act[4]);
-    conv_fwd<uint16_t>(u, L[5], u->act[4], D / 4, H / 4, W / 4, nullptr, u->act[5]);
-    hipLaunchKernelGGL(k_pool_fwd, dim3(nb((long long)D / 8 * H / 8 * W / 8 * c[2])), dim3(256), 0, u->stream, u->act[5], c[2],
-                       D / 8, H / 8, W / 8, u->p3);
+    conv_fwd<uint16_t>(u, L[5], u->act[4], D / 4, H / 4, W / 4, nullptr, u->act[5], 0, nullptr, 0, u->p3);
     conv_fwd<uint16_t>(u, L[6], u->p3, D / 8, H / 8, W / 8, nullptr, u->act[6]);
     conv_fwd<uint16_t>(u, L[7], u->act[6], D / 8, H / 8, W / 8, nullptr, u->act[7]);
     // decoder: each layer reads up(coarser) ++ skip in place; training also materialises the
@@ -609,7 +682,7 @@ int forward(efs_unet_s* u, bool compact = false) {
         hipLaunchKernelGGL(k_up_concat, dim3(nb((long long)D * H * W * (c[1] + c[0]))), dim3(256), 0, u->stream, u->act[9], c[1],
                            u->act[1], c[0], D, H, W, u->cat0);
     }
-    if (compact) conv_fwd<uint16_t>(u, L[11], u->act[10], D, H, W, u->out0, nullptr, 1);
+    if (compact) conv_fwd<uint16_t>(u, L[11], u->act[10], D, H, W, out0 ? out0 : u->out0, nullptr, 1);
     else conv_fwd<uint16_t>(u, L[11], u->act[10], D, H, W, u->out, nullptr);
     UHIP(hipGetLastError());
     return 0;
@@ -812,10 +885,7 @@ int efs_unet_forward_device(efs_unet u, const float* in, int32_t D, int32_t H, i
     if (!u || !in || !out) return -1;
     int rc = unet_alloc_dims(u, D, H, W);
     if (rc) return rc;
-    const long long n = (long long)D * H * W * 16;
-    hipLaunchKernelGGL(k_to_bf16, dim3(nb(n)), dim3(256), 0, u->stream, n, in, u->inb);
-    if ((rc = forward(u, true))) return rc;
-    UHIP(hipMemcpyAsync(out, u->out0, (size_t)D * H * W * 4, hipMemcpyDeviceToDevice, u->stream));
+    if ((rc = forward(u, true, in, out))) return rc;
     UHIP(hipStreamSynchronize(u->stream));
     return 0;
 }
