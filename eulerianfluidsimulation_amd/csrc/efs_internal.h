@@ -362,6 +362,11 @@ struct CgPrecond {
     int batch;
     int first_batch = 0;
     bool writes_rz = false;  // apply also writes the r . z partials (no separate k_cg_rz)
+    // stall detection: stop when r.r has not fallen below stall_ratio times its value
+    // stall_window iterations earlier (0: off); `stalled` reports it
+    int stall_window = 0;
+    double stall_ratio = 0.5;
+    mutable bool stalled = false;
 };
 int efs_cg_run_pc(efs_sim* s, efs_stats* st, bool border, int max_it, const CgPrecond* pc);
 int efs_cg_setup_x0(efs_sim* s);                   // x64 = b, xd = 0

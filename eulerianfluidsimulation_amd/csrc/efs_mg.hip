@@ -728,6 +728,12 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
     pc.batch = EFS_MG_BATCH;
     pc.first_batch = m->last_its > 0 ? m->last_its : EFS_MG_FIRST_BATCH;
     pc.writes_rz = m->fused_rz;
+    // A V-cycle normally cuts r.r by an order of magnitude per iteration.  On a fluid region with
+    // no air neighbour (a pure-Neumann block, A 1 = 0) its output carries a null-space component,
+    // d.Ad collapses and FP32 rounding stalls the residual (a closed box full of liquid: r.r stayed
+    // at 0.3 for 2000 iterations where plain CG met 1e-5 in 15); such a solve restarts as plain CG.
+    pc.stall_window = 20;
+    pc.stall_ratio = 0.5;
     // The recurrence residual stops the iteration; with the true residual enabled and the relative
     // criterion, a solve whose FP64 true residual is still above the tolerance restarts from its
     // iterate (at most twice) while iterations remain under max_steps.  A restart never runs past
@@ -735,6 +741,7 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
     // recurrence criterion, with its true residual in the stats.
     efs_stats local{};
     efs_stats* o = out ? out : &local;
+    o->fallback = 0;
     int total = 0, first_rc = 0;
     for (int attempt = 0; attempt < 3; attempt++) {
         if (attempt > 0 && total >= s->P.max_steps) break;
@@ -745,6 +752,15 @@ int efs_mg_cg_solve(efs_sim* s, efs_stats* out) {
         efs_trim_timer(s->ktimers[KT_MG_DOWN], kt1 + ((size_t)s->st_host->iter + 1));
         if (rc < 0) return rc;
         total += o->iterations;
+        if (pc.stalled) {  // plain CG from x0 = b with the iterations left
+            if ((rc = efs_cg_defer_x0(s))) return rc;
+            rc = efs_cg_run(s, o, false, std::max(1, s->P.max_steps - total));
+            if (rc < 0) return rc;
+            total += o->iterations;
+            o->fallback = 1;
+            first_rc = rc;
+            break;
+        }
         if (attempt == 0) {
             m->last_its = o->iterations;
             first_rc = rc;

@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 #include <cmath>
 #include <cstring>
 
@@ -875,6 +876,8 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
     const int batch = pc ? pc->batch : 64;
     int next_batch = (pc && pc->first_batch > 0) ? pc->first_batch : batch;
     int launched = 0;
+    if (pc) pc->stalled = false;
+    std::vector<std::pair<int, double>> hist;  // (iteration, r.r) at the host checks
     const size_t kt0 = s->ktimers[KT_CG_DIR_SPMV].start.size(), kt1 = s->ktimers[KT_CG_UPDATE].start.size();
     while (true) {
         const int n = std::min(next_batch, max_it - launched);
@@ -917,8 +920,18 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
         EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
         EFS_HIP(hipStreamSynchronize(s->stream));
         if (s->st_host->done || launched >= max_it) break;
+        if (pc && pc->stall_window > 0) {
+            hist.emplace_back(launched, s->st_host->gamma);
+            for (auto h = hist.rbegin(); h != hist.rend(); ++h)
+                if (h->first <= launched - pc->stall_window) {
+                    pc->stalled = !(s->st_host->gamma < pc->stall_ratio * h->second);
+                    break;
+                }
+            if (pc->stalled) break;
+        }
     }
-    const CGState fin = *s->st_host;
+    CGState fin = *s->st_host;
+    if (pc && pc->stalled) fin.iter = launched;  // stopped by the host between checks
     // launches after convergence exit at once; keep the events of real iterations only
     efs_trim_timer(s->ktimers[KT_CG_DIR_SPMV], kt0 + fin.iter);
     size_t repl = (size_t)(fin.iter + 63) / 64;  // replacement iterations have no K2

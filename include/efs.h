@@ -103,7 +103,8 @@ typedef struct {
     double rel_residual;      /* sqrt(rr) / ||b||                                          */
     double true_rel_residual; /* ||b - A p||_2 / ||b||_2 in FP64, or -1 if not computed     */
     double bnorm;             /* ||b||_2                                                   */
-    int32_t fallback;         /* learned preconditioner fell back to plain CG              */
+    int32_t fallback;         /* the preconditioned solve (learned or multigrid) stalled and
+                                 fell back to plain CG                                       */
     int32_t warm_started;     /* learned warm start used                                   */
     float stage_ms[7];        /* device time per stage (0 when timing is off)              */
 } efs_stats;

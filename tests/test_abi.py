@@ -71,3 +71,16 @@ def test_available_solvers_without_gpu():
     assert lib.efs_available_solvers(3, buf, 4) == 3 and list(buf[:3]) == [2, 3, 4]
     assert lib.efs_available_solvers(2, buf, 4) == 2 and list(buf[:2]) == [2, 3]
     assert lib.efs_available_solvers(1, buf, 4) == -1
+
+
+def test_create_rejects_grids_beyond_the_index_range_without_gpu():
+    """The advection arrays hold 8 entries per cell with 32-bit indices: a grid of 2^28 cells or
+    more is refused before any device call (646^3 = 269.6 M cells), and so are grids thinner than
+    one interior cell."""
+    lib = _abi.load()
+    h = ctypes.c_void_p()
+    assert lib.efs_create_3d(646, 646, 646, None, ctypes.byref(h)) == -1 and not h.value
+    assert b"too large" in lib.efs_last_error()
+    assert lib.efs_create_3d(2, 10, 10, None, ctypes.byref(h)) == -1
+    assert b"at least 3 cells" in lib.efs_last_error()
+    assert lib.efs_create_2d(16384, 16385, None, ctypes.byref(h)) == -1
