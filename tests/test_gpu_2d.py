@@ -5,7 +5,8 @@ divergence, the 5-point operator, Jacobi and a Jacobi projection.  Within a stat
 the CG pressure (FP64 dots and iterate vs the reference's sequential FP32 dots), rho_advect
 (fixed-point sums vs std::map-order FP32 sums), rho_sharpen (apply-all-then-trace vs the
 in-place loop) and free-running trajectories of the 128^2 dam break (config 1).  The solvers
-also on a 514^2 grid, larger than one CG launch covers.
+also on a 514^2 grid, larger than one CG launch covers, and edge cases (no fluid, all fluid, one
+cell, odd ragged dims).
 """
 import ctypes
 
@@ -13,6 +14,7 @@ import numpy as np
 import pytest
 
 from eulerianfluidsimulation_amd import HipSimulator2D, scenes
+from eulerianfluidsimulation_amd.scenes import BIG, SPHERE, Prim, Scene, _box
 from oracle.oracle import Oracle2D, params_from_scene
 from tests.helpers import gpu_params, maxdiff, nbits_equal
 
@@ -285,6 +287,53 @@ def test_solvers_on_a_grid_larger_than_one_launch(hiplib):
           f"FP32 reference status={ost.status} its={ost.iterations} rr={ost.err:.3g}", flush=True)
     assert rc == 0 and st.rr < 1e-5 and e < 1e-4
     assert ost.status == 1
+    sim.close()
+
+
+EDGE2 = {
+    "no_fluid": Scene(name="no_fluid2", dims=(14, 12, 0), prims=[_box((0.3, 0.3, -BIG), (0.6, 0.5, BIG), solid=1)],
+                      dt=1 / 120, gravity_force=9.81 * 12),
+    "all_fluid": Scene(name="all_fluid2", dims=(14, 12, 0), prims=[_box((-BIG, -BIG, -BIG), (BIG, BIG, BIG))],
+                       dt=1 / 120, gravity_force=9.81 * 12),
+    "one_cell": Scene(name="one_cell2", dims=(3, 3, 0), prims=[_box((-BIG, -BIG, -BIG), (BIG, BIG, BIG))],
+                      dt=1 / 120, gravity_force=9.81),
+    "odd_ragged": Scene(name="odd_ragged2", dims=(41, 23, 0),
+                        prims=[_box((-BIG, -BIG, -BIG), (BIG, 0.35, BIG)), Prim(SPHERE, 0, (0.3, 0.7, 0.0), (0.2, 0.0, 0.0)),
+                               _box((0.6, 0.2, -BIG), (0.8, 0.6, BIG), solid=1)],
+                        dt=1 / 120, gravity_force=9.81 * 22),
+}
+
+
+@pytest.mark.parametrize("name", sorted(EDGE2))
+def test_edge_scene_2d(hiplib, name):
+    """2D edge cases, four free-running steps from the same seeded scene against the FP32 2D
+    oracle: no fluid (no CG row), all fluid (no air: pure Neumann), one interior cell, odd ragged
+    dims with a solid.  Same status, CG iterations within max(3, 10 %), density within
+    TOL2_C1_RHO, velocity within 1e-4 of its scale, mass to 1e-5."""
+    sc = EDGE2[name]
+    sim = HipSimulator2D(sc.dims[0], sc.dims[1], params=gpu_params(sc))
+    sim.load_scene(sc)
+    o = Oracle2D(sc.dims[0], sc.dims[1], params=params_from_scene(sc))
+    rho0, bnd0, vel0 = o.load_scene(sc)
+    g = sim.grid()
+    assert nbits_equal(g.boundary(), bnd0) and nbits_equal(g.rho(), rho0) and nbits_equal(g.vel(), vel0)
+    worst = 0.0
+    for k in range(4):
+        rc = sim.update()
+        ost = o.step()
+        g = sim.grid()
+        dr = maxdiff(g.rho(), o.field("rho"))
+        worst = max(worst, dr)
+        vscale = max(1.0, float(np.abs(o.field("vel")).max()))
+        dv = maxdiff(g.vel(), o.field("vel"))
+        print(f"[edge2 {name}] step {k}: rc={rc}/{ost.status} its {sim.last_stats.iterations}/{ost.iterations} rho {dr:.3e} "
+              f"vel {dv:.3e} (scale {vscale:.2f})", flush=True)
+        assert rc == ost.status == 0
+        assert abs(sim.last_stats.iterations - ost.iterations) <= max(3, ost.iterations // 10)
+        assert dv <= 1e-4 * vscale
+    m_gpu, m_ref = float(g.rho().astype(np.float64).sum()), float(o.field("rho").astype(np.float64).sum())
+    assert worst < TOL2_C1_RHO
+    assert abs(m_gpu - m_ref) <= 1e-5 * max(1.0, m_ref)
     sim.close()
 
 
