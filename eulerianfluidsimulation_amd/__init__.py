@@ -183,6 +183,17 @@ class HipSimulator3D:
     def get_render_data(self):
         return {"dim": 3, "grid": self.grid()}
 
+    def render_feed(self, vel=True):
+        """The marching-cubes feed (MarchingCubes.cpp:75-105): the density, shaped like the grid,
+        and with vel the velocity as vec4 per cell (w = 0), packed on the device."""
+        import numpy as np
+        n = int(np.prod(self.dims))
+        rho = np.empty(n, np.float32)
+        vel4 = np.empty((n, 4), np.float32) if vel else None
+        check(self.lib.efs_render_feed(self.h, rho.ctypes.data, vel4.ctypes.data if vel else None))
+        shape = tuple(reversed(self.dims)) if len(self.dims) == 3 else (self.dims[1], self.dims[0])
+        return rho.reshape(shape), (vel4.reshape(shape + (4,)) if vel else None)
+
     # -- scenes --
     def load_scene(self, scene):
         prims = (Prim * max(1, len(scene.prims)))()

@@ -101,6 +101,8 @@ SIGNATURES = {
     "efs_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]),
     "efs_upload_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]),
     "efs_download_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]),
+    "efs_render_feed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "efs_render_feed_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "efs_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     "efs_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Stats)]),
     "efs_solve_pressure": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),

@@ -65,6 +65,12 @@ __global__ void k_interleave2(int n, const float* __restrict__ a, const float* _
     aos[2 * (size_t)i] = a[i];
     aos[2 * (size_t)i + 1] = b[i];
 }
+// The face velocities of cell i as (u, v, w, 0); w = 0 in 2D (the renderer's vec4 layout)
+__global__ void k_pack_vel4(int n, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                            float4* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float4(u[i], v[i], w ? w[i] : 0.f, 0.f);
+}
 __global__ void k_fill_f32(int n, float* a, float v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -196,7 +202,7 @@ int transfer(efs_sim* s, int f, void* other, size_t bytes, bool upload, hipMemcp
 
 void free_all(efs_sim* s) {
     float* fl[] = {s->u, s->v, s->w, s->fu, s->fv, s->fw, s->rho, s->gamma, s->beta, s->div, s->p, s->bndf,
-                   s->tu, s->tv, s->tw, s->s0, s->s1, s->bwd_w, s->fwd_w, s->stage_bytes};
+                   s->tu, s->tv, s->tw, s->s0, s->s1, s->bwd_w, s->fwd_w, s->stage_bytes, s->render4};
     for (float* p : fl)
         if (p) hipFree(p);
     if (s->bf) hipFree(s->bf);
@@ -492,6 +498,31 @@ int efs_upload_device(efs_handle h, int32_t f, const void* dev, size_t bytes) {
 }
 int efs_download_device(efs_handle h, int32_t f, void* dev, size_t bytes) {
     return transfer(h, f, dev, bytes, false, hipMemcpyDeviceToDevice);
+}
+
+// The buffers MarchingCubes::updateMesh feeds its shader (MarchingCubes.cpp:75-105): the density
+// (N floats) and the velocity as vec4 (N x 16 bytes), packed on the device.
+static int render_feed(efs_sim* s, float* rho, float* vel4, hipMemcpyKind kind) {
+    if (!valid_handle(s)) return -1;
+    const size_t n = (size_t)s->N;
+    if (rho) EFS_HIP(hipMemcpyAsync(rho, s->rho, n * 4, kind, s->stream));
+    if (vel4) {
+        float4* dst = reinterpret_cast<float4*>(vel4);
+        if (kind != hipMemcpyDeviceToDevice) {
+            if (!s->render4) EFS_HIP(hipMalloc(&s->render4, n * 16));
+            dst = reinterpret_cast<float4*>(s->render4);
+        }
+        hipLaunchKernelGGL(k_pack_vel4, dim3(blocks1d(s->N)), dim3(256), 0, s->stream, s->N, s->u, s->v,
+                           s->dim == 3 ? (const float*)s->w : nullptr, dst);
+        EFS_HIP(hipGetLastError());
+        if (kind != hipMemcpyDeviceToDevice) EFS_HIP(hipMemcpyAsync(vel4, s->render4, n * 16, kind, s->stream));
+    }
+    EFS_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+int efs_render_feed(efs_handle h, float* rho, float* vel4) { return render_feed(h, rho, vel4, hipMemcpyDeviceToHost); }
+int efs_render_feed_device(efs_handle h, float* rho, float* vel4) {
+    return render_feed(h, rho, vel4, hipMemcpyDeviceToDevice);
 }
 
 int efs_stage(efs_handle h, int32_t stage, efs_stats* st) {

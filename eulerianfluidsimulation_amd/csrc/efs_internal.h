@@ -292,6 +292,7 @@ struct efs_sim {
     float *bwd_w = nullptr, *fwd_w = nullptr;             // 8 entries per cell, entry-major
     unsigned long long *acc0 = nullptr, *acc1 = nullptr, *acc2 = nullptr;  // fixed-point accumulators
     float* stage_bytes = nullptr;                         // AoS staging for upload/download
+    float* render4 = nullptr;                             // vec4 velocity of the render feed (allocated on first use)
     // solver
     uint16_t* rowcode = nullptr;
     float *cg_r = nullptr, *cg_d0 = nullptr, *cg_d1 = nullptr, *cg_x1 = nullptr, *cg_z = nullptr;

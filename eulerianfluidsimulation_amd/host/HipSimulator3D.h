@@ -182,6 +182,12 @@ public:
         _grid->setDirty();
         return _h;
     }
+    // The marching-cubes feed (MarchingCubes.cpp:75-105) straight from the device: the density
+    // and the velocity as vec4 per cell (either may be null), after pending mirror edits.
+    void renderFeed(float* rho, float* vel4) {
+        _grid->flush();
+        check(efs_render_feed(_h, rho, vel4), "efs_render_feed");
+    }
 
     // the protected stages of ISimulator (ISimulator.h:43-50)
     void vel_decay() { stage(EFS_STAGE_VEL_DECAY); }

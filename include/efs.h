@@ -142,6 +142,15 @@ int efs_download(efs_handle h, int32_t field, void* host, size_t bytes);
 int efs_upload_device(efs_handle h, int32_t field, const void* device_ptr, size_t bytes);
 int efs_download_device(efs_handle h, int32_t field, void* device_ptr, size_t bytes);
 
+/* Render readback.  Reference: the buffers MarchingCubes::updateMesh (MarchingCubes.cpp:54-105)
+ * feeds its shader for the FluidNode: the density (N floats, rho) and, for the gradient colour
+ * mode, the velocity repacked per cell as vec4 (x, y, z, 0) by a host loop over N cells
+ * (:75-81).  Here the packing runs on the device, and either pointer may be NULL (vel4: N x 16
+ * bytes; 2D: z = 0).  _device writes device buffers (e.g. a shared graphics buffer) without a
+ * host copy. */
+int efs_render_feed(efs_handle h, float* rho, float* vel4);
+int efs_render_feed_device(efs_handle h, float* rho_device, float* vel4_device);
+
 /* Reference: ISimulator::update (ISimulator.h:12-38): the seven stages with disable_mask. */
 int efs_step(efs_handle h, efs_stats* stats);
 /* One stage in isolation (the protected stage methods of ISimulator, ISimulator.h:43-50). */

@@ -1,6 +1,7 @@
 """The C++ host adapter (HipSimulator3D.h: HipSimulator3D / HipSimulator2D) drives the backend:
 mass is conserved and every step converges on a small dam break (3D plain CG, 3D multigrid
-preconditioned CG, 2D)."""
+preconditioned CG, 2D); its mirrors follow the CudaGrid3D protocol; the render feed returns the
+fields the marching-cubes pass reads."""
 import os
 import re
 import subprocess
@@ -36,3 +37,41 @@ def test_cpp_adapter_mirror_protocol(hiplib):
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     assert "MIRROR_CHECK_OK" in out.stdout
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_render_feed_matches_the_fields(hiplib, dim):
+    """efs_render_feed (the marching-cubes feed of MarchingCubes.cpp:75-105): the density equals
+    the downloaded field bit for bit, and the vec4 velocity holds each cell's face velocities with
+    0 in the unused components; the device variant writes the same into device buffers, and the
+    density alone can be asked for."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from eulerianfluidsimulation_amd import HipSimulator2D, HipSimulator3D, scenes
+    from tests.helpers import gpu_params, nbits_equal
+
+    sc = scenes.dam3d(22) if dim == 3 else scenes.dam2d(40)
+    sim = HipSimulator3D(*sc.dims, params=gpu_params(sc)) if dim == 3 else HipSimulator2D(sc.dims[0], sc.dims[1],
+                                                                                          params=gpu_params(sc))
+    sim.load_scene(sc)
+    for _ in range(3):
+        assert sim.update() == 0
+    g = sim.grid()
+    rho, vel = g.rho(), g.vel()
+    frho, fvel4 = sim.render_feed()
+    assert nbits_equal(frho, rho)
+    assert nbits_equal(fvel4[..., :dim], vel)
+    assert not np.any(fvel4[..., dim:])
+    rho_only, none = sim.render_feed(vel=False)
+    assert none is None and nbits_equal(rho_only, rho)
+    n = rho.size
+    drho = torch.empty(n, dtype=torch.float32, device="cuda")
+    dvel = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert sim.lib.efs_render_feed_device(sim.h, ctypes.c_void_p(drho.data_ptr()), ctypes.c_void_p(dvel.data_ptr())) == 0
+    assert nbits_equal(drho.cpu().numpy().reshape(rho.shape), rho)
+    assert nbits_equal(dvel.cpu().numpy(), fvel4.reshape(n, 4))
+    sim.close()
