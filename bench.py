@@ -345,8 +345,8 @@ def main():
         unet_bench()
         return
     if ARGS.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {ARGS.gpus} is not supported (the path would shard "
-              "by z-slabs across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {ARGS.gpus} is not supported (the step would shard by "
+              "z-slabs, one per GPU, with halo-plane exchanges and all-reduced CG dot products)", file=sys.stderr)
         sys.exit(2)
     import copy
 
