@@ -20,11 +20,13 @@
 //   Row arithmetic.  A row's update is (sum - b / s) * (1 / n): s is a power of two, so b / s is
 //     the exact product b * (1 / s); 1 / n is the hardware reciprocal (within an ulp of the
 //     IEEE quotient; the same in every pass, so the V-cycle stays symmetric).
-//   Coarsest level (<= 8 cells per axis): 16 symmetric sweeps in one workgroup, in LDS.
+//   Coarsest level (<= 8 cells per axis): 16 symmetric sweeps in one workgroup, in LDS.  The
+//     level above it runs its whole V-cycle in the same workgroup (k_mg_tail).
 // Non-fluid rows of the fine level return z = 0 (their residual is 0: identity rows).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -89,6 +91,7 @@ struct MG {
     float* z = nullptr;
     double* p_rz = nullptr;
     bool fused_rz = false;  // the fine up sweep writes the r . z partials
+    bool tail = false;      // the last two levels run in one workgroup (k_mg_tail)
     int last_its = 0;       // iterations of the previous solve
 };
 
@@ -529,6 +532,138 @@ __global__ __launch_bounds__(MG_CO_NT) void k_mg_coarsest(Lvl l, float inv_s, in
     for (int t = threadIdx.x; t < N; t += MG_CO_NT) l.x[t] = xs[t];
 }
 
+// The last two levels of the V-cycle in one workgroup (replaces the five sweep and restriction
+// launches of level T and the coarsest solve of level C, which cost more in launch gaps than in
+// work at 16^3 and 8^3).  Both levels sit in LDS with their borders: T's x, b and row codes (up
+// to 18^3 cells, 58 KB) and C's (up to 12^3).  The arithmetic is that of the separate kernels,
+// in the same order, so the result is bit-identical:
+//   down: two red-black sweeps from x = 0 (red, then black), over-relaxed as in k_mg_sweep;
+//   the residual of T's fluid rows restricted to C (child order x, y, z, times 1/8);
+//   C: `sweeps` symmetric sweep pairs from 0 (k_mg_coarsest);
+//   up: x += the parent's correction on T's fluid cells, then two sweeps (black, then red).
+// Within one workgroup the colour passes run in place between barriers: a pass reads only the
+// other colour, which is what the halo-buffered multi-block sweeps read too.  Cells that are not
+// fluid rows hold +0, so the neighbour sums need no masks (on coarse levels a row's neighbour bit
+// is set exactly for its fluid neighbours, and the others add +0 in the same order).
+constexpr int MG_TAIL_MAX = 5832;  // (16 + 2)^3
+constexpr int MG_TAIL_PER = 2;     // cells of one colour per thread (interiors up to 16 cells per axis)
+__global__ __launch_bounds__(MG_CO_NT) void k_mg_tail(Lvl T, Lvl C, float inv_st, float inv_sc, int sweeps, const CGState* st) {
+    if (st->done) return;
+    __shared__ float tx[MG_TAIL_MAX], tb[MG_TAIL_MAX];
+    __shared__ uint16_t tr[MG_TAIL_MAX];
+    __shared__ float cx[MG_CO_MAX], cb[MG_CO_MAX];
+    __shared__ uint16_t cr[MG_CO_MAX];
+    constexpr bool SOR = EFS_MG_NU == 2 && EFS_MG_OMEGA != 1.0f;
+    const int t = threadIdx.x;
+    const int NTc = T.W * T.H * T.D, NCc = C.W * C.H * C.D;
+    for (int i = t; i < NTc; i += MG_CO_NT) {
+        tx[i] = 0.f;
+        tb[i] = T.b[i];
+        tr[i] = T.rc[i];
+    }
+    for (int i = t; i < NCc; i += MG_CO_NT) {
+        cx[i] = 0.f;
+        cb[i] = 0.f;
+        cr[i] = C.rc[i];
+    }
+    // this thread's cells of each colour on T (colour = (x + y + z) & 1; -1: none)
+    int own[2][MG_TAIL_PER];
+    {
+        const int hw = (T.nx + 1) / 2, slots = hw * T.ny * T.nz;
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int j = 0; j < MG_TAIL_PER; j++) {
+                const int sl = t + j * MG_CO_NT;
+                own[c][j] = -1;
+                if (sl < slots) {
+                    const int k = sl % hw, r = sl / hw;
+                    const int y = 1 + r % T.ny, z = 1 + r / T.ny;
+                    const int x = 1 + (((1 + y + z) & 1) != c) + 2 * k;
+                    if (x <= T.nx) own[c][j] = lidx(T, x, y, z);
+                }
+            }
+    }
+    const int sy = T.W, sz = T.W * T.H;
+    __syncthreads();
+    auto tpass = [&](int c) {
+#pragma unroll
+        for (int j = 0; j < MG_TAIL_PER; j++) {
+            const int i = own[c][j];
+            if (i < 0) continue;
+            const unsigned rc = tr[i];
+            const float sum = 0.f + tx[i - 1] + tx[i + 1] + tx[i - sy] + tx[i + sy] + tx[i + sz] + tx[i - sz];
+            tx[i] = relax<SOR>(rc, row_update(rc, sum, tb[i], inv_st), tx[i]);
+        }
+        __syncthreads();
+    };
+    // down: two sweeps, red then black
+    for (int sw = 0; sw < EFS_MG_NU; sw++) {
+        tpass(0);
+        tpass(1);
+    }
+    // restriction of T's residual to C (k_mg_restrict_x)
+    const int cn = C.nx * C.ny * C.nz;
+    for (int q = t; q < cn; q += MG_CO_NT) {
+        const int X = 1 + q % C.nx, Y = 1 + (q / C.nx) % C.ny, Z = 1 + q / (C.nx * C.ny);
+        const int ci = lidx(C, X, Y, Z);
+        float acc = 0.f;
+        if (cr[ci] & RC_CG_ROW) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int fx = 2 * X - 1 + (k & 1), fy = 2 * Y - 1 + ((k >> 1) & 1), fz = 2 * Z - 1 + (k >> 2);
+                if (fx > T.nx || fy > T.ny || fz > T.nz) continue;
+                const int i = lidx(T, fx, fy, fz);
+                const unsigned rc = tr[i];
+                float r = 0.f;
+                if (rc & RC_CG_ROW) {
+                    const float v[6] = {tx[i - 1], tx[i + 1], tx[i - sy], tx[i + sy], tx[i + sz], tx[i - sz]};
+                    const float ax = T.s * (masked_sum6(rc, v) - (float)((rc >> RC_CNT_SHIFT) & 7u) * tx[i]);
+                    r = tb[i] - ax;
+                }
+                acc += r;
+            }
+            acc *= 0.125f;
+        }
+        cb[ci] = acc;
+    }
+    __syncthreads();
+    // coarsest: symmetric sweeps from 0, one cell per thread (k_mg_coarsest)
+    {
+        const int csy = C.W, csz = C.W * C.H;
+        const bool co = t < cn;
+        const int ccx = 1 + t % C.nx, ccy = 1 + (t / C.nx) % C.ny, ccz = 1 + t / (C.nx * C.ny);
+        const int ci = co ? lidx(C, ccx, ccy, ccz) : 0, ccol = (ccx + ccy + ccz) & 1;
+        for (int sw = 0; sw < 2 * sweeps; sw++) {
+            for (int h = 0; h < 2; h++) {
+                const int color = (sw & 1) ? 1 - h : h;
+                if (co && ccol == color) {
+                    const unsigned rc = cr[ci];
+                    const float v[6] = {cx[ci - 1], cx[ci + 1], cx[ci - csy], cx[ci + csy], cx[ci + csz], cx[ci - csz]};
+                    cx[ci] = relax<SOR>(rc, row_update(rc, masked_sum6(rc, v), cb[ci], inv_sc), cx[ci]);
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // up: the parent's correction on T's fluid cells, then two sweeps, black then red
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int j = 0; j < MG_TAIL_PER; j++) {
+            const int i = own[c][j];
+            if (i < 0 || !(tr[i] & RC_CONTRIB)) continue;
+            const int x = i % T.W, y = (i / T.W) % T.H, z = i / (T.W * T.H);
+            tx[i] = tx[i] + cx[lidx(C, (x + 1) >> 1, (y + 1) >> 1, (z + 1) >> 1)];
+        }
+    __syncthreads();
+    for (int sw = 0; sw < EFS_MG_NU; sw++) {
+        tpass(1);
+        tpass(0);
+    }
+    for (int i = t; i < NTc; i += MG_CO_NT) T.x[i] = tx[i];
+}
+
 inline dim3 cell_grid(const Lvl& l) { return dim3((l.W + MB - 1) / MB, (l.H + MBY - 1) / MBY, l.D); }
 
 void free_mg(MG* m) {
@@ -598,6 +733,18 @@ int ensure(efs_sim* s) {
         efs_set_error("multigrid coarsest level too large for the LDS solve");
         return -1;
     }
+    // the tail kernel takes the last two levels when the second-to-last is a coarse level that
+    // fits its LDS arrays and thread slots (EFS_MG_TAIL=0 in the environment keeps the separate
+    // kernels, which compute the same bits)
+    const char* env = std::getenv("EFS_MG_TAIL");
+    m->tail = !(env && env[0] == '0');
+    if (m->L.size() >= 3) {
+        const Lvl& t = m->L[m->L.size() - 2];
+        const long long hw = (t.nx + 1) / 2;
+        m->tail = m->tail && (long long)t.W * t.H * t.D <= MG_TAIL_MAX && hw * t.ny * t.nz <= (long long)MG_TAIL_PER * MG_CO_NT;
+    } else {
+        m->tail = false;
+    }
     return 0;
 }
 
@@ -647,7 +794,8 @@ int vcycle(efs_sim* s, void*) {
     const CGState* st = s->st;
     const dim3 pb(TX, PTY);
     hipEvent_t ta = nullptr, tb = nullptr;
-    for (int k = 0; k < nl - 1; k++) {
+    const int nsep = m->tail ? nl - 2 : nl - 1;  // levels whose down sweeps run as separate kernels
+    for (int k = 0; k < nsep; k++) {
         const Lvl& l = m->L[k];
         const Lvl& c = m->L[k + 1];
         const Dom d = post_dom(l);
@@ -668,9 +816,13 @@ int vcycle(efs_sim* s, void*) {
         }
         if (k == 0) efs_ktimer_add(s, KT_MG_DOWN, ta, tb);
     }
-    hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(MG_CO_NT), 0, s->stream, m->L[nl - 1], 1.f / m->L[nl - 1].s,
-                       EFS_MG_COARSE_SWEEPS, st);
-    for (int k = nl - 2; k >= 0; k--) {
+    if (m->tail)
+        hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(MG_CO_NT), 0, s->stream, m->L[nl - 2], m->L[nl - 1], 1.f / m->L[nl - 2].s,
+                           1.f / m->L[nl - 1].s, EFS_MG_COARSE_SWEEPS, st);
+    else
+        hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(MG_CO_NT), 0, s->stream, m->L[nl - 1], 1.f / m->L[nl - 1].s,
+                           EFS_MG_COARSE_SWEEPS, st);
+    for (int k = nsep - 1; k >= 0; k--) {
         const Lvl& l = m->L[k];
         const Lvl& c = m->L[k + 1];
         const Dom d = post_dom(l);
