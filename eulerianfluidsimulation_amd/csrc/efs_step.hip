@@ -392,13 +392,14 @@ __global__ __launch_bounds__(256) void k_adv_bwd(DevGrid g, float dt, float* __r
         live = sum != 0;
         bp0[c] = live ? p0 : -1;
         if (live) {
+            float a[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                float a = m[k] / sum;
-                if (ga > 1.e-5f) a /= ga;
-                bw[(size_t)k * g.N + c] = a;
-                q[k] = to_fix(a);
+                a[k] = m[k] / sum;
+                if (ga > 1.e-5f) a[k] /= ga;
+                bw[(size_t)k * g.N + c] = a[k];
             }
+            to_fix_n(a, q);
         }
     }
     sbox_vote(box, t >> 6, live, px, py, pz);
@@ -468,12 +469,13 @@ __global__ __launch_bounds__(256) void k_adv_fwd(DevGrid g, float dt, const unsi
         live = sum != 0;
         fq0[c] = live ? q0 : -1;
         if (live) {
+            float a[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const float a = (1 - bj) * (m[k] / sum);
-                fw[(size_t)k * g.N + c] = a;
-                q[k] = to_fix(a);
+                a[k] = (1 - bj) * (m[k] / sum);
+                fw[(size_t)k * g.N + c] = a[k];
             }
+            to_fix_n(a, q);
         }
     }
 #if EFS_FWD_BOX
@@ -518,21 +520,23 @@ __global__ __launch_bounds__(256) void k_adv_rows(DevGrid g, const int* __restri
         float a[8];
         if (live) {
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                a[k] = bw[(size_t)k * g.N + c];
-                s += to_fix(a[k]);
-            }
+            for (int k = 0; k < 8; k++) a[k] = bw[(size_t)k * g.N + c];
+            long long qa[8];
+            to_fix_n(a, qa);
+#pragma unroll
+            for (int k = 0; k < 8; k++) s += qa[k];
         }
         const float gi = (float)((double)s * FIX_INV);
         g6[c] = gi;
         if (live) {
             pc = cell_coords(g, p0);
+            float v[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                float v = a[k];
-                if (gi > 1.e-5f) v /= gi;
-                q[k] = to_fix(v);
+                v[k] = a[k];
+                if (gi > 1.e-5f) v[k] /= gi;
             }
+            to_fix_n(v, q);
         }
     }
     sbox_vote(box, t >> 6, live, pc.x, pc.y, pc.z);
@@ -575,23 +579,27 @@ __global__ __launch_bounds__(256) void k_adv_fwd_cols(DevGrid g, const float* __
                 a[k] = fw[(size_t)k * g.N + c];
             }
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
+            for (int k = 0; k < 8; k++)
                 if (gk[k] > 1.e-5f) a[k] /= gk[k];
-                sum += to_fix(a[k]);
-            }
+            long long qa[8];
+            to_fix_n(a, qa);
+#pragma unroll
+            for (int k = 0; k < 8; k++) sum += qa[k];
         }
         const float bj = (float)from_fix((unsigned long long)sum);
         g.beta[c] = bj;
         if (live) {
             const float rj = g.rho[c];
             qc = cell_coords(g, q0);
+            float v[8], vr[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                float v = a[k];
-                if (bj > 1.e-5f) v /= bj;
-                qg[k] = to_fix(v);
-                qr[k] = to_fix(v * rj);
+                v[k] = a[k];
+                if (bj > 1.e-5f) v[k] /= bj;
+                vr[k] = v[k] * rj;
             }
+            to_fix_n(v, qg);
+            to_fix_n(vr, qr);
         }
     }
     // the gamma terms, then the rho terms through the same box
@@ -631,13 +639,21 @@ __global__ __launch_bounds__(256) void k_adv_final(DevGrid g, const int* __restr
             bc[k] = g.beta[j];
             rc[k] = g.rho[j];
         }
+        float a[8], ar[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            float a = e[k];
-            if (gi > 1.e-5f) a /= gi;
-            if (bc[k] > 1.e-5f) a /= bc[k];
-            gs += to_fix(a);
-            rs += to_fix(a * rc[k]);
+            a[k] = e[k];
+            if (gi > 1.e-5f) a[k] /= gi;
+            if (bc[k] > 1.e-5f) a[k] /= bc[k];
+            ar[k] = a[k] * rc[k];
+        }
+        long long qa[8], qr[8];
+        to_fix_n(a, qa);
+        to_fix_n(ar, qr);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            gs += qa[k];
+            rs += qr[k];
         }
     }
     g.gamma[c] = (float)((double)gs * FIX_INV);
