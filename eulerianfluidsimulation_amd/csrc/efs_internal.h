@@ -62,20 +62,34 @@ constexpr int kBufDword3 = 0x00020000;  // gfx9 buffer descriptor word 3 for raw
 // before and most neighbouring block rows were read through the same L2 (gradient 167 -> 118 us,
 // face pass 108 -> 87 us at 256^3).  The traces, the row codes and the other per-cell passes
 // measured slower so, and so did a division-free variant that gives each XCD whole block rows.
-// Blocks past the last column of a range exit at once.
+// Blocks past the last column of a range exit at once.  The grid is 2D, NXCD * per blocks along x
+// and one row of blocks per plane: the hardware numbers them x fastest, so block x of row z is
+// dealt to XCD x mod NXCD exactly as in a 1D grid of planes, and the plane needs no division.
 constexpr int NXCD = 8;
 struct XcdBlock {
     int bx, by, bz;
 };
+// q = n / d, r = n % d for 0 <= n < 2^22, d >= 1: a float estimate off by at most one, corrected
+// (a runtime integer division costs about thirty scalar instructions)
+__device__ inline void divmod_small(int n, int d, int& q, int& r) {
+    q = (int)((float)n * __builtin_amdgcn_rcpf((float)d));
+    r = n - q * d;
+    if (r < 0) {
+        q--;
+        r += d;
+    } else if (r >= d) {
+        q++;
+        r -= d;
+    }
+}
 __device__ inline bool xcd_block(int gx, int gy, XcdBlock& b) {
     const int ncol = gx * gy, per = (ncol + NXCD - 1) / NXCD;
-    const int k = (int)blockIdx.x / NXCD, col = ((int)blockIdx.x % NXCD) * per + k % per;
-    b.bz = k / per;
-    b.bx = col % gx;
-    b.by = col / gx;
+    const int col = ((int)blockIdx.x % NXCD) * per + (int)blockIdx.x / NXCD;
+    b.bz = blockIdx.y;
+    divmod_small(col, gx, b.by, b.bx);
     return col < ncol;
 }
-inline dim3 xcd_grid(int gx, int gy, int nz) { return dim3(NXCD * ((gx * gy + NXCD - 1) / NXCD) * nz); }
+inline dim3 xcd_grid(int gx, int gy, int nz) { return dim3(NXCD * ((gx * gy + NXCD - 1) / NXCD), nz); }
 
 __device__ inline long long to_fix(float v) { return (long long)rint((double)v * FIX_SCALE); }
 // The same integer in three instructions instead of eight when |v| < 2^15: v * 2^36 + 1.5 * 2^52 is
