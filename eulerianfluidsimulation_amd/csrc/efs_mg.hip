@@ -92,6 +92,7 @@ struct MG {
     double* p_rz = nullptr;
     bool fused_rz = false;  // the fine up sweep writes the r . z partials
     bool tail = false;      // the last two levels run in one workgroup (k_mg_tail)
+    bool down2 = false;     // the two down sweeps of a level run in one kernel (k_mg_down2)
     int last_its = 0;       // iterations of the previous solve
 };
 
@@ -449,6 +450,162 @@ __global__ __launch_bounds__(POST_NT) void k_mg_sweep(Lvl l, Lvl cl, float inv_s
     }
 }
 
+// Both down sweeps of the V(2,2) cycle in one plane march: the work of k_mg_sweep<SW_DOWN0> and then
+// k_mg_sweep<SW_RB>, with the same operations in the same order (bit-identical), without their
+// intermediate x buffer and second fetch of b and the row codes.
+//   red0   the first sweep's red values: pointwise in b and the row code, since it starts from 0;
+//   black0 from the red0 neighbours (its old value 0);
+//   red1   from the black0 neighbours (its old value red0);
+//   black1 from the red1 neighbours (its old value black0).
+// The output x holds red1 and black1.  The tile's black1 needs red1 on a halo of 1, which needs
+// black0 on a halo of 2, which needs red0 (and so b and the row codes) on a halo of 3.  Iteration i
+// of the march commits red0 of plane i + 1 (from registers loaded two planes earlier), then computes
+// black0 of plane i, red1 of plane i - 1 and black1 of plane i - 2, with a barrier between stages;
+// each ring holds the three planes its next stage reads.  The row code and b of the tile's black
+// cells are copied aside in the red1 stage, since the three-plane ring of b and codes has moved on
+// when black1 reads them.  LDS values are masked by RC_CONTRIB as in k_mg_sweep.
+constexpr int D3X = TX + 6, D3Y = PTY + 6, D3P = D3X * D3Y;  // halo 3 (fetch, red0)
+constexpr int D2X = TX + 4, D2Y = PTY + 4, D2P = D2X * D2Y;  // halo 2 (black0, b, codes)
+constexpr int D1X = TX + 2, D1Y = PTY + 2, D1P = D1X * D1Y;  // halo 1 (red1)
+static_assert(D3P <= 2 * POST_NT && (D2X / 2) * D2Y <= POST_NT, "two fetch cells and one black0 pair per lane");
+__global__ __launch_bounds__(POST_NT) void k_mg_down2(Lvl l, float inv_s, Dom d, const CGState* st, float* __restrict__ xout) {
+    constexpr bool SOR = EFS_MG_OMEGA != 1.0f;
+    if (st->done) return;
+    __shared__ float r0[3][D3P], b0[3][D2P], r1[3][D1P], bs[3][D2P], tb[2][NT2];
+    __shared__ uint16_t rs[3][D2P], tr[2][NT2];
+    const int lane = threadIdx.x + threadIdx.y * TX;
+    int bx, by, bz;
+    if (!block_coords(d, bx, by, bz)) return;
+    const int x0 = 1 + bx * TX, y0 = 1 + by * PTY;
+    const int zb = 1 + bz * d.zc, ze = min(zb + d.zc, 1 + l.nz);  // output planes [zb, ze)
+    const int plane = l.W * l.H;
+    auto slot3 = [](int p) { return (p + 3) % 3; };  // p >= -2
+    // fetch lanes: cells k = lane and lane + POST_NT of the halo-3 plane, row-major
+    int fk2[2], fo[2], fpar[2];
+    bool fv[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int k = lane + i * POST_NT;
+        fv[i] = k < D3P;
+        const int kk = fv[i] ? k : 0;
+        const int hx = kk % D3X, hy = kk / D3X;
+        const int xu = x0 - 3 + hx, yu = y0 - 3 + hy;
+        fo[i] = min(max(yu, 0), l.H - 1) * l.W + min(max(xu, 0), l.W - 1);
+        fpar[i] = (xu + yu) & 1;
+        fk2[i] = (hx >= 1 && hx < D3X - 1 && hy >= 1 && hy < D3Y - 1) ? (hy - 1) * D2X + (hx - 1) : -1;
+    }
+    const int nb = l.W * l.H * l.D;
+    const __amdgpu_buffer_rsrc_t r_rc = __builtin_amdgcn_make_buffer_rsrc(l.rc, (short)0, nb * 2, kBufDword3);
+    const __amdgpu_buffer_rsrc_t r_b = __builtin_amdgcn_make_buffer_rsrc(l.b, (short)0, nb * 4, kBufDword3);
+    struct Fetched {
+        unsigned rc[2];
+        float b[2];
+    };
+    auto fetch = [&](int p, Fetched& f) {
+        const int zo = min(max(p, 0), l.D - 1) * plane;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            if (i == 1 && !fv[1]) continue;
+            f.rc[i] = __builtin_amdgcn_raw_buffer_load_b16(r_rc, (unsigned)fo[i] * 2u, zo * 2, 0);
+            f.b[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_b, (unsigned)fo[i] * 4u, zo * 4, 0));
+        }
+    };
+    auto commit = [&](int p, const Fetched& f) {  // red0 of plane p; b and codes of its halo-2 part
+        const int s = slot3(p);
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            if (i == 1 && !fv[1]) continue;
+            const unsigned rc = f.rc[i];
+            const float v = relax<SOR>(rc, row_update(rc, 0.f, f.b[i], inv_s), 0.f);
+            const bool red = ((fpar[i] + p) & 1) == 0;
+            r0[s][lane + i * POST_NT] = (red && (rc & RC_CONTRIB)) ? v : 0.f;
+            if (fk2[i] >= 0) {
+                bs[s][fk2[i]] = f.b[i];
+                rs[s][fk2[i]] = (uint16_t)rc;
+            }
+        }
+    };
+    auto in_level = [&](int x, int y, int p) { return x >= 1 && x <= l.nx && y >= 1 && y <= l.ny && p >= 1 && p <= l.nz; };
+    // black0 of plane p on the halo-2 region: lane pair (2 j, 2 j + 1) of row hy
+    const bool l_b0 = lane < (D2X / 2) * D2Y;
+    const int b0j = lane % (D2X / 2), b0y = min(lane / (D2X / 2), D2Y - 1);
+    auto black0 = [&](int p) {
+        if (!l_b0) return;
+        const int xu = x0 - 2 + 2 * b0j, yu = y0 - 2 + b0y;
+        const int q = ((xu + yu + p) & 1) == 1 ? 0 : 1;
+        const int k2 = b0y * D2X + 2 * b0j + q, k3 = (b0y + 1) * D3X + 2 * b0j + q + 1;
+        const float *rm = r0[slot3(p - 1)], *rc0 = r0[slot3(p)], *rp = r0[slot3(p + 1)];
+        const float sum = 0.f + rc0[k3 - 1] + rc0[k3 + 1] + rc0[k3 - D3X] + rc0[k3 + D3X] + rp[k3] + rm[k3];
+        const unsigned rc = rs[slot3(p)][k2];
+        const float val = in_level(xu + q, yu, p) ? relax<SOR>(rc, row_update(rc, sum, bs[slot3(p)][k2], inv_s), 0.f) : 0.f;
+        b0[slot3(p)][k2] = (rc & RC_CONTRIB) ? val : 0.f;
+    };
+    // red1 of plane p on the halo-1 region (stored for the tile's cells); the b and code of the
+    // tile's black cells of plane p are copied aside for black1
+    const bool l_r1 = lane < (D1X / 2) * D1Y;
+    const int r1j = lane % (D1X / 2), r1y = min(lane / (D1X / 2), D1Y - 1);
+    const bool l_t = lane < NT2;
+    const int tj = lane % (TX / 2), ty = lane / (TX / 2);
+    auto red1 = [&](int p) {
+        if (l_r1) {
+            const int xu = x0 - 1 + 2 * r1j, yu = y0 - 1 + r1y;
+            const int q = ((xu + yu + p) & 1) == 0 ? 0 : 1;
+            const int k1 = r1y * D1X + 2 * r1j + q, k2 = (r1y + 1) * D2X + 2 * r1j + q + 1;
+            const int k3 = (r1y + 2) * D3X + 2 * r1j + q + 2;
+            const float *bm = b0[slot3(p - 1)], *bc = b0[slot3(p)], *bp = b0[slot3(p + 1)];
+            const float sum = 0.f + bc[k2 - 1] + bc[k2 + 1] + bc[k2 - D2X] + bc[k2 + D2X] + bp[k2] + bm[k2];
+            const unsigned rc = rs[slot3(p)][k2];
+            const int x = xu + q;
+            const bool inside = in_level(x, yu, p);
+            const float val = inside ? relax<SOR>(rc, row_update(rc, sum, bs[slot3(p)][k2], inv_s), r0[slot3(p)][k3]) : 0.f;
+            r1[slot3(p)][k1] = (rc & RC_CONTRIB) ? val : 0.f;
+            if (inside && x < x0 + TX && yu >= y0 && yu < y0 + PTY && p >= zb && p < ze) xout[p * plane + yu * l.W + x] = val;
+        }
+        if (l_t) {
+            const int q = ((x0 + 2 * tj + y0 + ty + p) & 1) == 1 ? 0 : 1;
+            const int k2 = (ty + 2) * D2X + 2 * tj + q + 2;
+            tb[p & 1][lane] = bs[slot3(p)][k2];
+            tr[p & 1][lane] = rs[slot3(p)][k2];
+        }
+    };
+    // black1 of plane p on the tile
+    auto black1 = [&](int p) {
+        if (!l_t) return;
+        const int x = x0 + 2 * tj, y = y0 + ty;
+        const int q = ((x + y + p) & 1) == 1 ? 0 : 1;
+        const int k1 = (ty + 1) * D1X + 2 * tj + q + 1, k2 = (ty + 2) * D2X + 2 * tj + q + 2;
+        const float *rm = r1[slot3(p - 1)], *rc1 = r1[slot3(p)], *rp = r1[slot3(p + 1)];
+        const float sum = 0.f + rc1[k1 - 1] + rc1[k1 + 1] + rc1[k1 - D1X] + rc1[k1 + D1X] + rp[k1] + rm[k1];
+        const unsigned rc = tr[p & 1][lane];
+        const float val = relax<SOR>(rc, row_update(rc, sum, tb[p & 1][lane], inv_s), b0[slot3(p)][k2]);
+        if (x + q <= l.nx && y <= l.ny) xout[p * plane + y * l.W + x + q] = val;
+    };
+    // planes zb - 3 .. ze + 2 are committed; black0 runs on zb - 2 .. ze + 1, red1 on zb - 1 .. ze,
+    // black1 on zb .. ze - 1.  Two register sets alternate: plane i + 1 is committed while plane
+    // i + 2 is in flight, then plane i + 3 is loaded into the set just committed.
+    Fetched fa, fb;
+    fetch(zb - 3, fa);
+    commit(zb - 3, fa);
+    fetch(zb - 2, fb);
+    commit(zb - 2, fb);
+    fetch(zb - 1, fa);
+    fetch(zb, fb);
+    auto body = [&](int i, Fetched& f) {
+        commit(i + 1, f);
+        if (i + 3 <= ze + 2) fetch(i + 3, f);
+        lds_barrier();
+        black0(i);
+        lds_barrier();
+        if (i >= zb) red1(i - 1);
+        lds_barrier();
+        if (i >= zb + 2) black1(i - 2);
+    };
+    for (int i = zb - 2; i <= ze + 1; i += 2) {
+        body(i, fa);
+        if (i + 1 <= ze + 1) body(i + 1, fb);
+    }
+}
+
 // residual b - s A x of level f (x stored, FP32) restricted to level c: b_c = 1/8 of the sum of
 // the fluid children's residuals (child order x, y, z), 0 on non-fluid coarse cells.  A block
 // takes a 64 x 4 tile of fine cells in the two fine planes of one coarse plane: each thread
@@ -738,6 +895,8 @@ int ensure(efs_sim* s) {
     // kernels, which compute the same bits)
     const char* env = std::getenv("EFS_MG_TAIL");
     m->tail = !(env && env[0] == '0');
+    const char* env2 = std::getenv("EFS_MG_DOWN2");  // EFS_MG_DOWN2=0: the two separate sweep kernels
+    m->down2 = EFS_MG_NU == 2 && !(env2 && env2[0] == '0');
     if (m->L.size() >= 3) {
         const Lvl& t = m->L[m->L.size() - 2];
         const long long hw = (t.nx + 1) / 2;
@@ -805,9 +964,13 @@ int vcycle(efs_sim* s, void*) {
         ta = tb = nullptr;
         if (k == 0) efs_ktimer_pair(s, &ta, &tb);
         if (EFS_MG_NU == 2) {
-            hipExtLaunchKernelGGL(k_mg_sweep<SW_DOWN0>, grid, pb, 0, s->stream, ta, nullptr, 0, l, c, is, d, st, (double*)nullptr,
-                                  (const float*)nullptr, l.x2);
-            hipLaunchKernelGGL(k_mg_sweep<SW_RB>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, l.x2, l.x);
+            if (m->down2) {
+                hipExtLaunchKernelGGL(k_mg_down2, grid, pb, 0, s->stream, ta, nullptr, 0, l, is, d, st, l.x);
+            } else {
+                hipExtLaunchKernelGGL(k_mg_sweep<SW_DOWN0>, grid, pb, 0, s->stream, ta, nullptr, 0, l, c, is, d, st,
+                                      (double*)nullptr, (const float*)nullptr, l.x2);
+                hipLaunchKernelGGL(k_mg_sweep<SW_RB>, grid, pb, 0, s->stream, l, c, is, d, st, nullptr, l.x2, l.x);
+            }
             hipExtLaunchKernelGGL(k_mg_restrict_x, dim3((l.nx + MB - 1) / MB, (l.ny + MBY - 1) / MBY, c.nz), dim3(MB, MBY), 0,
                                   s->stream, nullptr, tb, 0, l, (const float*)l.x, c, st);
         } else {
