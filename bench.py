@@ -15,9 +15,11 @@ same number of steps with plain mixed-precision CG, once from the previous press
 guess) and once from x0 = b (the reference CPU backend's solve).
 
 Prints one JSON line with steps/s, CG iterations/step, the roofline of the dominant event-timed
-kernel (HIP events recorded on the launches themselves, on the library's stream), the rooflines
-of every event-timed kernel, and the CPU baseline (the oracle, one thread, on a bounded sample;
-see `cpu_baseline.sample`).
+kernel (HIP events recorded on the launches themselves, on the library's stream, in every
+--kernel-timing-every-th timed step: a timed launch is followed by a gap of several microseconds,
+which on every launch would slow the measured step rate by about 3 %), the rooflines of every
+event-timed kernel, and the CPU baseline (the oracle, one thread, on a bounded sample; see
+`cpu_baseline.sample`).
 """
 import argparse
 import ctypes
@@ -39,10 +41,9 @@ K2_BYTES_PC = 2 + 4 + 8 + 4 + 8 + 4  # preconditioned CG: the FP64 iterate is re
 # up: first sweep (row code 2 + b 4 + stored red 4 + parents' correction 4/8 read, x written 4)
 # and second sweep (row code 2 + b 4 + stored red 4 read, x written 4)
 MG_UP_BYTES = (2 + 4 + 4 + 4 / 8 + 4) + (2 + 4 + 4 + 4)
-# down: first sweep from zero (row code 2 + b 4 read, x written 4), second sweep (row code 2 +
-# b 4 + stored black 4 read, x written 4), residual restriction (row code 2 + b 4 + x 4 read,
-# coarse b written 4/8)
-MG_DOWN_BYTES = (2 + 4 + 4) + (2 + 4 + 4 + 4) + (2 + 4 + 4 + 4 / 8)
+# down: both sweeps in one kernel (row code 2 + b 4 read, x written 4), residual restriction (row
+# code 2 + b 4 + x 4 read, coarse b written 4/8)
+MG_DOWN_BYTES = (2 + 4 + 4) + (2 + 4 + 4 + 4 / 8)
 # rho_advect backward trace: u, v, w 12 + boundary class 1 + gamma 4 read; velocity sample 12,
 # advected gamma 4, corner base 4, 8 row entries 32 written; column-sum accumulator 8 read + 8
 # written (fixed-point atomics, summed per block in LDS first)
@@ -101,6 +102,8 @@ def parse():
                          "(inference forward and one training step) against the BF16 MFMA peak")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=80, help="interior resolution of the CPU sample's stages and solve")
+    ap.add_argument("--kernel-timing-every", type=int, default=4, metavar="N",
+                    help="record the kernel events in every N-th timed step (1: every step)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="before the run, one device-to-device copy of MIB MiB (a counter calibration kernel of "
                          "known traffic for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
@@ -211,7 +214,7 @@ def run(sc, params, warmup, steps, timed_params=None, kernel_timing=False, keep_
         sim.set_params(timed_params)
     lib = sim.lib
     if kernel_timing:
-        lib.efs_set_kernel_timing(sim.h, 1)
+        lib.efs_set_kernel_timing(sim.h, max(1, ARGS.kernel_timing_every))
     sim.synchronize()
     iters, true_res, rcs, fallbacks = [], [], [], []
     t0 = time.perf_counter()
@@ -435,6 +438,7 @@ def main():
         "plain_cg": plain,
         "fp64_check": main_run["fp64"],
         "kernel_events": {k: {"launches": v[0], "avg_us": 1e3 * v[1] / max(1, v[0]), "total_ms": v[1]} for k, v in kt.items()},
+        "kernel_events_every": max(1, ARGS.kernel_timing_every),
         "roofline": rooflines[dom],
         "rooflines": timed,
     }

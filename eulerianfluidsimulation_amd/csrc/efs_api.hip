@@ -545,6 +545,7 @@ int efs_step(efs_handle h, efs_stats* st) {
         st->true_rel_residual = -1;
     }
     int status = 0;
+    if (h->ktiming_every > 0) h->ktiming = (h->ktiming_steps++ % h->ktiming_every) == 0;
     stage_event(h, 0);
     const unsigned on = ~h->P.disable_mask;
     auto enabled = [&](int k) { return (on >> k) & 1u; };
@@ -636,6 +637,8 @@ int efs_set_kernel_timing(efs_handle h, int32_t on) {
         t.total_ms = 0;
     }
     h->ktiming = on != 0;
+    h->ktiming_every = on > 1 ? on : 0;
+    h->ktiming_steps = 0;
     return 0;
 }
 int efs_kernel_time(efs_handle h, int32_t k, int64_t* launches, double* total_ms) {

@@ -346,6 +346,8 @@ struct efs_sim {
     // timing
     bool timing = false;
     bool ktiming = false;
+    int ktiming_every = 0;      // > 0: kernel timing in every ktiming_every-th efs_step (efs_set_kernel_timing)
+    long long ktiming_steps = 0;  // efs_step calls since kernel timing was set
     TimerSlot ktimers[9];
     hipEvent_t ev_stage[8] = {};
     // learned preconditioner state (owned by the preconditioner module)

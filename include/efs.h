@@ -171,8 +171,11 @@ void* efs_get_stream(efs_handle h);
 int efs_synchronize(efs_handle h);
 /* 1 = record device time per stage into efs_stats.stage_ms (adds event pairs per stage). */
 int efs_set_timing(efs_handle h, int32_t on);
-/* Per-kernel event timing of the dominant kernels: 1 = record start and end events on each launch
- * of the tracked kernels; efs_kernel_time reads the accumulated count and milliseconds of kernel
+/* Per-kernel event timing of the dominant kernels: on = 1 records start and end events on each
+ * launch of the tracked kernels; on = n > 1 records them in every n-th efs_step only (the first, then
+ * every n-th after it; a timed launch is followed by a gap of several microseconds, so sampling
+ * keeps the step rate of the untimed steps); 0 stops.  Every call clears what was recorded.
+ * efs_kernel_time reads the accumulated count and milliseconds of kernel
  * 0 (CG direction + SpMV), 1 (CG update), 2 (Jacobi), 3 (multigrid fine up sweep), 4 (multigrid
  * fine down sweep with restriction), 5 (rho_advect backward trace), 6 (rho_advect forward trace),
  * 7 (project's wall enforcement + divergence) or 8 (project's gradient + wall enforcement). */
