@@ -416,7 +416,7 @@ int efs_cg_fold_all(efs_sim* s);                    // x64 += xd, xd = 0
 int efs_cg_finalize(efs_sim* s, bool xd_zero = false);  // x64 += xd, pressure = (float)x64
 int efs_jacobi_solve(efs_sim* s, efs_stats* st);
 int efs_launch_spmv(efs_sim* s, const float* x, float* y);
-int efs_true_residual(efs_sim* s, double* num, double* den);
+int efs_true_residual(efs_sim* s, double* num, double* den, bool write_p = false);
 
 // 2D step (efs_2d.hip)
 int efs2_alloc(efs_sim* s);

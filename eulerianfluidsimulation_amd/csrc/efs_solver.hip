@@ -531,11 +531,13 @@ __global__ void k_cg_fold_all(int n, double* __restrict__ x64, float* __restrict
 
 // FP64 true residual ||b - A x||^2 and ||b||^2 (x in FP64 if given, else the FP32 field): block
 // partials in partials[block] and partials[gridDim + block], plain stores (no fence), summed in a
-// fixed order by k_true_residual_sum.
+// fixed order by k_true_residual_sum.  With pout (and x in FP64) each visited cell also gets its
+// FP32 pressure, the rounding of the iterate (the work of k_cg_finalize after a preconditioned
+// solve, done in the same pass over the iterate; the launch domain is the whole grid).
 constexpr int TR_NT = 256;
 __global__ __launch_bounds__(TR_NT) void k_true_residual(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
                                                          const float* __restrict__ xf, const double* __restrict__ xdbl,
-                                                         double* partials) {
+                                                         double* partials, float* __restrict__ pout) {
     int bx, by, bz;
     double v[2] = {0.0, 0.0};
     if (block_coords(d, bx, by, bz)) {
@@ -554,6 +556,7 @@ __global__ __launch_bounds__(TR_NT) void k_true_residual(Dom d, const uint16_t* 
                 const double rr = (double)b[c] - t;
                 v[0] += rr * rr;
                 v[1] += (double)b[c] * (double)b[c];
+                if (pout) pout[c] = (float)xc;
                 xzm = xc;
                 xc = xzp;
             }
@@ -771,12 +774,13 @@ int efs_launch_spmv(efs_sim* s, const float* x, float* y) {
     return 0;
 }
 
-int efs_true_residual(efs_sim* s, double* num, double* den) {
+int efs_true_residual(efs_sim* s, double* num, double* den, bool write_p) {
     const Dom d = dom_simple(s, true);
     static_assert(TX * 4 == TR_NT, "dom_simple tiles are TX x 4");
     const int nb = launch_blocks(d);
+    write_p = write_p && s->p64_valid;
     hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(TX, d.ty), 0, s->stream, d, s->rowcode, s->div, s->p,
-                       s->p64_valid ? s->cg_x64 : nullptr, s->partials);
+                       s->p64_valid ? s->cg_x64 : nullptr, s->partials, write_p ? s->p : nullptr);
     hipLaunchKernelGGL(k_true_residual_sum, dim3(1), dim3(TR_NT), 0, s->stream, nb, s->partials, s->st);
     EFS_HIP(hipGetLastError());
     EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
@@ -937,8 +941,15 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
     size_t repl = (size_t)(fin.iter + 63) / 64;  // replacement iterations have no K2
     if (pc && repl > 0) repl--;                   // (preconditioned CG: not i = 0)
     efs_trim_timer(s->ktimers[KT_CG_UPDATE], kt1 + (size_t)fin.iter - std::min<size_t>(repl, fin.iter));
-    // preconditioned CG updates the FP64 iterate directly and folds clear xd: it is 0 here
-    if ((rc = efs_cg_finalize(s, pc != nullptr))) return rc;
+    // preconditioned CG updates the FP64 iterate directly and folds clear xd: it is 0 here, and
+    // with the true residual requested the FP32 pressure is written by that pass
+    const bool fused_p = pc != nullptr && s->P.true_residual && out;
+    if (fused_p) {
+        s->p64_valid = true;
+        s->x64_has_previous = true;
+    } else if ((rc = efs_cg_finalize(s, pc != nullptr))) {
+        return rc;
+    }
     if (out) {
         out->iterations = fin.iter;
         out->rr = fin.gamma;
@@ -948,7 +959,7 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
     }
     if (s->P.true_residual && out) {
         double num = 0, den = 0;
-        rc = efs_true_residual(s, &num, &den);
+        rc = efs_true_residual(s, &num, &den, fused_p);
         if (rc) return rc;
         out->true_rel_residual = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
     }
