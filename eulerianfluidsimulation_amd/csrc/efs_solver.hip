@@ -47,44 +47,69 @@ namespace {
 // contributing = counted and rho > 0.5; CG row = !(rho <= 0.5 || boundary == 1);
 // Jacobi row = !(rho <= 0.5 || boundary >= 0.5).
 // ---------------------------------------------------------------------------------------
+// A thread marches up RCZ planes of one (x, y) column: the rho and boundary byte of the cell and
+// of its -z neighbour come from the planes before, and the loads of the next plane are issued before
+// the current plane is computed (clamped into the array; unused values are ignored).  The blocks
+// take (tile, z chunk) in XCD order.
+constexpr int RCZ = 16;
 __global__ __launch_bounds__(256) void k_rowcode(DevGrid g, uint16_t* rc, CGState* st) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    XcdBlock blk;
+    if (!xcd_block((g.W + 63) / 64, (g.H + 3) / 4, blk)) return;
+    const int x = blk.bx * 64 + threadIdx.x, y = blk.by * 4 + threadIdx.y;
     if (x >= g.W || y >= g.H) return;
-    const int c = cidx(g, x, y, z);
-    const int sy = g.W, sz = g.W * g.H;
-    // neighbour indices clamped into the array; the in-range test below discards the clamped ones
-    const bool ok[6] = {x > 0, x < g.W - 1, y > 0, y < g.H - 1, z < g.D - 1, z > 0};
-    const int q[6] = {ok[0] ? c - 1 : c, ok[1] ? c + 1 : c, ok[2] ? c - sy : c, ok[3] ? c + sy : c, ok[4] ? c + sz : c,
-                      ok[5] ? c - sz : c};
-    // a neighbour on the outer layer is never counted (CGSolver.cpp:477-482): its displaced
-    // coordinate is on the layer, or the cell itself is on it along another axis
-    const bool ox = x == 0 || x == g.W - 1, oy = y == 0 || y == g.H - 1, oz = z == 0 || z == g.D - 1;
-    const bool outer[6] = {x - 1 == 0 || oy || oz, x + 1 == g.W - 1 || oy || oz, y - 1 == 0 || ox || oz,
-                           y + 1 == g.H - 1 || ox || oz, z + 1 == g.D - 1 || ox || oy, z - 1 == 0 || ox || oy};
+    const int zb = blk.bz * RCZ, ze = min(zb + RCZ, g.D);
+    const int sy = g.W, sz = g.W * g.H, last = g.N - 1;
+    const bool ox = x == 0 || x == g.W - 1, oy = y == 0 || y == g.H - 1;
+    const int dxm = x > 0 ? -1 : 0, dxp = x < g.W - 1 ? 1 : 0, dym = y > 0 ? -sy : 0, dyp = y < g.H - 1 ? sy : 0;
     const unsigned bits[6] = {RC_NB_XM, RC_NB_XP, RC_NB_YM, RC_NB_YP, RC_NB_ZP, RC_NB_ZM};
-    unsigned bq[6];
-    float rq[6];
+    struct Plane {
+        float r[5];     // the cell's +z neighbour, then x-, x+, y-, y+ of the cell
+        unsigned b[5];
+    };
+    auto load = [&](int z, Plane& q) {  // the loads of plane z (clamped into the array)
+        const int c = min(cidx(g, x, y, min(z, g.D - 1)), last);
+        const int o[5] = {min(c + sz, last), c + dxm, c + dxp, c + dym, c + dyp};
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        bq[k] = g.bf[q[k]];
-        rq[k] = g.rho[q[k]];
-    }
-    const float r = g.rho[c];
-    const unsigned b = g.bf[c];
-    unsigned code = 0, cnt = 0;
+        for (int k = 0; k < 5; k++) {
+            q.r[k] = g.rho[o[k]];
+            q.b[k] = g.bf[o[k]];
+        }
+    };
+    const int c0 = cidx(g, x, y, zb);
+    float r = g.rho[c0], rzm = zb > 0 ? g.rho[c0 - sz] : 0.f;
+    unsigned b = g.bf[c0], bzm = zb > 0 ? (unsigned)g.bf[c0 - sz] : 0u;
+    Plane q, qn;
+    load(zb, q);
+    for (int z = zb; z < ze; z++) {
+        load(z + 1, qn);
+        // neighbour order x-, x+, y-, y+, z+, z- and the tests of CGSolver.cpp:477-490
+        const bool oz = z == 0 || z == g.D - 1;
+        const bool ok[6] = {x > 0, x < g.W - 1, y > 0, y < g.H - 1, z < g.D - 1, z > 0};
+        // a neighbour on the outer layer is never counted: its displaced coordinate is on the
+        // layer, or the cell itself is on it along another axis
+        const bool outer[6] = {x - 1 == 0 || oy || oz, x + 1 == g.W - 1 || oy || oz, y - 1 == 0 || ox || oz,
+                               y + 1 == g.H - 1 || ox || oz, z + 1 == g.D - 1 || ox || oy, z - 1 == 0 || ox || oy};
+        const float rq[6] = {q.r[1], q.r[2], q.r[3], q.r[4], q.r[0], rzm};
+        const unsigned bq[6] = {q.b[1], q.b[2], q.b[3], q.b[4], q.b[0], bzm};
+        unsigned code = 0, cnt = 0;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const bool counted = ok[k] && !outer[k] && !(bq[k] & BF_GT05);
-        cnt += counted ? 1u : 0u;
-        if (counted && rq[k] > 0.5f) code |= bits[k];
+        for (int k = 0; k < 6; k++) {
+            const bool counted = ok[k] && !outer[k] && !(bq[k] & BF_GT05);
+            cnt += counted ? 1u : 0u;
+            if (counted && rq[k] > 0.5f) code |= bits[k];
+        }
+        code |= cnt << RC_CNT_SHIFT;
+        if (!(ox || oy || oz) && !(b & BF_GT05) && r > 0.5f) code |= RC_CONTRIB;
+        if (!(r <= 0.5f || (b & BF_EQ1))) code |= RC_CG_ROW;
+        if (!(r <= 0.5f || (b & BF_GE05))) code |= RC_JAC_ROW;
+        rc[cidx(g, x, y, z)] = (uint16_t)code;
+        if ((ox || oy || oz) && (code & (RC_CG_ROW | RC_JAC_ROW))) atomicOr(&st->flags, 1);
+        rzm = r;
+        bzm = b;
+        r = q.r[0];
+        b = q.b[0];
+        q = qn;
     }
-    code |= cnt << RC_CNT_SHIFT;
-    if (!(ox || oy || oz) && !(b & BF_GT05) && r > 0.5f) code |= RC_CONTRIB;
-    if (!(r <= 0.5f || (b & BF_EQ1))) code |= RC_CG_ROW;
-    if (!(r <= 0.5f || (b & BF_GE05))) code |= RC_JAC_ROW;
-    rc[c] = (uint16_t)code;
-    const bool border = x == 0 || y == 0 || z == 0 || x == g.W - 1 || y == g.H - 1 || z == g.D - 1;
-    if (border && (code & (RC_CG_ROW | RC_JAC_ROW))) atomicOr(&st->flags, 1);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -721,8 +746,8 @@ int efs_launch_rowcode(efs_sim* s, int* border_rows) {
     CGState init{};
     std::memcpy(s->st_host, &init, sizeof(CGState));
     EFS_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(CGState), hipMemcpyHostToDevice, s->stream));
-    dim3 blk(64, 4, 1), grd((g.W + 63) / 64, (g.H + 3) / 4, g.D);
-    hipLaunchKernelGGL(k_rowcode, grd, blk, 0, s->stream, g, s->rowcode, s->st);
+    hipLaunchKernelGGL(k_rowcode, xcd_grid((g.W + 63) / 64, (g.H + 3) / 4, (g.D + RCZ - 1) / RCZ), dim3(64, 4, 1), 0, s->stream, g,
+                       s->rowcode, s->st);
     EFS_HIP(hipGetLastError());
     EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
     EFS_HIP(hipStreamSynchronize(s->stream));
