@@ -103,7 +103,9 @@ __global__ __launch_bounds__(256) void k_decay(DevGrid g, float decay) {
 // before k", so the result does not depend on the order of the threads; a cell that is not on
 // the front costs one byte read instead of a copy.
 // With DECAY the velocity decay of the step (k_decay) is applied in the same pass: it reads only
-// rho, which the stamps read too.
+// rho, which the stamps read too.  A solid cell (boundary == 1) is stamped 254: it never becomes
+// valid (only 255 does) and never counts as a valid neighbour (254 is not below any pass), so the
+// passes need not read the boundary.
 template <bool DECAY>
 __global__ __launch_bounds__(256) void k_extrap_init(DevGrid g, uint8_t* __restrict__ stamp, float decay) {
     FULL_XYZ(g);
@@ -114,35 +116,60 @@ __global__ __launch_bounds__(256) void k_extrap_init(DevGrid g, uint8_t* __restr
         g.v[c] = g.v[c] * decay;
         g.w[c] = g.w[c] * decay;
     }
-    stamp[c] = (interior && r >= 0.5f && !(g.bf[c] & BF_EQ1)) ? 0 : 255;
+    const bool solid = (g.bf[c] & BF_EQ1) != 0;
+    stamp[c] = solid ? 254 : ((interior && r >= 0.5f) ? 0 : 255);
 }
 
-// The stamps and the boundary byte are loaded together, unconditionally (one memory round trip
-// for a cell off the front, instead of one per test).
+// A thread takes EXZ interior planes of one (x, y) column and loads every stamp they read at
+// once (the four lateral neighbours of each plane and the column from one plane below to one
+// above, all addresses in the array), so a chunk costs one memory round trip; then it tests the
+// planes in order.  The column's own stamps are carried as this pass leaves them (a neighbour
+// stamped in this pass counts as not valid either way, 255 or the pass, so values loaded before
+// another thread stamps them are just as good).  The blocks take (tile, z chunk) in XCD order, so
+// neighbouring tiles' rows are read through the same L2.
+constexpr int EXZ = 8;
 __global__ __launch_bounds__(256) void k_extrap_pass(DevGrid g, uint8_t* stamp, int pass) {
-    INTERIOR_XYZ(g);
+    XcdBlock blk;
+    if (!xcd_block((g.W - 2 + BX - 1) / BX, (g.H - 2 + BY - 1) / BY, blk)) return;
+    const int x = 1 + blk.bx * BX + threadIdx.x, y = 1 + blk.by * BY + threadIdx.y;
+    if (x > g.W - 2 || y > g.H - 2) return;
+    const int zb = 1 + blk.bz * EXZ, nz = min(EXZ, g.D - 1 - zb);  // planes [zb, zb + nz)
     const int sy = g.W, sz = g.W * g.H;
-    const int nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};  // the reference's order
-    const unsigned own = stamp[c], b = g.bf[c];
-    unsigned st[6];
+    const int c0 = cidx(g, x, y, zb);
+    unsigned lat[EXZ][4], col[EXZ + 2];  // col[k]: the column's stamp at plane zb - 1 + k
 #pragma unroll
-    for (int k = 0; k < 6; k++) st[k] = stamp[nb[k]];
-    unsigned m = 0;
+    for (int k = 0; k < EXZ + 2; k++) col[k] = stamp[c0 + (min(k, nz + 1) - 1) * sz];
 #pragma unroll
-    for (int k = 0; k < 6; k++) m |= ((int)st[k] < pass ? 1u : 0u) << k;
-    // one combined test (short-circuit tests would let the compiler sink the loads below them)
-    const unsigned front = (unsigned)(own == 255) & (unsigned)((b & BF_EQ1) == 0) & (unsigned)(m != 0);
-    if (!front) return;
-    float ax = 0.f, ay = 0.f, az = 0.f, n = 0;
+    for (int k = 0; k < EXZ; k++) {
+        const int c = c0 + min(k, nz - 1) * sz;
+        lat[k][0] = stamp[c - 1];
+        lat[k][1] = stamp[c + 1];
+        lat[k][2] = stamp[c - sy];
+        lat[k][3] = stamp[c + sy];
+    }
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        if (m & (1u << k)) {
-            ax = ax + g.u[nb[k]]; ay = ay + g.v[nb[k]]; az = az + g.w[nb[k]];
-            n++;
+    for (int k = 0; k < EXZ; k++) {
+        if (k >= nz) break;
+        const int c = c0 + k * sz;
+        const unsigned st[6] = {lat[k][0], lat[k][1], lat[k][2], lat[k][3], col[k], col[k + 2]};  // the reference's order
+        unsigned m = 0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) m |= ((int)st[j] < pass ? 1u : 0u) << j;
+        if (col[k + 1] == 255 && m != 0) {
+            const int nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
+            float ax = 0.f, ay = 0.f, az = 0.f, n = 0;
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                if (m & (1u << j)) {
+                    ax = ax + g.u[nb[j]]; ay = ay + g.v[nb[j]]; az = az + g.w[nb[j]];
+                    n++;
+                }
+            }
+            g.u[c] = ax / n; g.v[c] = ay / n; g.w[c] = az / n;
+            stamp[c] = (uint8_t)pass;
+            col[k + 1] = pass;
         }
     }
-    g.u[c] = ax / n; g.v[c] = ay / n; g.w[c] = az / n;
-    stamp[c] = (uint8_t)pass;
 }
 
 // ---- rho_sharpen (CPUSimulator3D.cpp:366-456) ----------------------------------------------
@@ -251,13 +278,40 @@ __device__ inline void scatter_fix(const DevGrid& g, float px, float py, float p
     for (int k = 0; k < 8; k++) fix_add(acc, q[k], value * w[k] * inv);
 }
 
+// The cells that scatter (|delta| >= 1e-6) are the few near the free surface, spread over many
+// waves, and a wave runs the whole trace loop when any of its lanes has one.  So each BX x BY block
+// first packs its scattering cells into its lowest lanes through LDS; waves left without a cell
+// exit.  The scatters are fixed-point sums, so the order in which lanes land does not matter.
 __global__ __launch_bounds__(256) void k_sharpen_scatter(DevGrid g, const float* __restrict__ del, float rhoD,
                                                          unsigned long long* acc) {
-    INTERIOR_XYZ(g);
-    const float dr = del[c];
-    if (fabsf(dr) < 1e-6f) return;
-    const float3 p = trace_along_field(g, (float)x, (float)y, (float)z, rhoD, 10);
-    scatter_fix(g, p.x, p.y, p.z, -dr, acc);
+    __shared__ unsigned short s_src[BX * BY];
+    __shared__ float s_dr[BX * BY];
+    __shared__ int s_n;
+    const int t = threadIdx.x + BX * threadIdx.y, lane = t & 63;
+    if (t == 0) s_n = 0;
+    __syncthreads();
+    const int x = 1 + blockIdx.x * BX + threadIdx.x, y = 1 + blockIdx.y * BY + threadIdx.y, z = 1 + blockIdx.z;
+    bool act = false;
+    float dr = 0.f;
+    if (x <= g.W - 2 && y <= g.H - 2) {
+        dr = del[cidx(g, x, y, z)];
+        act = !(fabsf(dr) < 1e-6f);
+    }
+    const unsigned long long bal = __ballot(act);
+    int base = 0;
+    if (lane == 0 && bal) base = atomicAdd(&s_n, __popcll(bal));
+    base = __shfl(base, 0);
+    if (act) {
+        const int slot = base + __popcll(bal & ((1ull << lane) - 1ull));
+        s_src[slot] = (unsigned short)t;
+        s_dr[slot] = dr;
+    }
+    __syncthreads();
+    if (t >= s_n) return;
+    const int src = s_src[t];
+    const int sx = 1 + blockIdx.x * BX + src % BX, sy = 1 + blockIdx.y * BY + src / BX;
+    const float3 p = trace_along_field(g, (float)sx, (float)sy, (float)z, rhoD, 10);
+    scatter_fix(g, p.x, p.y, p.z, -s_dr[t], acc);
 }
 
 __global__ void k_add_fix(int n, float* __restrict__ f, unsigned long long* __restrict__ acc) {
@@ -1033,7 +1087,8 @@ int efs_launch_extrapolate(efs_sim* s, bool with_decay) {
     else
         hipLaunchKernelGGL(k_extrap_init<false>, full_grid(s), blk, 0, s->stream, g, s->valA, 1.f);
     for (int pass = 1; pass <= 6; pass++)  // the reference's 3 double passes
-        hipLaunchKernelGGL(k_extrap_pass, grd, blk, 0, s->stream, g, s->valA, pass);
+        hipLaunchKernelGGL(k_extrap_pass, xcd_grid((s->W - 2 + BX - 1) / BX, (s->H - 2 + BY - 1) / BY, (s->D - 2 + EXZ - 1) / EXZ),
+                           blk, 0, s->stream, g, s->valA, pass);
     EFS_HIP(hipGetLastError());
     return 0;
 }
