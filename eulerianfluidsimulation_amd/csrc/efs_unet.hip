@@ -40,60 +40,91 @@ constexpr int MAXNT = 8;  // up to 128 output channels per wavefront (weight gra
 // tiles.  Per 16-channel input chunk the block stages the tile's input halo (HT_X x HT_Y x HT_Z
 // voxels x 16 channels, 23 KB) in LDS once and reuses it for all 27 taps: the GEMM's K runs over
 // (tap, channel) in 14 steps of 32 (two taps of 16 channels; the 28th tap has zero weights).  Wave
-// w owns output rows y0 + 4 (w & 1) + r (r = 0..3) of plane z0 + (w >> 1): four 16-voxel M tiles.
-// Operand layout of the 16x16x32 BF16 MFMA: lane l supplies A row l % 16 (a voxel) and B column
-// l % 16 (an output channel) for K elements 8 (l / 16) .. 8 (l / 16) + 7, here 8 consecutive
-// channels of one tap, a 16-byte LDS (A) or global (B) load; lane l receives D rows
-// 4 (l / 16) + i, i = 0..3, of column l % 16.
+// w owns output rows y0 + 4 (w & 1) + r (r = 0..3) of plane z0 + (w >> 1): four 16-voxel tiles.
+// Operand layout of the 16x16x32 BF16 MFMA: lane l supplies A row l % 16 and B column l % 16 for K
+// elements 8 (l / 16) .. 8 (l / 16) + 7 and receives D rows 4 (l / 16) + i, i = 0..3, of column
+// l % 16.  A holds the weights (row = output channel, a 16-byte global load) and B the voxels
+// (column = voxel, 8 consecutive channels of one tap, a 16-byte LDS load), so D is output channel x
+// voxel: lane l ends with 4 consecutive output channels of voxel l % 16, which it writes as one
+// 8-byte (BF16) or 16-byte (FP32) store.
 // The halo is held as two planes of 16-byte halves, channels 0-7 of every voxel, then channels
 // 8-15: the 16 lanes that read one half for 16 consecutive voxels then read 256 contiguous bytes,
 // which spreads them over all LDS banks (voxel-major, 32 bytes apart, every bank took two).
-// With PRE a chunk's B operands for all 14 K steps are loaded into registers before the halo is
+// Global loads go through raw buffer resources with 32-bit byte offsets (tensor sizes are checked
+// on the host to stay below 2^31 bytes); a load at offset OOB returns zeros, which reads the halo's
+// padding and the 28th tap's weights without branches.
+// With PRE a chunk's A operands for all 14 K steps are loaded into registers before the halo is
 // staged, so the weight loads overlap the staging and the MFMA chain never waits on them: that
-// pays on the coarse levels, whose few blocks leave the latency exposed (16^3: 23 -> 15 us), and
-// costs occupancy where thousands of blocks hide it (128^3: 79 -> 102 us), so PRE is chosen per
-// launch.
+// pays on the coarse levels, whose few blocks leave the latency exposed, and costs occupancy where
+// thousands of blocks hide it, so PRE is chosen per launch.
 constexpr int CT_X = 16, CT_Y = 8, CT_Z = 2;
 constexpr int HT_X = CT_X + 2, HT_Y = CT_Y + 2, HT_Z = CT_Z + 2, HT_N = HT_X * HT_Y * HT_Z;
 constexpr int KSTEPS = 14;  // 27 taps x 16 channels in steps of 32, the 28th tap zero
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
-// 8 channels starting at p as 8 BF16 values in a uint4: BF16 sources are copied, FP32 sources
-// (the network input of the C entry point) are rounded to nearest even.
-__device__ inline uint4 load8(const uint16_t* p) { return *reinterpret_cast<const uint4*>(p); }
-__device__ inline uint4 load8(const float* p) {
-    const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
-    auto pk = [](float lo, float hi) { return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16); };
-    return make_uint4(pk(a.x, a.y), pk(a.z, a.w), pk(b.x, b.y), pk(b.z, b.w));
+constexpr unsigned OOB = 0x80000000u;  // a byte offset past every buffer the kernels address
+
+// raw buffer resource over `bytes` bytes at p
+__device__ inline __amdgpu_buffer_rsrc_t bufres(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kBufDword3);
+}
+
+// two FP32 values as a packed BF16 pair (lo in bits 0-15), rounded to nearest even
+__device__ inline uint32_t pk_bf16(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
+}
+
+// 8 channels at byte offset `off` of a buffer as 8 BF16 values in a uint4: BF16 sources are copied,
+// FP32 sources (the network input of the C entry point) are rounded to nearest even; zeros at OOB.
+__device__ inline uint4 load8(__amdgpu_buffer_rsrc_t r, unsigned off, const uint16_t*) {
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ inline uint4 load8(__amdgpu_buffer_rsrc_t r, unsigned off, const float*) {
+    const f32x4 a = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    const f32x4 b = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 16, 0));
+    return make_uint4(pk_bf16(a.x, a.y), pk_bf16(a.z, a.w), pk_bf16(b.x, b.y), pk_bf16(b.z, b.w));
 }
 
 // Stages the halo of the block's tile for channels [c0, c0 + 16) of a source with Csrc channels
 // (zero outside the D x H x W volume) into tile[half * HT_N + voxel].  With `up`, the source is at
-// half resolution and each fine voxel reads its parent (nearest upsampling).
+// half resolution and each fine voxel reads its parent (nearest upsampling).  Thread t stages
+// channel half t & 1 of voxels t / 2 + 128 k and issues all its loads before its first LDS write.
 template <typename TS>
 __device__ inline void stage_halo(uint4* tile, const TS* __restrict__ src, int Csrc, int c0, int x0, int y0, int z0, int D,
                                   int H, int W, bool up) {
-    const int sH = up ? H >> 1 : H, sW = up ? W >> 1 : W, sh = up ? 1 : 0;
-    for (int i = threadIdx.x; i < HT_N * 2; i += 256) {
-        const int v = i >> 1, half = i & 1;
+    const int sh = up ? 1 : 0, sD = D >> sh, sH = H >> sh, sW = W >> sh;
+    const __amdgpu_buffer_rsrc_t r = bufres(src, (unsigned)sD * sH * sW * Csrc * (unsigned)sizeof(TS));
+    const int half = threadIdx.x & 1;
+    constexpr int NI = (HT_N * 2 + 255) / 256;
+    uint4 val[NI];
+#pragma unroll
+    for (int k = 0; k < NI; k++) {
+        const int v = (threadIdx.x >> 1) + 128 * k;
         const int hx = v % HT_X, hy = (v / HT_X) % HT_Y, hz = v / (HT_X * HT_Y);
         const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-        uint4 val = make_uint4(0u, 0u, 0u, 0u);
-        if (gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D)
-            val = load8(src + (((size_t)(gz >> sh) * sH + (gy >> sh)) * sW + (gx >> sh)) * Csrc + c0 + half * 8);
-        tile[half * HT_N + v] = val;
+        const bool inside = v < HT_N && gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D;
+        const unsigned vox = (unsigned)(((gz >> sh) * sH + (gy >> sh)) * sW + (gx >> sh));
+        val[k] = load8(r, inside ? (vox * Csrc + c0 + half * 8) * (unsigned)sizeof(TS) : OOB, src);
+    }
+#pragma unroll
+    for (int k = 0; k < NI; k++) {
+        const int v = (threadIdx.x >> 1) + 128 * k;
+        if (v < HT_N) tile[half * HT_N + v] = val[k];
     }
 }
 
 // out[v][co] = bias[co] + sum_tap sum_ci in[v + off(tap)][ci] * w[tap][co][ci], then ReLU (relu),
-// written as FP32 (out_f) and / or BF16 (out_b).  With `up` (Cup channels at half resolution) the
-// input is the decoder's concatenation, channels [0, Cup) the upsampled `up` and the rest `in`
-// (Cin - Cup channels), read in place.  blockIdx.z = z tile * ngroups + output-channel group; a
-// group covers NT 16-channel tiles (tiles past Cout are skipped).  Cin, Cup, Cout: multiples of 16;
-// D, H, W: any (partial tiles are masked; with `up`, even).  `in` is BF16, or FP32 rounded to BF16
-// as it is staged (TIN = float).  With out_pool (D, H, W even) the block also writes the 2x2x2
-// average of its BF16 outputs, the next encoder level's input.
+// written as FP32 (out_f) or BF16 (out_b).  With `up` (Cup channels at half resolution) the input
+// is the decoder's concatenation, channels [0, Cup) the upsampled `up` and the rest `in` (Cin - Cup
+// channels), read in place.  blockIdx.z = z tile * ngroups + output-channel group; a group covers
+// NT 16-channel tiles (tiles past Cout are skipped).  Cin, Cup, Cout: multiples of 16; D, H, W: any
+// (partial tiles are masked; with `up`, even).  `in` is BF16, or FP32 rounded to BF16 as it is
+// staged (TIN = float).  With out_pool (D, H, W even) the block also writes the 2x2x2 average of
+// its BF16 outputs, the next encoder level's input.
 template <int NT, bool PRE, typename TIN>
 __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, int Cup, const TIN* __restrict__ in, int Cin,
                                                const uint16_t* __restrict__ w, const float* __restrict__ bias, int Cout, int D,
@@ -118,36 +149,43 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, 
         const int tc = tap < 27 ? tap : 26;  // the padding tap reads valid LDS; its weights are 0
         hoff[s] = (g & 1) * HT_N + ((rz + tc / 9) * HT_Y + (ry + (tc / 3) % 3)) * HT_X + (m + tc % 3);
     }
-    // the B operand of K step s, output tile n, input chunk ch
-    auto load_b = [&](int s, int n, int ch) -> bf16x8 {
-        const int tap = 2 * s + (g >> 1), co = (nt0 + n) * 16 + m;
-        uint4 bv = make_uint4(0u, 0u, 0u, 0u);
-        if (tap < 27 && co < Cout)
-            bv = *reinterpret_cast<const uint4*>(w + ((size_t)tap * Cout + co) * Cin + ch * 16 + (g & 1) * 8);
-        return __builtin_bit_cast(bf16x8, bv);
+    // the lane's weights: output channel (nt0 + n) 16 + m, tap 2 s + (g >> 1), input channels
+    // 16 ch + 8 (g & 1) + 0..7; byte offset woff[n] + s tapstep + 32 ch
+    const __amdgpu_buffer_rsrc_t rw = bufres(w, 27u * Cout * Cin * 2u);
+    const unsigned tapstep = 4u * Cout * Cin;  // two taps of BF16 weights
+    unsigned woff[NT];
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        const int co = (nt0 + n) * 16 + m;
+        woff[n] = co < Cout ? (((unsigned)(g >> 1) * Cout + co) * Cin + (g & 1) * 8) * 2u : OOB;
+    }
+    // the A operand of K step s, output tile n, input chunk ch (tap 27 reads zeros)
+    auto load_a = [&](int s, int n, int ch) -> bf16x8 {
+        const unsigned off = (s == KSTEPS - 1 && (g >> 1)) ? OOB : woff[n];
+        return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rw, off, (int)(s * tapstep + ch * 32u), 0));
     };
-    auto mfma_step = [&](int s, const bf16x8 (&b)[NT]) {
+    auto mfma_step = [&](int s, const bf16x8 (&a)[NT]) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const bf16x8 a = __builtin_bit_cast(bf16x8, tile[hoff[s] + r * HT_X]);
+            const bf16x8 b = __builtin_bit_cast(bf16x8, tile[hoff[s] + r * HT_X]);
 #pragma unroll
-            for (int n = 0; n < NT; n++) acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[n], acc[r][n], 0, 0, 0);
+            for (int n = 0; n < NT; n++) acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[n], b, acc[r][n], 0, 0, 0);
         }
     };
     const int nchunks = Cin >> 4;
     for (int ch = 0; ch < nchunks; ch++) {
         if constexpr (PRE) {
-            bf16x8 b[KSTEPS][NT];
+            bf16x8 a[KSTEPS][NT];
 #pragma unroll
             for (int s = 0; s < KSTEPS; s++)
 #pragma unroll
-                for (int n = 0; n < NT; n++) b[s][n] = load_b(s, n, ch);
+                for (int n = 0; n < NT; n++) a[s][n] = load_a(s, n, ch);
             if (ch) __syncthreads();
             if (up && ch * 16 < Cup) stage_halo(tile, up, Cup, ch * 16, x0, y0, z0, D, H, W, true);
             else stage_halo(tile, in, Cin - Cup, ch * 16 - Cup, x0, y0, z0, D, H, W, false);
             __syncthreads();
 #pragma unroll
-            for (int s = 0; s < KSTEPS; s++) mfma_step(s, b[s]);
+            for (int s = 0; s < KSTEPS; s++) mfma_step(s, a[s]);
         } else {
             if (ch) __syncthreads();
             if (up && ch * 16 < Cup) stage_halo(tile, up, Cup, ch * 16, x0, y0, z0, D, H, W, true);
@@ -155,73 +193,83 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, 
             __syncthreads();
 #pragma unroll 2
             for (int s = 0; s < KSTEPS; s++) {
-                bf16x8 b[NT];
+                bf16x8 a[NT];
 #pragma unroll
-                for (int n = 0; n < NT; n++) b[n] = load_b(s, n, ch);
-                mfma_step(s, b);
+                for (int n = 0; n < NT; n++) a[n] = load_a(s, n, ch);
+                mfma_step(s, a);
             }
         }
     }
-    const int z = z0 + rz;
-    float bv[NT];
+    // lane (m, g) holds, for row r and tile n, output channels (nt0 + n) 16 + 4 g + i (i = 0..3) of
+    // voxel (x0 + m, y0 + ry + r, z0 + rz)
+    const int x = x0 + m, z = z0 + rz;
+    float bv[NT][4];
 #pragma unroll
     for (int n = 0; n < NT; n++) {
-        const int co = (nt0 + n) * 16 + m;
-        bv[n] = bias && co < Cout ? bias[co] : 0.f;
+        const int co = (nt0 + n) * 16 + 4 * g;
+#pragma unroll
+        for (int i = 0; i < 4; i++) bv[n][i] = bias && co < Cout ? bias[co + i] : 0.f;
     }
-    auto value = [&](int n, int r, int e) {  // the output before rounding
-        const float v = acc[r][n][e] + bv[n];
+    auto value = [&](int n, int r, int i) {  // the output before rounding
+        const float v = acc[r][n][i] + bv[n][i];
         return relu ? (v > 0.f ? v : 0.f) : v;
     };
+    const size_t row0 = (((size_t)z * H + y0 + ry) * W + x) * Cout, rowstride = (size_t)W * Cout;
 #pragma unroll
     for (int n = 0; n < NT; n++) {
-        const int co = (nt0 + n) * 16 + m;
+        const int co = (nt0 + n) * 16 + 4 * g;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int y = y0 + ry + r;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int x = x0 + 4 * g + e;
-                if (co >= Cout || z >= D || y >= H || x >= W) continue;
-                const float v = value(n, r, e);
-                const size_t o = (((size_t)z * H + y) * W + x) * Cout + co;
-                if (out_f) out_f[o] = v;
-                if (out_b) out_b[o] = f2bf(v);
-            }
+            if (co >= Cout || z >= D || y0 + ry + r >= H || x >= W) continue;
+            const size_t o = row0 + r * rowstride + co;
+            if (out_f) *reinterpret_cast<f32x4*>(out_f + o) = f32x4{value(n, r, 0), value(n, r, 1), value(n, r, 2), value(n, r, 3)};
+            if (out_b)
+                *reinterpret_cast<u32x2*>(out_b + o) =
+                    u32x2{pk_bf16(value(n, r, 0), value(n, r, 1)), pk_bf16(value(n, r, 2), value(n, r, 3))};
         }
     }
     if (!out_pool) return;
-    // A pooled cell (the lane's x pair k & 1, y pair k >> 1) averages its 8 children's BF16 values,
-    // summed z-major, then y, then x: the plane z0 wave (rz = 0) adds its four from 0 and hands
-    // the partial through LDS to the plane z0 + 1 wave with the same rows (rz = 1), which adds its
-    // four.
-    auto add4 = [&](float acc4, int n, int k) {
-        const int r = 2 * (k >> 1), e = 2 * (k & 1);
-        acc4 = acc4 + bf2f(f2bf(value(n, r, e)));
-        acc4 = acc4 + bf2f(f2bf(value(n, r, e + 1)));
-        acc4 = acc4 + bf2f(f2bf(value(n, r + 1, e)));
-        return acc4 + bf2f(f2bf(value(n, r + 1, e + 1)));
+    // A pooled cell averages its 8 children's BF16 values, summed z-major, then y, then x.  The x
+    // pairs sit in lanes m and m ^ 1 (exchanged by a DPP quad permutation), the y pairs in rows
+    // 2 j and 2 j + 1 of one lane, the z pairs in waves rz = 0 and 1: the plane z0 wave adds its
+    // four from 0 and hands the partial through LDS to the plane z0 + 1 wave with the same rows,
+    // which adds its four.  The even-m lanes hold the cells' sums.
+    auto bfr = [](float v) { return __uint_as_float(pk_bf16(v, 0.f) << 16); };
+    auto add4 = [&](float sum, int n, int j, int i) {
+        const float a0 = bfr(value(n, 2 * j, i)), a1 = bfr(value(n, 2 * j + 1, i));
+        const float b0 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(a0), 0xB1, 0xF, 0xF, false));
+        const float b1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(a1), 0xB1, 0xF, 0xF, false));
+        sum = sum + a0;
+        sum = sum + b0;
+        sum = sum + a1;
+        return sum + b1;
     };
-    float* part = reinterpret_cast<float*>(tile);  // NT x 2 (ry) x 4 cells x 64 lanes
-    __syncthreads();                               // every wave is done with the halo
+    float* part = reinterpret_cast<float*>(tile);  // NT x 2 (ry) x 2 (j) x 4 channels x 64 lanes
+    auto pidx = [&](int n, int j, int i) { return (((n * 2 + (wave & 1)) * 2 + j) * 4 + i) * 64 + lane; };
+    __syncthreads();  // every wave is done with the halo
     if (rz == 0) {
 #pragma unroll
         for (int n = 0; n < NT; n++)
 #pragma unroll
-            for (int k = 0; k < 4; k++) part[((n * 2 + (wave & 1)) * 4 + k) * 64 + lane] = add4(0.f, n, k);
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) part[pidx(n, j, i)] = add4(0.f, n, j, i);
     }
     __syncthreads();
     if (rz == 1) {
-        const int D2 = D >> 1, H2 = H >> 1, W2 = W >> 1, pz = z0 >> 1;
+        const int D2 = D >> 1, H2 = H >> 1, W2 = W >> 1, pz = z0 >> 1, px = (x0 >> 1) + (m >> 1);
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            const int co = (nt0 + n) * 16 + m;
+            const int co = (nt0 + n) * 16 + 4 * g;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int px = (x0 >> 1) + 2 * g + (k & 1), py = ((y0 + ry) >> 1) + (k >> 1);
-                const float sum = add4(part[((n * 2 + (wave & 1)) * 4 + k) * 64 + lane], n, k);
-                if (co < Cout && pz < D2 && py < H2 && px < W2)
-                    out_pool[(((size_t)pz * H2 + py) * W2 + px) * Cout + co] = f2bf(sum * 0.125f);
+            for (int j = 0; j < 2; j++) {
+                float s[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) s[i] = add4(part[pidx(n, j, i)], n, j, i) * 0.125f;
+                const int py = ((y0 + ry) >> 1) + j;
+                if (!(m & 1) && co < Cout && pz < D2 && py < H2 && px < W2)
+                    *reinterpret_cast<u32x2*>(out_pool + (((size_t)pz * H2 + py) * W2 + px) * Cout + co) =
+                        u32x2{pk_bf16(s[0], s[1]), pk_bf16(s[2], s[3])};
             }
         }
     }
@@ -557,6 +605,15 @@ int unet_alloc_dims(efs_unet_s* u, int D, int H, int W) {
     if (D % 8 || H % 8 || W % 8) {
         efs_set_error("U-Net dims must be multiples of 8");
         return -1;
+    }
+    {  // the convolutions address their input tensors with 32-bit byte offsets
+        const size_t nv = (size_t)D * H * W;
+        size_t big = nv * 16 * 4;  // the FP32 network input
+        for (auto& l : u->layers) big = std::max(big, nv * (size_t)std::max(l.cin, l.cout) * 2);
+        if (big > 0x7fffffffull) {
+            efs_set_error("U-Net grid too large: a convolution input would exceed 2 GiB");
+            return -1;
+        }
     }
     for (void* p : u->bufs) hipFree(p);
     u->bufs.clear();
