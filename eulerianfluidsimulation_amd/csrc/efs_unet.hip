@@ -36,30 +36,39 @@ __device__ inline float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) 
 constexpr int MAXNT = 8;  // up to 128 output channels per wavefront (weight gradients)
 
 // ---- 3x3x3 convolution (zero padding) as an implicit GEMM on v_mfma_f32_16x16x32_bf16 --------
-// A 256-thread block computes a CT_X x CT_Y x CT_Z tile of output voxels for NT 16-channel output
-// tiles.  Per 16-channel input chunk the block stages the tile's input halo (HT_X x HT_Y x HT_Z
-// voxels x 16 channels, 23 KB) in LDS once and reuses it for all 27 taps: the GEMM's K runs over
-// (tap, channel) in 14 steps of 32 (two taps of 16 channels; the 28th tap has zero weights).  Wave
-// w owns output rows y0 + 4 (w & 1) + r (r = 0..3) of plane z0 + (w >> 1): four 16-voxel tiles.
+// A 256-thread block computes a 16 x 8 x ZT tile of output voxels (ZT = 2 or 4) for NT 16-channel
+// output tiles.  Per 16-channel input chunk the block stages the tile's input halo (18 x 10 x
+// (ZT + 2) voxels x 16 channels, 23 or 35 KB) in LDS once and reuses it for all 27 taps; the GEMM's
+// K runs over (tap, channel) in 15 steps of 32 (pairs of taps, see k_conv3).  Each wave owns 2 ZT
+// rows of 16 voxels: with ZT = 2, rows y0 + 4 (w & 1) + r (r = 0..3) of plane z0 + (w >> 1); with
+// ZT = 4, rows y0 + r (r = 0..7) of plane z0 + w.  The deeper tile stages 2.1 instead of 2.8 halo
+// voxels per output voxel; the shallower one makes twice the blocks for the coarse levels.
 // Operand layout of the 16x16x32 BF16 MFMA: lane l supplies A row l % 16 and B column l % 16 for K
 // elements 8 (l / 16) .. 8 (l / 16) + 7 and receives D rows 4 (l / 16) + i, i = 0..3, of column
-// l % 16.  A holds the weights (row = output channel, a 16-byte global load) and B the voxels
-// (column = voxel, 8 consecutive channels of one tap, a 16-byte LDS load), so D is output channel x
-// voxel: lane l ends with 4 consecutive output channels of voxel l % 16, which it writes as one
-// 8-byte (BF16) or 16-byte (FP32) store.
+// l % 16.  A holds the weights (row = output channel) and B the voxels (column = voxel, 8
+// consecutive channels of one tap, a 16-byte LDS load), so D is output channel x voxel: lane l ends
+// with 4 consecutive output channels of voxel l % 16, which it writes as one 8-byte (BF16) or
+// 16-byte (FP32) store.
 // The halo is held as two planes of 16-byte halves, channels 0-7 of every voxel, then channels
-// 8-15: the 16 lanes that read one half for 16 consecutive voxels then read 256 contiguous bytes,
-// which spreads them over all LDS banks (voxel-major, 32 bytes apart, every bank took two).
+// 8-15: the 16 lanes that read one half for 16 consecutive voxels then read 256 contiguous bytes.
 // Global loads go through raw buffer resources with 32-bit byte offsets (tensor sizes are checked
 // on the host to stay below 2^31 bytes); a load at offset OOB returns zeros, which reads the halo's
-// padding and the 28th tap's weights without branches.
-// With PRE a chunk's A operands for all 14 K steps are loaded into registers before the halo is
-// staged, so the weight loads overlap the staging and the MFMA chain never waits on them: that
-// pays on the coarse levels, whose few blocks leave the latency exposed, and costs occupancy where
-// thousands of blocks hide it, so PRE is chosen per launch.
-constexpr int CT_X = 16, CT_Y = 8, CT_Z = 2;
-constexpr int HT_X = CT_X + 2, HT_Y = CT_Y + 2, HT_Z = CT_Z + 2, HT_N = HT_X * HT_Y * HT_Z;
-constexpr int KSTEPS = 14;  // 27 taps x 16 channels in steps of 32, the 28th tap zero
+// padding and the missing tap's zero weights without branches.
+// Launches of many blocks stage each chunk's weights in LDS once per block (the 4 waves share
+// them); launches of few blocks (PRE, the coarse levels) load a chunk's A operands for all 15 K
+// steps into registers before the halo is staged, so the weight loads overlap the staging.
+constexpr int CT_X = 16, CT_Y = 8;
+template <int ZT>
+struct Tile {
+    static constexpr int HX = CT_X + 2, HY = CT_Y + 2, HZ = ZT + 2, HN = HX * HY * HZ;  // the input halo
+    // LDS stride of the halves: a multiple of 256 bytes, so that the lanes reading the two halves
+    // of one voxel fall in the same banks only in different cycles (others conflict 2-way)
+    static constexpr int HS = (HN + 15) / 16 * 16;
+    static constexpr int NI = (HN * 2 + 255) / 256;  // 16-byte halo halves per thread
+    static constexpr int RW = 2 * ZT;                // rows per wave
+    // LDS offset (in halo voxels) of tap t from the tile position it applies to
+    static constexpr int tap_off(int t) { return (t / 9) * HY * HX + ((t / 3) % 3) * HX + t % 3; }
+};
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -78,201 +87,284 @@ __device__ inline uint32_t pk_bf16(float lo, float hi) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
 }
 
-// 8 channels at byte offset `off` of a buffer as 8 BF16 values in a uint4: BF16 sources are copied,
-// FP32 sources (the network input of the C entry point) are rounded to nearest even; zeros at OOB.
-__device__ inline uint4 load8(__amdgpu_buffer_rsrc_t r, unsigned off, const uint16_t*) {
-    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-}
-__device__ inline uint4 load8(__amdgpu_buffer_rsrc_t r, unsigned off, const float*) {
-    const f32x4 a = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-    const f32x4 b = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 16, 0));
-    return make_uint4(pk_bf16(a.x, a.y), pk_bf16(a.z, a.w), pk_bf16(b.x, b.y), pk_bf16(b.z, b.w));
-}
-
-// Stages the halo of the block's tile for channels [c0, c0 + 16) of a source with Csrc channels
-// (zero outside the D x H x W volume) into tile[half * HT_N + voxel].  With `up`, the source is at
-// half resolution and each fine voxel reads its parent (nearest upsampling).  Thread t stages
-// channel half t & 1 of voxels t / 2 + 128 k and issues all its loads before its first LDS write.
-template <typename TS>
-__device__ inline void stage_halo(uint4* tile, const TS* __restrict__ src, int Csrc, int c0, int x0, int y0, int z0, int D,
-                                  int H, int W, bool up) {
+// A tile's halo for channels [c0, c0 + 16) of a source with Csrc channels (zero outside the
+// D x H x W volume): thread t holds half t & 1 (8 channels, BF16) of voxels t / 2 + 128 k.
+// halo_issue loads it into registers (FP32 sources rounded to nearest even), halo_put writes
+// tile[half * HS + voxel].  With `up`, the source is at half resolution and each fine voxel reads
+// its parent (nearest upsampling).
+template <int ZT>
+struct Halo {
+    uint4 a[Tile<ZT>::NI];
+};
+template <int ZT, typename TS>
+__device__ inline void halo_issue(Halo<ZT>& h, const TS* __restrict__ src, int Csrc, int c0, int x0, int y0, int z0, int D, int H,
+                                  int W, bool up) {
+    using T = Tile<ZT>;
     const int sh = up ? 1 : 0, sD = D >> sh, sH = H >> sh, sW = W >> sh;
     const __amdgpu_buffer_rsrc_t r = bufres(src, (unsigned)sD * sH * sW * Csrc * (unsigned)sizeof(TS));
     const int half = threadIdx.x & 1;
-    constexpr int NI = (HT_N * 2 + 255) / 256;
-    uint4 val[NI];
 #pragma unroll
-    for (int k = 0; k < NI; k++) {
+    for (int k = 0; k < T::NI; k++) {
         const int v = (threadIdx.x >> 1) + 128 * k;
-        const int hx = v % HT_X, hy = (v / HT_X) % HT_Y, hz = v / (HT_X * HT_Y);
+        const int hx = v % T::HX, hy = (v / T::HX) % T::HY, hz = v / (T::HX * T::HY);
         const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-        const bool inside = v < HT_N && gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D;
+        const bool inside = v < T::HN && gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D;
         const unsigned vox = (unsigned)(((gz >> sh) * sH + (gy >> sh)) * sW + (gx >> sh));
-        val[k] = load8(r, inside ? (vox * Csrc + c0 + half * 8) * (unsigned)sizeof(TS) : OOB, src);
+        const unsigned off = inside ? (vox * Csrc + c0 + half * 8) * (unsigned)sizeof(TS) : OOB;
+        if constexpr (std::is_same<TS, float>::value) {
+            const f32x4 lo = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+            const f32x4 hi = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 16, 0));
+            h.a[k] = make_uint4(pk_bf16(lo.x, lo.y), pk_bf16(lo.z, lo.w), pk_bf16(hi.x, hi.y), pk_bf16(hi.z, hi.w));
+        } else {
+            h.a[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+        }
     }
+}
+template <int ZT>
+__device__ inline void halo_put(uint4* tile, const Halo<ZT>& h) {
+    using T = Tile<ZT>;
+    const int half = threadIdx.x & 1;
 #pragma unroll
-    for (int k = 0; k < NI; k++) {
+    for (int k = 0; k < T::NI; k++) {
         const int v = (threadIdx.x >> 1) + 128 * k;
-        if (v < HT_N) tile[half * HT_N + v] = val[k];
+        if (v < T::HN) tile[half * T::HS + v] = h.a[k];
     }
+}
+
+// The block's position (bx, by, bz) in its grid, renumbered so that each XCD (the hardware deals
+// workgroup i to XCD i mod 8, each with its own L2) takes a contiguous run of the x-fastest block
+// order: neighbouring tiles, whose input halos overlap, then share an L2 instead of each XCD
+// fetching the overlap again.
+__device__ inline void xcd_tile(int& bx, int& by, int& bz) {
+    const int gx = gridDim.x, gy = gridDim.y, n = gx * gy * gridDim.z;
+    const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int t = n % NXCD == 0 ? (b % NXCD) * (n / NXCD) + b / NXCD : b;
+    bx = t % gx;
+    by = (t / gx) % gy;
+    bz = t / (gx * gy);
 }
 
 // out[v][co] = bias[co] + sum_tap sum_ci in[v + off(tap)][ci] * w[tap][co][ci], then ReLU (relu),
 // written as FP32 (out_f) or BF16 (out_b).  With `up` (Cup channels at half resolution) the input
 // is the decoder's concatenation, channels [0, Cup) the upsampled `up` and the rest `in` (Cin - Cup
-// channels), read in place.  blockIdx.z = z tile * ngroups + output-channel group; a group covers
-// NT 16-channel tiles (tiles past Cout are skipped).  Cin, Cup, Cout: multiples of 16; D, H, W: any
-// (partial tiles are masked; with `up`, even).  `in` is BF16, or FP32 rounded to BF16 as it is
+// channels), read in place.  The block's z index (after the XCD renumbering) = z tile * ngroups +
+// output-channel group; a group covers NT 16-channel tiles (tiles past Cout are skipped).  Cin, Cup, Cout: multiples of 16; D, H, W:
+// any (partial tiles are masked; with `up`, even).  `in` is BF16, or FP32 rounded to BF16 as it is
 // staged (TIN = float).  With out_pool (D, H, W even) the block also writes the 2x2x2 average of
 // its BF16 outputs, the next encoder level's input.
-template <int NT, bool PRE, typename TIN>
+template <int NT, int ZT, bool PRE, typename TIN>
 __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, int Cup, const TIN* __restrict__ in, int Cin,
                                                const uint16_t* __restrict__ w, const float* __restrict__ bias, int Cout, int D,
                                                int H, int W, int relu, int ngroups, float* __restrict__ out_f,
                                                uint16_t* __restrict__ out_b, uint16_t* __restrict__ out_pool) {
-    __shared__ uint4 tile[HT_N * 2];
+    using T = Tile<ZT>;
+    constexpr int RW = T::RW, NFAM = 5;
+    __shared__ uint4 tile[T::HS * 2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m = lane & 15, g = lane >> 4;
-    const int x0 = blockIdx.x * CT_X, y0 = blockIdx.y * CT_Y;
-    const int z0 = (blockIdx.z / ngroups) * CT_Z, nt0 = (blockIdx.z % ngroups) * NT;
-    const int ry = (wave & 1) * 4, rz = wave >> 1;
-    floatx4 acc[4][NT];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int n = 0; n < NT; n++) acc[r][n] = floatx4{0.f, 0.f, 0.f, 0.f};
-    // the lane's LDS offsets of the K steps' taps (two taps per step, selected by g >> 1)
-    int hoff[KSTEPS];
-#pragma unroll
-    for (int s = 0; s < KSTEPS; s++) {
-        const int tap = 2 * s + (g >> 1);
-        const int tc = tap < 27 ? tap : 26;  // the padding tap reads valid LDS; its weights are 0
-        hoff[s] = (g & 1) * HT_N + ((rz + tc / 9) * HT_Y + (ry + (tc / 3) % 3)) * HT_X + (m + tc % 3);
-    }
-    // the lane's weights: output channel (nt0 + n) 16 + m, tap 2 s + (g >> 1), input channels
-    // 16 ch + 8 (g & 1) + 0..7; byte offset woff[n] + s tapstep + 32 ch
+    const int m = lane & 15, g = lane >> 4, hsel = g >> 1;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int x0 = bx * CT_X, y0 = by * CT_Y;
+    const int zt = bz / ngroups, nt0 = (bz % ngroups) * NT;
+    const int rz = wave / (4 / ZT), ry = (wave % (4 / ZT)) * RW;  // the wave's plane and first row
+    floatx4 acc[RW][NT];
+    // K steps.  The 27 taps are 9 (dz, dx) combos c = 3 dz + dx times 3 dy.  A K step holds two
+    // taps with the same dy, one per lane half hsel = g >> 1: combos 2 f and 2 f + 1 of family f
+    // (f = 0..4; family 4 pairs combo 8 with nothing, zero weights).  The B operand of row r, step
+    // (f, dy) is voxel row r + dy of the family's combos, so a family's 3 steps over RW rows read
+    // only RW + 2 rows from LDS: row q feeds the MFMAs of (r, dy) = (q - dy, dy).  15 steps of 32
+    // (7 % more MFMAs than 14) for 2.2 (RW = 8) or 1.9 (RW = 4) times fewer LDS reads.
+    const int hbase = (g & 1) * T::HS + (rz * T::HY + ry) * T::HX + m;
+    auto fam_base = [&](int f) {  // LDS index of the lane's row q = 0 in family f
+        const int c0 = 2 * f, c1 = 2 * f + 1 < 9 ? 2 * f + 1 : 8;  // the missing combo reads valid LDS
+        const int o0 = (c0 / 3) * T::HY * T::HX + c0 % 3, o1 = (c1 / 3) * T::HY * T::HX + c1 % 3;
+        return hbase + o0 + hsel * (o1 - o0);
+    };
+    // the lane's weights: output channel (nt0 + n) 16 + m, input channels 16 ch + 8 (g & 1) + 0..7
+    // of its tap: byte offset woff[n] + tap Cout Cin 2 + 32 ch
     const __amdgpu_buffer_rsrc_t rw = bufres(w, 27u * Cout * Cin * 2u);
-    const unsigned tapstep = 4u * Cout * Cin;  // two taps of BF16 weights
+    const unsigned tapbytes = 2u * Cout * Cin;
     unsigned woff[NT];
 #pragma unroll
     for (int n = 0; n < NT; n++) {
         const int co = (nt0 + n) * 16 + m;
-        woff[n] = co < Cout ? (((unsigned)(g >> 1) * Cout + co) * Cin + (g & 1) * 8) * 2u : OOB;
+        woff[n] = co < Cout ? ((unsigned)co * Cin + (g & 1) * 8) * 2u : OOB;
     }
-    // the A operand of K step s, output tile n, input chunk ch (tap 27 reads zeros)
-    auto load_a = [&](int s, int n, int ch) -> bf16x8 {
-        const unsigned off = (s == KSTEPS - 1 && (g >> 1)) ? OOB : woff[n];
-        return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rw, off, (int)(s * tapstep + ch * 32u), 0));
-    };
-    auto mfma_step = [&](int s, const bf16x8 (&a)[NT]) {
+    // the A operands of family f (steps dy = 0..2), output tile n, input chunk ch
+    auto load_fam = [&](bf16x8 (&a)[3][NT], int f, int ch) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const bf16x8 b = __builtin_bit_cast(bf16x8, tile[hoff[s] + r * HT_X]);
+        for (int dy = 0; dy < 3; dy++) {
+            const int c0 = 2 * f, c1 = 2 * f + 1;
+            const int t0 = (c0 / 3) * 9 + dy * 3 + c0 % 3, t1 = (c1 / 3) * 9 + dy * 3 + c1 % 3;
 #pragma unroll
-            for (int n = 0; n < NT; n++) acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[n], b, acc[r][n], 0, 0, 0);
-        }
-    };
-    const int nchunks = Cin >> 4;
-    for (int ch = 0; ch < nchunks; ch++) {
-        if constexpr (PRE) {
-            bf16x8 a[KSTEPS][NT];
-#pragma unroll
-            for (int s = 0; s < KSTEPS; s++)
-#pragma unroll
-                for (int n = 0; n < NT; n++) a[s][n] = load_a(s, n, ch);
-            if (ch) __syncthreads();
-            if (up && ch * 16 < Cup) stage_halo(tile, up, Cup, ch * 16, x0, y0, z0, D, H, W, true);
-            else stage_halo(tile, in, Cin - Cup, ch * 16 - Cup, x0, y0, z0, D, H, W, false);
-            __syncthreads();
-#pragma unroll
-            for (int s = 0; s < KSTEPS; s++) mfma_step(s, a[s]);
-        } else {
-            if (ch) __syncthreads();
-            if (up && ch * 16 < Cup) stage_halo(tile, up, Cup, ch * 16, x0, y0, z0, D, H, W, true);
-            else stage_halo(tile, in, Cin - Cup, ch * 16 - Cup, x0, y0, z0, D, H, W, false);
-            __syncthreads();
-#pragma unroll 2
-            for (int s = 0; s < KSTEPS; s++) {
-                bf16x8 a[NT];
-#pragma unroll
-                for (int n = 0; n < NT; n++) a[n] = load_a(s, n, ch);
-                mfma_step(s, a);
+            for (int n = 0; n < NT; n++) {
+                const unsigned off = c1 < 9 ? woff[n] + (hsel ? (unsigned)(t1 - t0) * tapbytes : 0u) : (hsel ? OOB : woff[n]);
+                a[dy][n] = __builtin_bit_cast(
+                    bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rw, off, (int)(t0 * tapbytes + ch * 32u), 0));
             }
         }
-    }
+    };
+    auto mfma_fam = [&](int f, const bf16x8 (&a)[3][NT]) {
+        const int base = fam_base(f);
+#pragma unroll
+        for (int q = 0; q < RW + 2; q++) {
+            const bf16x8 b = __builtin_bit_cast(bf16x8, tile[base + q * T::HX]);
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const int r = q - dy;
+                if (r < 0 || r >= RW) continue;
+#pragma unroll
+                for (int n = 0; n < NT; n++) acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[dy][n], b, acc[r][n], 0, 0, 0);
+            }
+        }
+    };
+    auto zero_acc = [&]() {
+#pragma unroll
+        for (int r = 0; r < RW; r++)
+#pragma unroll
+            for (int n = 0; n < NT; n++) acc[r][n] = floatx4{0.f, 0.f, 0.f, 0.f};
+    };
+    const int nchunks = Cin >> 4;
+    auto issue = [&](Halo<ZT>& h, int zt, int ch) {
+        if (up && ch * 16 < Cup) halo_issue<ZT>(h, up, Cup, ch * 16, x0, y0, zt * ZT, D, H, W, true);
+        else halo_issue<ZT>(h, in, Cin - Cup, ch * 16 - Cup, x0, y0, zt * ZT, D, H, W, false);
+    };
     // lane (m, g) holds, for row r and tile n, output channels (nt0 + n) 16 + 4 g + i (i = 0..3) of
     // voxel (x0 + m, y0 + ry + r, z0 + rz)
-    const int x = x0 + m, z = z0 + rz;
-    float bv[NT][4];
+    f32x4 bv[NT];
 #pragma unroll
     for (int n = 0; n < NT; n++) {
         const int co = (nt0 + n) * 16 + 4 * g;
-#pragma unroll
-        for (int i = 0; i < 4; i++) bv[n][i] = bias && co < Cout ? bias[co + i] : 0.f;
+        bv[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (bias)
+            bv[n] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(bufres(bias, Cout * 4u), co < Cout ? co * 4u : OOB, 0, 0));
     }
     auto value = [&](int n, int r, int i) {  // the output before rounding
         const float v = acc[r][n][i] + bv[n][i];
         return relu ? (v > 0.f ? v : 0.f) : v;
     };
-    const size_t row0 = (((size_t)z * H + y0 + ry) * W + x) * Cout, rowstride = (size_t)W * Cout;
-#pragma unroll
-    for (int n = 0; n < NT; n++) {
-        const int co = (nt0 + n) * 16 + 4 * g;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (co >= Cout || z >= D || y0 + ry + r >= H || x >= W) continue;
-            const size_t o = row0 + r * rowstride + co;
-            if (out_f) *reinterpret_cast<f32x4*>(out_f + o) = f32x4{value(n, r, 0), value(n, r, 1), value(n, r, 2), value(n, r, 3)};
-            if (out_b)
-                *reinterpret_cast<u32x2*>(out_b + o) =
-                    u32x2{pk_bf16(value(n, r, 0), value(n, r, 1)), pk_bf16(value(n, r, 2), value(n, r, 3))};
-        }
-    }
-    if (!out_pool) return;
-    // A pooled cell averages its 8 children's BF16 values, summed z-major, then y, then x.  The x
-    // pairs sit in lanes m and m ^ 1 (exchanged by a DPP quad permutation), the y pairs in rows
-    // 2 j and 2 j + 1 of one lane, the z pairs in waves rz = 0 and 1: the plane z0 wave adds its
-    // four from 0 and hands the partial through LDS to the plane z0 + 1 wave with the same rows,
-    // which adds its four.  The even-m lanes hold the cells' sums.
-    auto bfr = [](float v) { return __uint_as_float(pk_bf16(v, 0.f) << 16); };
-    auto add4 = [&](float sum, int n, int j, int i) {
-        const float a0 = bfr(value(n, 2 * j, i)), a1 = bfr(value(n, 2 * j + 1, i));
-        const float b0 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(a0), 0xB1, 0xF, 0xF, false));
-        const float b1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(a1), 0xB1, 0xF, 0xF, false));
-        sum = sum + a0;
-        sum = sum + b0;
-        sum = sum + a1;
-        return sum + b1;
-    };
-    float* part = reinterpret_cast<float*>(tile);  // NT x 2 (ry) x 2 (j) x 4 channels x 64 lanes
-    auto pidx = [&](int n, int j, int i) { return (((n * 2 + (wave & 1)) * 2 + j) * 4 + i) * 64 + lane; };
-    __syncthreads();  // every wave is done with the halo
-    if (rz == 0) {
-#pragma unroll
-        for (int n = 0; n < NT; n++)
-#pragma unroll
-            for (int j = 0; j < 2; j++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) part[pidx(n, j, i)] = add4(0.f, n, j, i);
-    }
-    __syncthreads();
-    if (rz == 1) {
-        const int D2 = D >> 1, H2 = H >> 1, W2 = W >> 1, pz = z0 >> 1, px = (x0 >> 1) + (m >> 1);
+    // the outputs of the tile at z0 (and with out_pool its 2x2x2 averages); uses the LDS tile
+    auto finish = [&](int z0) {
+        const int x = x0 + m, z = z0 + rz;
+        const size_t row0 = (((size_t)z * H + y0 + ry) * W + x) * Cout, rowstride = (size_t)W * Cout;
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             const int co = (nt0 + n) * 16 + 4 * g;
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                float s[4];
+            for (int r = 0; r < RW; r++) {
+                if (co >= Cout || z >= D || y0 + ry + r >= H || x >= W) continue;
+                const size_t o = row0 + r * rowstride + co;
+                if (out_f)
+                    *reinterpret_cast<f32x4*>(out_f + o) = f32x4{value(n, r, 0), value(n, r, 1), value(n, r, 2), value(n, r, 3)};
+                if (out_b)
+                    *reinterpret_cast<u32x2*>(out_b + o) =
+                        u32x2{pk_bf16(value(n, r, 0), value(n, r, 1)), pk_bf16(value(n, r, 2), value(n, r, 3))};
+            }
+        }
+        if (!out_pool) return;
+        // A pooled cell averages its 8 children's BF16 values, summed z-major, then y, then x.  The
+        // x pairs sit in lanes m and m ^ 1 (exchanged by a DPP quad permutation), the y pairs in rows
+        // 2 j and 2 j + 1 of one lane, the z pairs in the waves of planes rz = 2 p and 2 p + 1 with
+        // the same rows: the even plane's wave adds its four from 0 and hands the partial through
+        // LDS to the odd plane's wave, which adds its four.  The even-m lanes hold the cells' sums.
+        auto bfr = [](float v) { return __uint_as_float(pk_bf16(v, 0.f) << 16); };
+        auto add4 = [&](float sum, int n, int j, int i) {
+            const float a0 = bfr(value(n, 2 * j, i)), a1 = bfr(value(n, 2 * j + 1, i));
+            const float b0 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(a0), 0xB1, 0xF, 0xF, false));
+            const float b1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(a1), 0xB1, 0xF, 0xF, false));
+            sum = sum + a0;
+            sum = sum + b0;
+            sum = sum + a1;
+            return sum + b1;
+        };
+        // NT x 2 (wave pairs) x RW / 2 (j) x 4 channels x 64 lanes
+        float* part = reinterpret_cast<float*>(tile);
+        const int pair = (rz >> 1) * (4 / ZT) + wave % (4 / ZT);
+        auto pidx = [&](int n, int j, int i) { return (((n * 2 + pair) * (RW / 2) + j) * 4 + i) * 64 + lane; };
+        static_assert(NT * 2 * (RW / 2) * 4 * 64 <= T::HS * 2 * 4, "pool partials must fit the halo tile");
+        __syncthreads();  // every wave is done with the halo
+        if (!(rz & 1)) {
 #pragma unroll
-                for (int i = 0; i < 4; i++) s[i] = add4(part[pidx(n, j, i)], n, j, i) * 0.125f;
-                const int py = ((y0 + ry) >> 1) + j;
-                if (!(m & 1) && co < Cout && pz < D2 && py < H2 && px < W2)
-                    *reinterpret_cast<u32x2*>(out_pool + (((size_t)pz * H2 + py) * W2 + px) * Cout + co) =
-                        u32x2{pk_bf16(s[0], s[1]), pk_bf16(s[2], s[3])};
+            for (int n = 0; n < NT; n++)
+#pragma unroll
+                for (int j = 0; j < RW / 2; j++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) part[pidx(n, j, i)] = add4(0.f, n, j, i);
+        }
+        __syncthreads();
+        if (rz & 1) {
+            const int D2 = D >> 1, H2 = H >> 1, W2 = W >> 1, pz = (z0 + rz) >> 1, px = (x0 >> 1) + (m >> 1);
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const int co = (nt0 + n) * 16 + 4 * g;
+#pragma unroll
+                for (int j = 0; j < RW / 2; j++) {
+                    float sum[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) sum[i] = add4(part[pidx(n, j, i)], n, j, i) * 0.125f;
+                    const int py = ((y0 + ry) >> 1) + j;
+                    if (!(m & 1) && co < Cout && pz < D2 && py < H2 && px < W2)
+                        *reinterpret_cast<u32x2*>(out_pool + (((size_t)pz * H2 + py) * W2 + px) * Cout + co) =
+                            u32x2{pk_bf16(sum[0], sum[1]), pk_bf16(sum[2], sum[3])};
+                }
+            }
+        }
+    };
+    zero_acc();
+    if constexpr (PRE) {
+        for (int ch = 0; ch < nchunks; ch++) {
+            bf16x8 a[NFAM][3][NT];
+#pragma unroll
+            for (int f = 0; f < NFAM; f++) load_fam(a[f], f, ch);
+            Halo<ZT> h;
+            issue(h, zt, ch);
+            if (ch) __syncthreads();  // the previous chunk is done with the LDS tile
+            halo_put(tile, h);
+            __syncthreads();
+#pragma unroll
+            for (int f = 0; f < NFAM; f++) mfma_fam(f, a[f]);
+        }
+    } else {
+        // The chunk's weights are staged in LDS once per block, so a wave reads its A operands
+        // from LDS instead of each wave loading all of them: wl[((f * 3 + dy) * NT + n) * 64 + lane]
+        // is the lane's A operand of step (f, dy), tile n.
+        constexpr int WL_N = NFAM * 3 * NT * 64, WL_NI = (WL_N + 255) / 256;
+        __shared__ uint4 wl[WL_N];
+        for (int ch = 0; ch < nchunks; ch++) {
+            uint4 wv[WL_NI];
+#pragma unroll
+            for (int k = 0; k < WL_NI; k++) {
+                const int i = threadIdx.x + 256 * k;
+                const int ln = i & 63, q = i >> 6;
+                const int n = q % NT, f = (q / NT) / 3, dy = (q / NT) % 3;
+                const int gg = ln >> 4, c = 2 * f + (gg >> 1), co = (nt0 + n) * 16 + (ln & 15);
+                const int tap = (c / 3) * 9 + dy * 3 + c % 3;
+                const bool ok = i < WL_N && c < 9 && co < Cout;
+                const unsigned off = ok ? (((unsigned)tap * Cout + co) * Cin + ch * 16 + (gg & 1) * 8) * 2u : OOB;
+                wv[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rw, off, 0, 0));
+            }
+            Halo<ZT> h;
+            issue(h, zt, ch);
+            if (ch) __syncthreads();  // the previous chunk is done with the LDS tiles
+            halo_put(tile, h);
+#pragma unroll
+            for (int k = 0; k < WL_NI; k++) {
+                const int i = threadIdx.x + 256 * k;
+                if (i < WL_N) wl[i] = wv[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int f = 0; f < NFAM; f++) {
+                bf16x8 a[3][NT];
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                    for (int n = 0; n < NT; n++) a[dy][n] = __builtin_bit_cast(bf16x8, wl[((f * 3 + dy) * NT + n) * 64 + lane]);
+                mfma_fam(f, a);
             }
         }
     }
+    finish(zt * ZT);
 }
 
 // Output channel 0 only of a 16 -> Cout 3x3x3 convolution without activation (the U-Net's last
@@ -281,9 +373,16 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, 
 // through the scalar path, so the LDS carries only the voxels' channels.
 __global__ __launch_bounds__(256) void k_conv3_ch0(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w, int Cout,
                                                    const float* __restrict__ bias, int D, int H, int W, float* __restrict__ out0) {
-    __shared__ uint4 tile[HT_N * 2];
-    const int x0 = blockIdx.x * CT_X, y0 = blockIdx.y * CT_Y, z0 = blockIdx.z * CT_Z;
-    stage_halo(tile, in, 16, 0, x0, y0, z0, D, H, W, false);
+    using T = Tile<2>;
+    __shared__ uint4 tile[T::HS * 2];
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int x0 = bx * CT_X, y0 = by * CT_Y, z0 = bz * 2;
+    {
+        Halo<2> h;
+        halo_issue<2>(h, in, 16, 0, x0, y0, z0, D, H, W, false);
+        halo_put(tile, h);
+    }
     __syncthreads();
     const int tx = threadIdx.x & 15, ty = (threadIdx.x >> 4) & 7, tz = threadIdx.x >> 7;
     const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
@@ -295,8 +394,8 @@ __global__ __launch_bounds__(256) void k_conv3_ch0(const uint16_t* __restrict__ 
 #pragma unroll
     for (int t9 = 0; t9 < 9; t9++) {
         const int tap = dz * 9 + t9, dy = t9 / 3, dx = t9 % 3;
-        const int hv = ((tz + dz) * HT_Y + (ty + dy)) * HT_X + (tx + dx);
-        const uint4 lo = tile[hv], hi = tile[HT_N + hv];  // the voxel's channels 0-7 and 8-15
+        const int hv = ((tz + dz) * T::HY + (ty + dy)) * T::HX + (tx + dx);
+        const uint4 lo = tile[hv], hi = tile[T::HS + hv];  // the voxel's channels 0-7 and 8-15
         const uint4* wt = reinterpret_cast<const uint4*>(w + (size_t)tap * Cout * 16);  // co 0: 16 channels
         const uint4 wl = wt[0], wh = wt[1];
         const unsigned xv[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
@@ -670,27 +769,34 @@ int prep_weights(efs_unet_s* u) {
 
 // A 3x3x3 convolution of BF16 channels-last input with `cin` channels into `cout` channels
 // (k_conv3: NT output tiles per block, the rest in further block groups).
+constexpr int DEEP_MAX_NT = 1;  // launches of many blocks with NT <= this use the 16 x 8 x 4 tile
 template <typename TIN>
 void conv3(hipStream_t st, const TIN* in, int cin, const uint16_t* w, const float* bias, int cout, int D, int H, int W,
            int relu, float* of, uint16_t* ob, const uint16_t* up = nullptr, int cup = 0, uint16_t* opool = nullptr) {
     const int ntiles = cout / 16;
-    const int ntile_blocks = ((W + CT_X - 1) / CT_X) * ((H + CT_Y - 1) / CT_Y) * ((D + CT_Z - 1) / CT_Z);
+    const int nxy = ((W + CT_X - 1) / CT_X) * ((H + CT_Y - 1) / CT_Y);
     // few voxel tiles (the coarse levels): one output tile per block, so that more blocks share
-    // the work, with its B operands preloaded; otherwise up to 4 per block, which reuses each
+    // the work, with its A operands preloaded; otherwise up to 4 per block, which reuses each
     // staged halo for more outputs
     int NT = ntiles <= 4 ? ntiles : (ntiles % 4 == 0 ? 4 : (ntiles % 3 == 0 ? 3 : 4));
-    const bool few = ntile_blocks * ((ntiles + NT - 1) / NT) < 1024;
+    const bool few = nxy * ((D + 1) / 2) * ((ntiles + NT - 1) / NT) < 1024;
     if (few) NT = 1;
-    const int ngroups = (ntiles + NT - 1) / NT;
-    const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, ((D + CT_Z - 1) / CT_Z) * ngroups);
-#define EFS_CONV3(nt, pre) \
-    hipLaunchKernelGGL((k_conv3<nt, pre, TIN>), grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, \
-                       ob, opool)
-    if (few) EFS_CONV3(1, true);  // B operands preloaded in registers (56 VGPRs)
-    else if (NT == 1) EFS_CONV3(1, false);
-    else if (NT == 2) EFS_CONV3(2, false);
-    else if (NT == 3) EFS_CONV3(3, false);
-    else EFS_CONV3(4, false);
+    const int ZT = !few && NT <= DEEP_MAX_NT ? 4 : 2;
+    const int ngroups = (ntiles + NT - 1) / NT, nz = (D + ZT - 1) / ZT;
+    const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, nz * ngroups);
+#define EFS_CONV3(nt, zt, pre) \
+    hipLaunchKernelGGL((k_conv3<nt, zt, pre, TIN>), grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, \
+                       of, ob, opool)
+    if (few) EFS_CONV3(1, 2, true);  // A operands preloaded in registers
+    else if (ZT == 4) {
+        if (NT == 1) EFS_CONV3(1, 4, false);
+        else if (NT == 2) EFS_CONV3(DEEP_MAX_NT >= 2 ? 2 : 1, 4, false);
+        else if (NT == 3) EFS_CONV3(DEEP_MAX_NT >= 3 ? 3 : 1, 4, false);
+        else EFS_CONV3(DEEP_MAX_NT >= 4 ? 4 : 1, 4, false);
+    } else if (NT == 1) EFS_CONV3(1, 2, false);
+    else if (NT == 2) EFS_CONV3(2, 2, false);
+    else if (NT == 3) EFS_CONV3(3, 2, false);
+    else EFS_CONV3(4, 2, false);
 #undef EFS_CONV3
 }
 
@@ -700,7 +806,7 @@ void conv_fwd(efs_unet_s* u, const Conv& l, const TI* in, int D, int H, int W, f
               const uint16_t* up = nullptr, int cup = 0, uint16_t* opool = nullptr) {
     if constexpr (std::is_same<TI, uint16_t>::value) {
         if (ch0) {  // channel 0 only (inference output); the layer has 16 input channels
-            const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, (D + CT_Z - 1) / CT_Z);
+            const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, (D + 1) / 2);
             hipLaunchKernelGGL(k_conv3_ch0, grid, dim3(256), 0, u->stream, in, u->wb + l.woff, l.cout, u->w + l.boff, D, H, W, of);
             return;
         }
