@@ -36,13 +36,13 @@ __device__ inline float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) 
 constexpr int MAXNT = 8;  // up to 128 output channels per wavefront (weight gradients)
 
 // ---- 3x3x3 convolution (zero padding) as an implicit GEMM on v_mfma_f32_16x16x32_bf16 --------
-// A 256-thread block computes a 16 x 8 x ZT tile of output voxels (ZT = 2 or 4) for NT 16-channel
-// output tiles.  Per 16-channel input chunk the block stages the tile's input halo (18 x 10 x
-// (ZT + 2) voxels x 16 channels, 23 or 35 KB) in LDS once and reuses it for all 27 taps; the GEMM's
-// K runs over (tap, channel) in 15 steps of 32 (pairs of taps, see k_conv3).  Each wave owns 2 ZT
-// rows of 16 voxels: with ZT = 2, rows y0 + 4 (w & 1) + r (r = 0..3) of plane z0 + (w >> 1); with
-// ZT = 4, rows y0 + r (r = 0..7) of plane z0 + w.  The deeper tile stages 2.1 instead of 2.8 halo
-// voxels per output voxel; the shallower one makes twice the blocks for the coarse levels.
+// A 256-thread block computes a 16 x 8 x ZT tile of output voxels (ZT = 1, 2 or 4) for NT
+// 16-channel output tiles.  Per 16-channel input chunk the block stages the tile's input halo (18 x
+// 10 x (ZT + 2) voxels x 16 channels, 17, 23 or 35 KB) in LDS once and reuses it for all 27 taps;
+// the GEMM's K runs over (tap, channel) in 15 steps of 32 (pairs of taps, see k_conv3).  Each wave
+// owns 2 ZT rows of 16 voxels: rows y0 + 2 ZT (w % (4 / ZT)) + r (r < 2 ZT) of plane
+// z0 + w / (4 / ZT).  The deeper tiles stage fewer halo voxels per output voxel (2.1 for ZT = 4,
+// 2.8 for 2, 4.2 for 1); the shallower ones make more blocks for the coarse levels.
 // Operand layout of the 16x16x32 BF16 MFMA: lane l supplies A row l % 16 and B column l % 16 for K
 // elements 8 (l / 16) .. 8 (l / 16) + 7 and receives D rows 4 (l / 16) + i, i = 0..3, of column
 // l % 16.  A holds the weights (row = output channel) and B the voxels (column = voxel, 8
@@ -54,9 +54,7 @@ constexpr int MAXNT = 8;  // up to 128 output channels per wavefront (weight gra
 // Global loads go through raw buffer resources with 32-bit byte offsets (tensor sizes are checked
 // on the host to stay below 2^31 bytes); a load at offset OOB returns zeros, which reads the halo's
 // padding and the missing tap's zero weights without branches.
-// Launches of many blocks stage each chunk's weights in LDS once per block (the 4 waves share
-// them); launches of few blocks (PRE, the coarse levels) load a chunk's A operands for all 15 K
-// steps into registers before the halo is staged, so the weight loads overlap the staging.
+// Each chunk's weights are staged in LDS once per block (the 4 waves share them) beside the halo.
 constexpr int CT_X = 16, CT_Y = 8;
 template <int ZT>
 struct Tile {
@@ -152,7 +150,7 @@ __device__ inline void xcd_tile(int& bx, int& by, int& bz) {
 // any (partial tiles are masked; with `up`, even).  `in` is BF16, or FP32 rounded to BF16 as it is
 // staged (TIN = float).  With out_pool (D, H, W even) the block also writes the 2x2x2 average of
 // its BF16 outputs, the next encoder level's input.
-template <int NT, int ZT, bool PRE, typename TIN>
+template <int NT, int ZT, typename TIN>
 __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, int Cup, const TIN* __restrict__ in, int Cin,
                                                const uint16_t* __restrict__ w, const float* __restrict__ bias, int Cout, int D,
                                                int H, int W, int relu, int ngroups, float* __restrict__ out_f,
@@ -180,30 +178,7 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, 
         const int o0 = (c0 / 3) * T::HY * T::HX + c0 % 3, o1 = (c1 / 3) * T::HY * T::HX + c1 % 3;
         return hbase + o0 + hsel * (o1 - o0);
     };
-    // the lane's weights: output channel (nt0 + n) 16 + m, input channels 16 ch + 8 (g & 1) + 0..7
-    // of its tap: byte offset woff[n] + tap Cout Cin 2 + 32 ch
     const __amdgpu_buffer_rsrc_t rw = bufres(w, 27u * Cout * Cin * 2u);
-    const unsigned tapbytes = 2u * Cout * Cin;
-    unsigned woff[NT];
-#pragma unroll
-    for (int n = 0; n < NT; n++) {
-        const int co = (nt0 + n) * 16 + m;
-        woff[n] = co < Cout ? ((unsigned)co * Cin + (g & 1) * 8) * 2u : OOB;
-    }
-    // the A operands of family f (steps dy = 0..2), output tile n, input chunk ch
-    auto load_fam = [&](bf16x8 (&a)[3][NT], int f, int ch) {
-#pragma unroll
-        for (int dy = 0; dy < 3; dy++) {
-            const int c0 = 2 * f, c1 = 2 * f + 1;
-            const int t0 = (c0 / 3) * 9 + dy * 3 + c0 % 3, t1 = (c1 / 3) * 9 + dy * 3 + c1 % 3;
-#pragma unroll
-            for (int n = 0; n < NT; n++) {
-                const unsigned off = c1 < 9 ? woff[n] + (hsel ? (unsigned)(t1 - t0) * tapbytes : 0u) : (hsel ? OOB : woff[n]);
-                a[dy][n] = __builtin_bit_cast(
-                    bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rw, off, (int)(t0 * tapbytes + ch * 32u), 0));
-            }
-        }
-    };
     auto mfma_fam = [&](int f, const bf16x8 (&a)[3][NT]) {
         const int base = fam_base(f);
 #pragma unroll
@@ -261,7 +236,7 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, 
                         u32x2{pk_bf16(value(n, r, 0), value(n, r, 1)), pk_bf16(value(n, r, 2), value(n, r, 3))};
             }
         }
-        if (!out_pool) return;
+        if (!out_pool || ZT < 2) return;  // pooling pairs planes within the tile (never asked with ZT = 1)
         // A pooled cell averages its 8 children's BF16 values, summed z-major, then y, then x.  The
         // x pairs sit in lanes m and m ^ 1 (exchanged by a DPP quad permutation), the y pairs in rows
         // 2 j and 2 j + 1 of one lane, the z pairs in the waves of planes rz = 2 p and 2 p + 1 with
@@ -311,20 +286,7 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, 
         }
     };
     zero_acc();
-    if constexpr (PRE) {
-        for (int ch = 0; ch < nchunks; ch++) {
-            bf16x8 a[NFAM][3][NT];
-#pragma unroll
-            for (int f = 0; f < NFAM; f++) load_fam(a[f], f, ch);
-            Halo<ZT> h;
-            issue(h, zt, ch);
-            if (ch) __syncthreads();  // the previous chunk is done with the LDS tile
-            halo_put(tile, h);
-            __syncthreads();
-#pragma unroll
-            for (int f = 0; f < NFAM; f++) mfma_fam(f, a[f]);
-        }
-    } else {
+    {
         // The chunk's weights are staged in LDS once per block, so a wave reads its A operands
         // from LDS instead of each wave loading all of them: wl[((f * 3 + dy) * NT + n) * 64 + lane]
         // is the lane's A operand of step (f, dy), tile n.
@@ -769,34 +731,29 @@ int prep_weights(efs_unet_s* u) {
 
 // A 3x3x3 convolution of BF16 channels-last input with `cin` channels into `cout` channels
 // (k_conv3: NT output tiles per block, the rest in further block groups).
-constexpr int DEEP_MAX_NT = 1;  // launches of many blocks with NT <= this use the 16 x 8 x 4 tile
 template <typename TIN>
 void conv3(hipStream_t st, const TIN* in, int cin, const uint16_t* w, const float* bias, int cout, int D, int H, int W,
            int relu, float* of, uint16_t* ob, const uint16_t* up = nullptr, int cup = 0, uint16_t* opool = nullptr) {
     const int ntiles = cout / 16;
     const int nxy = ((W + CT_X - 1) / CT_X) * ((H + CT_Y - 1) / CT_Y);
-    // few voxel tiles (the coarse levels): one output tile per block, so that more blocks share
-    // the work, with its A operands preloaded; otherwise up to 4 per block, which reuses each
-    // staged halo for more outputs
+    // Up to 4 output tiles per block, which reuses each staged halo for more outputs; few voxel
+    // tiles (the coarse levels): one per block, so that more blocks share the work, and one plane
+    // per tile where no pooling pairs planes.  Many blocks with one output tile: 4 planes.
     int NT = ntiles <= 4 ? ntiles : (ntiles % 4 == 0 ? 4 : (ntiles % 3 == 0 ? 3 : 4));
     const bool few = nxy * ((D + 1) / 2) * ((ntiles + NT - 1) / NT) < 1024;
     if (few) NT = 1;
-    const int ZT = !few && NT <= DEEP_MAX_NT ? 4 : 2;
+    const int ZT = few ? (opool ? 2 : 1) : (NT == 1 ? 4 : 2);
     const int ngroups = (ntiles + NT - 1) / NT, nz = (D + ZT - 1) / ZT;
     const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, nz * ngroups);
-#define EFS_CONV3(nt, zt, pre) \
-    hipLaunchKernelGGL((k_conv3<nt, zt, pre, TIN>), grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, \
-                       of, ob, opool)
-    if (few) EFS_CONV3(1, 2, true);  // A operands preloaded in registers
-    else if (ZT == 4) {
-        if (NT == 1) EFS_CONV3(1, 4, false);
-        else if (NT == 2) EFS_CONV3(DEEP_MAX_NT >= 2 ? 2 : 1, 4, false);
-        else if (NT == 3) EFS_CONV3(DEEP_MAX_NT >= 3 ? 3 : 1, 4, false);
-        else EFS_CONV3(DEEP_MAX_NT >= 4 ? 4 : 1, 4, false);
-    } else if (NT == 1) EFS_CONV3(1, 2, false);
-    else if (NT == 2) EFS_CONV3(2, 2, false);
-    else if (NT == 3) EFS_CONV3(3, 2, false);
-    else EFS_CONV3(4, 2, false);
+#define EFS_CONV3(nt, zt) \
+    hipLaunchKernelGGL((k_conv3<nt, zt, TIN>), grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, \
+                       ob, opool)
+    if (ZT == 1) EFS_CONV3(1, 1);
+    else if (ZT == 4) EFS_CONV3(1, 4);
+    else if (NT == 1) EFS_CONV3(1, 2);
+    else if (NT == 2) EFS_CONV3(2, 2);
+    else if (NT == 3) EFS_CONV3(3, 2);
+    else EFS_CONV3(4, 2);
 #undef EFS_CONV3
 }
 
