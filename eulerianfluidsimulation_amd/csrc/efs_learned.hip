@@ -34,6 +34,8 @@ struct Learned {
     float *z = nullptr, *q = nullptr, *y = nullptr;
     double *p_rz = nullptr, *p_zq = nullptr;
     size_t n = 0;
+    CGState* slot = nullptr;      // pinned host copies of the solver state, by iteration parity
+    hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
 __device__ inline uint16_t f2bf(float f) {
@@ -55,15 +57,30 @@ __device__ inline double sigma_from(const double* parts, int nparts, double nint
     return (s > 0.0 && isfinite(s)) ? s : 1.0;
 }
 
-// R: one thread per coarse cell; writes the 16-channel network input (BF16 or FP32).
+// sigma of a sum of squares ss over n cells
+__device__ inline double sigma_of(double ss, double n) {
+    const double s = sqrt(ss / n);
+    return (s > 0.0 && isfinite(s)) ? s : 1.0;
+}
+
+// R: one thread per coarse cell; writes the 16-channel network input (BF16 or FP32) as two
+// 16-byte (BF16) or four (FP32) stores.  sigma from `parts` (the sum-of-squares partials of v,
+// stored in st->sigma by block 0), or, with parts null (the flexible iterations), the value the
+// convergence check stored; then the pass is a no-op once the solve is done.
 template <typename TO>
 __global__ __launch_bounds__(CG_NT) void k_restrict(Coarse cg, const uint16_t* __restrict__ rcode, const uint8_t* __restrict__ bf,
                                                     const float* __restrict__ v, const double* parts, int nparts,
                                                     CGState* st, TO* __restrict__ out) {
     __shared__ double red[CG_NT / 64];
-    const double sigma = sigma_from(parts, nparts, (double)(cg.W - 2) * (cg.H - 2) * (cg.D - 2), red);
     const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-    if (blockIdx.x == 0 && lane == 0) st->sigma = sigma;
+    double sigma;
+    if (parts) {
+        sigma = sigma_from(parts, nparts, (double)(cg.W - 2) * (cg.H - 2) * (cg.D - 2), red);
+        if (blockIdx.x == 0 && lane == 0) st->sigma = sigma;
+    } else {
+        if (st->done) return;
+        sigma = st->sigma;
+    }
     const long long i = (long long)blockIdx.x * CG_NT + lane;
     const long long nc = (long long)cg.nx * cg.ny * cg.nz;
     if (i >= nc) return;
@@ -80,14 +97,15 @@ __global__ __launch_bounds__(CG_NT) void k_restrict(Coarse cg, const uint16_t* _
         if (bf[c] & BF_EQ1) ns++;
     }
     const float inv = (float)(1.0 / (8.0 * sigma));
-    float vals[16];
-    vals[0] = s * inv;
-    vals[1] = nf * 0.125f;
-    vals[2] = ns * 0.125f;
-    for (int k = 3; k < 16; k++) vals[k] = 0.f;
-    for (int k = 0; k < 16; k++) {
-        if constexpr (sizeof(TO) == 2) out[i * 16 + k] = f2bf(vals[k]);
-        else out[i * 16 + k] = vals[k];
+    const float v0 = s * inv, v1 = nf * 0.125f, v2 = ns * 0.125f;  // channels 3-15 are 0
+    if constexpr (sizeof(TO) == 2) {
+        uint4* o = reinterpret_cast<uint4*>(out + i * 16);
+        o[0] = make_uint4((uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16), (uint32_t)f2bf(v2), 0u, 0u);
+        o[1] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        float4* o = reinterpret_cast<float4*>(out + i * 16);
+        o[0] = make_float4(v0, v1, v2, 0.f);
+        o[1] = o[2] = o[3] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
@@ -119,6 +137,7 @@ __global__ __launch_bounds__(CG_NT) void k_prolong(Dom d, Coarse cg, const uint1
                                                    const float* __restrict__ y, const float* __restrict__ q, float out_scale,
                                                    float jw, const CGState* st, float* __restrict__ zout, double* __restrict__ x64,
                                                    int mode, double* p_rz, double* p_zq) {
+    if (mode == 0 && st->done) return;
     __shared__ double red[CG_NT / 64];
     const float scale = (float)(out_scale * st->sigma);
     int bx, by, bz;
@@ -154,71 +173,16 @@ __global__ __launch_bounds__(CG_NT) void k_prolong(Dom d, Coarse cg, const uint1
     }
 }
 
-// F1: beta, d_k = z_k + beta d_{k-1} (formed while staging), q_k = A d_k, partial d.q
-__global__ __launch_bounds__(CG_NT) void k_fcg_dir(Dom d, int it, const uint16_t* __restrict__ rcode, const float* __restrict__ zv,
-                                                   float* d0, float* d1, float* __restrict__ q, const CGState* st,
-                                                   const double* p_zq, double* p_dq) {
-    __shared__ double red[CG_NT / 64];
-    const double zq = sum_partials(p_zq, gridDim.x, red);
-    const double beta = (it == 0 || st->dq == 0.0) ? 0.0 : -zq / st->dq;
-    float* dnew = (it & 1) ? d1 : d0;
-    const float* dold = (it & 1) ? d0 : d1;
-    __shared__ float sh[2][CG_TY + 2][TX + 2];
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    auto Dv = [&](int c) -> float {
-        if (it == 0) return zv[c];
-        return (float)((double)zv[c] + beta * (double)dold[c]);
-    };
-    int bx, by, bz;
-    double v = 0.0;
-    if (block_coords(d, bx, by, bz)) {
-        const int x = d.x0 + bx * TX + tx, y = d.y0 + by * CG_TY + ty;
-        const bool in_dom = x < d.x0 + d.nx && y < d.y0 + d.ny;
-        Stager<CG_TY> S;
-        S.init(d, x, y);
-        const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
-        const int sz = d.W * d.H;
-        int c = (zb * d.H + y) * d.W + x;
-        float dm = (zb > 0 && S.own_ok) ? Dv(c - sz) : 0.f;
-        float dc = S.own_ok ? Dv(c) : 0.f;
-        sh[0][ty + 1][tx + 1] = dc;
-        if (S.hx_slot >= 0) sh[0][ty + 1][S.hx_slot] = S.hx_ok ? Dv(c + S.hx_off) : 0.f;
-        if (S.hy_slot >= 0) sh[0][S.hy_slot][tx + 1] = S.hy_ok ? Dv(c + S.hy_off) : 0.f;
-        __syncthreads();
-        int cur = 0;
-        for (int z = zb; z < ze; z++, c += sz) {
-            const bool has_next = z + 1 < d.D;
-            const float np = (has_next && S.own_ok) ? Dv(c + sz) : 0.f;
-            const float nhx = (has_next && S.hx_ok) ? Dv(c + sz + S.hx_off) : 0.f;
-            const float nhy = (has_next && S.hy_ok) ? Dv(c + sz + S.hy_off) : 0.f;
-            if (in_dom) {
-                const unsigned rc = rcode[c];
-                float t = dc;
-                if (rc & RC_CG_ROW)
-                    t = apply_row(rc, dc, sh[cur][ty + 1][tx], sh[cur][ty + 1][tx + 2], sh[cur][ty][tx + 1],
-                                  sh[cur][ty + 2][tx + 1], np, dm);
-                dnew[c] = dc;
-                q[c] = t;
-                v += (double)dc * (double)t;
-            }
-            sh[cur ^ 1][ty + 1][tx + 1] = np;
-            if (S.hx_slot >= 0) sh[cur ^ 1][ty + 1][S.hx_slot] = nhx;
-            if (S.hy_slot >= 0) sh[cur ^ 1][S.hy_slot][tx + 1] = nhy;
-            __syncthreads();
-            cur ^= 1;
-            dm = dc;
-            dc = np;
-        }
-    }
-    write_partial(v, p_dq, red);
-}
-
 // F2: alpha = r.z / d.q; xd += alpha d; r -= alpha q; partial r.r.  fold != 0: replacement
-// iteration, x64 += xd + alpha d and xd = 0 (the residual kernel then recomputes r).
+// iteration, x64 += xd + alpha d and xd = 0 (the residual kernel then recomputes r).  (F1, the
+// direction d_k = z_k + beta d_{k-1} with q_k = A d_k, is efs_fcg_dir.)  Each thread walks its
+// column chunk with the next cell's loads issued before the current cell's stores; a no-op once
+// the solve is done.
 __global__ __launch_bounds__(CG_NT) void k_fcg_update(Dom d, int it, const float* d0, const float* d1, const float* __restrict__ q,
                                                       float* __restrict__ r, float* __restrict__ xd, double* __restrict__ x64,
                                                       int fold, CGState* st, const double* p_rz, const double* p_dq,
                                                       double* p_rr) {
+    if (st->done) return;
     __shared__ double red[CG_NT / 64];
     const double rz = sum_partials(p_rz, gridDim.x, red);
     const double dq = sum_partials(p_dq, gridDim.x, red);
@@ -227,24 +191,32 @@ __global__ __launch_bounds__(CG_NT) void k_fcg_update(Dom d, int it, const float
         st->dq = dq;
         st->alpha = alpha;
     }
-    const float* dd = (it & 1) ? d1 : d0;
+    const float* __restrict__ dd = (it & 1) ? d1 : d0;
     int bx, by, bz;
     double v = 0.0;
     if (block_coords(d, bx, by, bz)) {
         const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * d.ty + threadIdx.y;
         const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
         if (x < d.x0 + d.nx && y < d.y0 + d.ny) {
-            for (int z = zb; z < ze; z++) {
-                const int c = (z * d.H + y) * d.W + x;
-                const float xn = (float)((double)xd[c] + alpha * (double)dd[c]);
+            const int sz = d.W * d.H, last = sz * d.D - 1;
+            int c = (zb * d.H + y) * d.W + x;
+            float xn = xd[c], dn = dd[c], rn = r[c], qn = q[c];
+            for (int z = zb; z < ze; z++, c += sz) {
+                const float xc = xn, dc = dn, rc = rn, qc = qn;
+                const int cn = min(c + sz, last);  // past the chunk: loaded, not used
+                xn = xd[cn];
+                dn = dd[cn];
+                rn = r[cn];
+                qn = q[cn];
+                const float xnew = (float)((double)xc + alpha * (double)dc);
                 if (fold) {
-                    x64[c] = x64[c] + (double)xn;
+                    x64[c] = x64[c] + (double)xnew;
                     xd[c] = 0.f;
                 } else {
-                    xd[c] = xn;
-                    const float rn = (float)((double)r[c] - alpha * (double)q[c]);
-                    r[c] = rn;
-                    v += (double)rn * (double)rn;
+                    xd[c] = xnew;
+                    const float rnew = (float)((double)rc - alpha * (double)qc);
+                    r[c] = rnew;
+                    v += (double)rnew * (double)rnew;
                 }
             }
         }
@@ -254,7 +226,9 @@ __global__ __launch_bounds__(CG_NT) void k_fcg_update(Dom d, int it, const float
 
 // FP64 residual r = b - A x64 (replacement iteration); partial r.r
 __global__ __launch_bounds__(CG_NT) void k_fcg_residual(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
-                                                        const double* __restrict__ x64, float* __restrict__ r, double* p_rr) {
+                                                        const double* __restrict__ x64, float* __restrict__ r, double* p_rr,
+                                                        const CGState* st) {
+    if (st->done) return;
     __shared__ double red[CG_NT / 64];
     int bx, by, bz;
     double v = 0.0;
@@ -293,9 +267,12 @@ __global__ __launch_bounds__(CG_NT) void k_sumsq(Dom d, const float* __restrict_
     write_partial(s, parts, red);
 }
 
-// convergence test of iteration `it` (one block)
-__global__ __launch_bounds__(CG_NT) void k_fcg_check(int nparts, int it, int max_it, CGState* st, const double* p_rr,
-                                                     const double* p_bb) {
+// convergence test of iteration `it` (one block); also stores sigma = RMS of r_it over the
+// `ninterior` cells for the restriction.  A no-op once the solve is done, so the state keeps the
+// iteration that stopped it.
+__global__ __launch_bounds__(CG_NT) void k_fcg_check(int nparts, int it, int max_it, double ninterior, CGState* st,
+                                                     const double* p_rr, const double* p_bb) {
+    if (st->done) return;
     __shared__ double red[CG_NT / 64];
     const double g = sum_partials(p_rr, nparts, red);
     double thr = st->thr, bb = st->bnorm2;
@@ -306,6 +283,7 @@ __global__ __launch_bounds__(CG_NT) void k_fcg_check(int nparts, int it, int max
     }
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         st->gamma = g;
+        st->sigma = sigma_of(g, ninterior);
         if (it == 0) {
             st->thr = thr;
             st->bnorm2 = bb;
@@ -353,6 +331,8 @@ int ensure_buffers(efs_sim* s, Learned* L) {
     EFS_HIP(hipMalloc(&L->p_rz, (size_t)s->max_blocks * 8));
     EFS_HIP(hipMalloc(&L->p_zq, (size_t)s->max_blocks * 8));
     EFS_HIP(hipMemset(L->p_zq, 0, (size_t)s->max_blocks * 8));
+    EFS_HIP(hipHostMalloc((void**)&L->slot, 2 * sizeof(CGState), hipHostMallocDefault));
+    for (auto& e : L->ev) EFS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     L->n = N;
     return 0;
 }
@@ -366,8 +346,10 @@ bool coarse_ok(const efs_sim* s) {
     return true;
 }
 
-// M applied to v (with the sum-of-squares partials of v in `parts`): network input, forward,
-// prolongation into z (mode 0) or into x64 (mode 1).  use_net = false keeps the diagonal term only.
+// M applied to v: network input, forward, prolongation into z (mode 0) or into x64 (mode 1).
+// sigma from the sum-of-squares partials of v in `parts`, or (parts null, mode 0) from the
+// convergence check; in mode 0 every pass, the network's included, is a no-op once the solve is
+// done.  use_net = false keeps the diagonal term only.
 int apply_M(efs_sim* s, Learned* L, const Dom& dt, const float* v, const double* parts, int mode, bool use_net = true) {
     const Coarse cg = coarse_of(s);
     const int nbt = launch_blocks(dt);
@@ -380,7 +362,7 @@ int apply_M(efs_sim* s, Learned* L, const Dom& dt, const float* v, const double*
         hipLaunchKernelGGL(k_restrict<uint16_t>, dim3((unsigned)((nc + CG_NT - 1) / CG_NT)), dim3(TX, CG_TY), 0, s->stream, cg,
                            s->rowcode, s->bf, v, parts, nbt, s->st, inb);
         EFS_HIP(hipGetLastError());
-        if ((rc = efs_unet_forward_on(L->net, s->stream, cg.nz, cg.ny, cg.nx, &y))) return rc;
+        if ((rc = efs_unet_forward_on(L->net, s->stream, cg.nz, cg.ny, cg.nx, &y, mode == 0 ? &s->st->done : nullptr))) return rc;
     }
     hipLaunchKernelGGL(k_prolong, dim3(nbt), dim3(TX, CG_TY), 0, s->stream, dt, cg, s->rowcode, v, y, L->q,
                        L->opts.out_scale, L->opts.jacobi_weight, s->st, L->z, s->cg_x64, mode, L->p_rz, L->p_zq);
@@ -397,6 +379,9 @@ void efs_learned_free(efs_sim* s) {
     hipFree(L->q);
     hipFree(L->p_rz);
     hipFree(L->p_zq);
+    if (L->slot) hipHostFree(L->slot);
+    for (auto e : L->ev)
+        if (e) hipEventDestroy(e);
     delete L;
     s->learned = nullptr;
 }
@@ -450,33 +435,54 @@ int efs_learned_cg_solve(efs_sim* s, efs_stats* out) {
     if ((rc = efs_cg_init_residual(s, false))) return rc;
     const int max_it = s->P.max_steps;
     const int fmax = L->opts.max_iters > 0 ? std::min(L->opts.max_iters, max_it) : max_it;
+    const double ninterior = (double)(s->W - 2) * (s->H - 2) * (s->D - 2);
+    // Iteration it queues the convergence check of r_it, then reads the check of iteration it - 1
+    // (copied to a pinned slot) and queues z = M r_it, the direction and the update: the host
+    // decides one iteration behind, so the GPU does not wait for it between iterations.  Once a
+    // check sets done, the passes queued after it, the network's included, return at once, and
+    // later checks keep the state of the iteration that stopped.  A stall seen in iteration
+    // it - 1's check stops the flexible solve after iteration it - 1.
     std::vector<double> hist;
-    int it = 0;
+    int it = 0, fcg_iters = 0;
     bool stalled = false;
+    CGState fin{};
     for (;; it++) {
-        hipLaunchKernelGGL(k_fcg_check, dim3(1), bt, 0, s->stream, nbt, it, max_it, s->st, s->p_rr, s->p_bb);
-        EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
-        EFS_HIP(hipStreamSynchronize(s->stream));
-        const CGState& h = *s->st_host;
-        if (h.done) break;
-        hist.push_back(h.gamma);
-        const int w = L->opts.stall_window;
-        if (it >= fmax || (w > 0 && (int)hist.size() > w && h.gamma > L->opts.stall_ratio * hist[hist.size() - 1 - w])) {
-            stalled = true;
+        hipLaunchKernelGGL(k_fcg_check, dim3(1), bt, 0, s->stream, nbt, it, max_it, ninterior, s->st, s->p_rr, s->p_bb);
+        EFS_HIP(hipMemcpyAsync(L->slot + (it & 1), s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
+        EFS_HIP(hipEventRecord(L->ev[it & 1], s->stream));
+        if (it > 0) {
+            EFS_HIP(hipEventSynchronize(L->ev[(it - 1) & 1]));
+            const CGState h = L->slot[(it - 1) & 1];
+            if (h.done) {
+                fin = h;
+                fcg_iters = it - 1;
+                break;
+            }
+            hist.push_back(h.gamma);
+            const int w = L->opts.stall_window;
+            if (w > 0 && (int)hist.size() > w && h.gamma > L->opts.stall_ratio * hist[hist.size() - 1 - w]) {
+                stalled = true;
+                fcg_iters = it;
+                break;
+            }
+        }
+        if (it >= fmax) {  // no flexible iteration past fmax: settle with this iteration's check
+            EFS_HIP(hipEventSynchronize(L->ev[it & 1]));
+            fin = L->slot[it & 1];
+            fcg_iters = it;
+            stalled = !fin.done;
             break;
         }
-        if ((rc = apply_M(s, L, dt, s->cg_r, s->p_rr, 0, it % L->opts.every == 0))) return rc;
-        hipLaunchKernelGGL(k_fcg_dir, dim3(nbt), bt, 0, s->stream, dt, it, s->rowcode, L->z, s->cg_d0, s->cg_d1, L->q, s->st,
-                           L->p_zq, s->p_dad);
+        if ((rc = apply_M(s, L, dt, s->cg_r, nullptr, 0, it % L->opts.every == 0))) return rc;
+        if ((rc = efs_fcg_dir(s, dt, it, L->z, L->p_zq, L->q))) return rc;
         const bool fold = (it & 0x3F) == 0;
         hipLaunchKernelGGL(k_fcg_update, dim3(nbt), bt, 0, s->stream, dt, it, s->cg_d0, s->cg_d1, L->q, s->cg_r, s->cg_xd,
                            s->cg_x64, fold ? 1 : 0, s->st, L->p_rz, s->p_dad, s->p_rr);
         if (fold)
-            hipLaunchKernelGGL(k_fcg_residual, dim3(nbt), bt, 0, s->stream, dt, s->rowcode, s->div, s->cg_x64, s->cg_r, s->p_rr);
+            hipLaunchKernelGGL(k_fcg_residual, dim3(nbt), bt, 0, s->stream, dt, s->rowcode, s->div, s->cg_x64, s->cg_r, s->p_rr,
+                               s->st);
         EFS_HIP(hipGetLastError());
     }
-    CGState fin = *s->st_host;
-    int fcg_iters = it;
     if (stalled) {
         // plain CG continues from the current iterate: fold xd, then r = b - A x
         if ((rc = efs_cg_fold_all(s))) return rc;

@@ -251,19 +251,30 @@ __device__ inline void k1_load(K1Plane& q, int c, int last, const Stager<CG_TY>&
 // K1 (iteration it): convergence test, d_it = r_it + beta d_{it-1}, t = A d_it, partial d.t.
 // Iteration 0 has beta = 0 and stages d_0 = r_0 (k_cg_init stored it in d0 as well).  With a
 // preconditioner (p_rz given) `r` is z = M r and d_it = z_it + beta d_{it-1}; iteration 0 then
-// stores d_0 = z_0.
+// stores d_0 = z_0.  FLEX (flexible CG with a nonlinear preconditioner, Notay's FCG(1)): `r` is
+// z, beta = -(z_it . q_{it-1}) / (d_{it-1} . q_{it-1}) from the partials p_zq and st->dq, the
+// convergence test is a separate kernel, and t = A d_it is also stored in q for the update and
+// the next iteration's z . q (qout; p_zq and qout are unused otherwise).
+template <bool FLEX>
 __global__ __launch_bounds__(CG_NT) void k_cg_dir_spmv(Dom d, int it, const uint16_t* __restrict__ rcode, const float* __restrict__ r,
                                                        float* d0, float* d1, CGState* st, const double* p_rr,
-                                                       const double* p_bb, double* p_dad, const double* p_rz) {
+                                                       const double* p_bb, double* p_dad, const double* p_rz,
+                                                       const double* p_zq, float* __restrict__ qout) {
     if (st->done) return;
     __shared__ double red[CG_NT / 64];
-    double g_it;
-    if (cg_check(st, it, p_rr, p_bb, p_rz, red, g_it)) return;
-    const double beta = (it == 0) ? 0.0 : g_it / st->gam[(it + 1) & 1];
-    const bool first = (it == 0) && !p_rz;
+    double beta;
+    if constexpr (FLEX) {
+        const double zq = sum_partials(p_zq, gridDim.x, red);
+        beta = (it == 0 || st->dq == 0.0) ? 0.0 : -zq / st->dq;
+    } else {
+        double g_it;
+        if (cg_check(st, it, p_rr, p_bb, p_rz, red, g_it)) return;
+        beta = (it == 0) ? 0.0 : g_it / st->gam[(it + 1) & 1];
+    }
+    const bool first = !FLEX && (it == 0) && !p_rz;
     float* dnew = (it & 1) ? d1 : d0;
     // iteration 0: d_0 = r_0 (already in d0) or z_0 (staged from z itself, beta = 0)
-    const float* dsrc = (it == 0) ? (p_rz ? r : dnew) : ((it & 1) ? d0 : d1);
+    const float* dsrc = (it == 0) ? ((FLEX || p_rz) ? r : dnew) : ((it & 1) ? d0 : d1);
     __shared__ float sh[2][CG_TY + 2][TX + 2];
     const int tx = threadIdx.x, ty = threadIdx.y;
     auto Dv = [&](float dold, float rr) -> float { return (float)((double)dold * beta + (double)rr); };
@@ -307,6 +318,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir_spmv(Dom d, int it, const uint
                             t = apply_row(rc_cur, dc, sh[cur][ty + 1][tx], sh[cur][ty + 1][tx + 2], sh[cur][ty][tx + 1],
                                           sh[cur][ty + 2][tx + 1], np, dm);
                         if (!first) dnew[c] = dc;
+                        if constexpr (FLEX) qout[c] = t;
                         v += (double)dc * (double)t;
                     }
                     sh[cur ^ 1][ty + 1][tx + 1] = np;
@@ -915,8 +927,8 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
             const int i = launched + k;
             hipEvent_t a, b;
             efs_ktimer_pair(s, &a, &b);
-            hipExtLaunchKernelGGL(k_cg_dir_spmv, dim3(nbt), bt, 0, s->stream, a, b, 0, dt, i, s->rowcode, dsrc, s->cg_d0,
-                                  s->cg_d1, s->st, s->p_rr, s->p_bb, s->p_dad, p_rz);
+            hipExtLaunchKernelGGL(k_cg_dir_spmv<false>, dim3(nbt), bt, 0, s->stream, a, b, 0, dt, i, s->rowcode, dsrc, s->cg_d0,
+                                  s->cg_d1, s->st, s->p_rr, s->p_bb, s->p_dad, p_rz, (const double*)nullptr, (float*)nullptr);
             efs_ktimer_add(s, KT_CG_DIR_SPMV, a, b);
             // residual replacement every 64 iterations (CGSolver.cpp:444-451), including i = 0 as in the
             // reference; preconditioned CG, which has no reference schedule to follow, updates its
@@ -993,6 +1005,15 @@ int efs_cg_run_pc(efs_sim* s, efs_stats* out, bool border, int max_it, const CgP
 }
 
 int efs_cg_run(efs_sim* s, efs_stats* out, bool border, int max_it) { return efs_cg_run_pc(s, out, border, max_it, nullptr); }
+
+// Flexible CG's direction pass (the learned preconditioner's solve): d_it = z + beta d_{it-1},
+// q = A d_it, partials of d.q into p_dad; a no-op once st->done is set.
+int efs_fcg_dir(efs_sim* s, const Dom& dt, int it, const float* z, const double* p_zq, float* q) {
+    hipLaunchKernelGGL(k_cg_dir_spmv<true>, dim3(launch_blocks(dt)), dim3(TX, CG_TY), 0, s->stream, dt, it, s->rowcode, z, s->cg_d0,
+                       s->cg_d1, s->st, s->p_rr, s->p_bb, s->p_dad, nullptr, p_zq, q);
+    EFS_HIP(hipGetLastError());
+    return 0;
+}
 
 int efs_jacobi_solve(efs_sim* s, efs_stats* out) {
     int border = 0;

@@ -173,3 +173,6 @@ inline int blocks1d(int n) { return (n + 255) / 256; }
 
 }  // namespace efs_stencil
 using namespace efs_stencil;
+
+// flexible CG's direction pass (efs_solver.hip): d_it = z + beta d_{it-1}, q = A d_it, partials of d.q
+int efs_fcg_dir(efs_sim* s, const Dom& dt, int it, const float* z, const double* p_zq, float* q);

@@ -154,7 +154,9 @@ template <int NT, int ZT, typename TIN>
 __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, int Cup, const TIN* __restrict__ in, int Cin,
                                                const uint16_t* __restrict__ w, const float* __restrict__ bias, int Cout, int D,
                                                int H, int W, int relu, int ngroups, float* __restrict__ out_f,
-                                               uint16_t* __restrict__ out_b, uint16_t* __restrict__ out_pool) {
+                                               uint16_t* __restrict__ out_b, uint16_t* __restrict__ out_pool,
+                                               const int* __restrict__ skip) {
+    if (skip && *skip) return;
     using T = Tile<ZT>;
     constexpr int RW = T::RW, NFAM = 5;
     __shared__ uint4 tile[T::HS * 2];
@@ -334,7 +336,9 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, 
 // products per v_dot2 with FP32 accumulation.  The weights are uniform across the block and come
 // through the scalar path, so the LDS carries only the voxels' channels.
 __global__ __launch_bounds__(256) void k_conv3_ch0(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w, int Cout,
-                                                   const float* __restrict__ bias, int D, int H, int W, float* __restrict__ out0) {
+                                                   const float* __restrict__ bias, int D, int H, int W, float* __restrict__ out0,
+                                                   const int* __restrict__ skip) {
+    if (skip && *skip) return;
     using T = Tile<2>;
     __shared__ uint4 tile[T::HS * 2];
     int bx, by, bz;
@@ -637,6 +641,7 @@ struct efs_unet_s {
     std::vector<Conv> layers;
     size_t nparams = 0;
     hipStream_t stream = nullptr;
+    const int* skip = nullptr;  // device flag: the forward's convolutions return at once when it is set
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *frozen = nullptr;  // FP32 params / grads / moments
     uint16_t *wb = nullptr, *wt = nullptr;                                          // BF16 copies (fwd / transposed)
     bool wdirty = true;
@@ -733,7 +738,8 @@ int prep_weights(efs_unet_s* u) {
 // (k_conv3: NT output tiles per block, the rest in further block groups).
 template <typename TIN>
 void conv3(hipStream_t st, const TIN* in, int cin, const uint16_t* w, const float* bias, int cout, int D, int H, int W,
-           int relu, float* of, uint16_t* ob, const uint16_t* up = nullptr, int cup = 0, uint16_t* opool = nullptr) {
+           int relu, float* of, uint16_t* ob, const uint16_t* up = nullptr, int cup = 0, uint16_t* opool = nullptr,
+           const int* skip = nullptr) {
     const int ntiles = cout / 16;
     const int nxy = ((W + CT_X - 1) / CT_X) * ((H + CT_Y - 1) / CT_Y);
     // Up to 4 output tiles per block, which reuses each staged halo for more outputs; few voxel
@@ -747,7 +753,7 @@ void conv3(hipStream_t st, const TIN* in, int cin, const uint16_t* w, const floa
     const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, nz * ngroups);
 #define EFS_CONV3(nt, zt) \
     hipLaunchKernelGGL((k_conv3<nt, zt, TIN>), grid, dim3(256), 0, st, up, cup, in, cin, w, bias, cout, D, H, W, relu, ngroups, of, \
-                       ob, opool)
+                       ob, opool, skip)
     if (ZT == 1) EFS_CONV3(1, 1);
     else if (ZT == 4) EFS_CONV3(1, 4);
     else if (NT == 1) EFS_CONV3(1, 2);
@@ -764,11 +770,12 @@ void conv_fwd(efs_unet_s* u, const Conv& l, const TI* in, int D, int H, int W, f
     if constexpr (std::is_same<TI, uint16_t>::value) {
         if (ch0) {  // channel 0 only (inference output); the layer has 16 input channels
             const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, (D + 1) / 2);
-            hipLaunchKernelGGL(k_conv3_ch0, grid, dim3(256), 0, u->stream, in, u->wb + l.woff, l.cout, u->w + l.boff, D, H, W, of);
+            hipLaunchKernelGGL(k_conv3_ch0, grid, dim3(256), 0, u->stream, in, u->wb + l.woff, l.cout, u->w + l.boff, D, H, W, of,
+                               u->skip);
             return;
         }
     }
-    conv3(u->stream, in, l.cin, u->wb + l.woff, u->w + l.boff, l.cout, D, H, W, l.relu, of, ob, up, cup, opool);
+    conv3(u->stream, in, l.cin, u->wb + l.woff, u->w + l.boff, l.cout, D, H, W, l.relu, of, ob, up, cup, opool, u->skip);
 }
 
 // forward from the BF16 input already in u->inb, or from the FP32 input in32 (rounded to BF16
@@ -1054,12 +1061,14 @@ void* efs_unet_stream(efs_unet u) { return u ? (void*)u->stream : nullptr; }
 
 }  // extern "C"
 
-int efs_unet_forward_on(efs_unet u, hipStream_t stream, int D, int H, int W, const float** out16) {
+int efs_unet_forward_on(efs_unet u, hipStream_t stream, int D, int H, int W, const float** out16, const int* skip) {
     int rc = unet_alloc_dims(u, D, H, W);
     if (rc) return rc;
     hipStream_t saved = u->stream;
     u->stream = stream;
+    u->skip = skip;
     rc = forward(u, true);
+    u->skip = nullptr;
     u->stream = saved;
     if (rc) return rc;
     *out16 = u->out0;
