@@ -114,6 +114,30 @@ def test_learned_solve_untrained_converges(torch_cuda):
     sim.close()
 
 
+def test_learned_max_iters_hands_over_to_plain_cg(torch_cuda):
+    """With max_iters the flexible CG stops after that many iterations (the host reads the
+    convergence checks one iteration behind, so this bound is the one it settles synchronously)
+    and plain CG finishes the solve from the current iterate: the fallback flag is set, the
+    reported count includes the flexible iterations, and the FP64 true residual meets the
+    tolerance.  Stall detection is off, so only the bound can stop the flexible solve (an
+    untrained network alone would run to max_steps)."""
+    sc = scenes.sphere_pool(32, seed=0)
+    net = learned.UNet(seed=6)
+    for max_iters in (1, 3):
+        sim = HipSimulator3D(*sc.dims, params=gpu_params(sc, criterion=1, true_residual=1, max_steps=4000))
+        sim.load_scene(sc)
+        for s in range(6):
+            sim._stage(s)
+        learned.attach(sim, net, learned.default_opts(learned.out_scale_for(16), max_iters=max_iters, stall_window=0))
+        rc = sim.project()
+        st = sim.last_stats
+        print(f"[learned max_iters={max_iters}] rc={rc} its={st.iterations} fallback={st.fallback} "
+              f"true={st.true_rel_residual:.3e}")
+        assert rc == 0 and st.true_rel_residual < 1.5e-5
+        assert st.fallback == 1 and st.iterations > max_iters
+        sim.close()
+
+
 def test_learned_preconditioner_gain(torch_cuda):
     """Trained on plain-CG triples, the learned flexible CG needs fewer iterations than plain CG
     on held-out scenes, at the same FP64-verified relative residual."""
