@@ -40,6 +40,21 @@ def test_unet_forward_matches_oracle(torch_cuda):
     assert rel < 5e-2
 
 
+def test_unet_forward_matches_oracle_64(torch_cuda):
+    """At 64^3 the full-resolution layers run the many-block convolution (16 x 8 x 4 tiles, the
+    weights staged in LDS per block), which the 16^3 case does not reach."""
+    torch = torch_cuda
+    rng = np.random.default_rng(8)
+    net = learned.UNet(seed=9)
+    p = net.get_params().astype(np.float64)
+    x = _input(rng, 64)
+    y = net.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    yr = unet_ref.forward(p, x)[..., 0]
+    rel = np.linalg.norm(y - yr) / np.linalg.norm(yr)
+    print(f"[unet fwd 64] |y|={np.linalg.norm(yr):.3e} rel err (BF16 vs FP64) {rel:.3e}")
+    assert rel < 5e-2
+
+
 def test_unet_gradient_matches_oracle(torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(1)
