@@ -544,6 +544,15 @@ int efs_set_learned_preconditioner(efs_handle s, efs_unet net, const efs_learned
     L->net = net;
     if (opts) L->opts = *opts;
     else efs_default_learned_opts(&L->opts);
+    // allocate here, not in the first solve: the solver's vectors, its pinned check slots and
+    // the network's buffers for this coarse grid (a grid the network cannot take runs plain CG)
+    if (net && coarse_ok(s)) {
+        int rc = ensure_buffers(s, L);
+        if (rc) return rc;
+        const Coarse c = coarse_of(s);
+        uint16_t* inb = nullptr;
+        if ((rc = efs_unet_input_on(net, c.nz, c.ny, c.nx, &inb))) return rc;
+    }
     return 0;
 }
 

@@ -332,45 +332,83 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ up, 
 }
 
 // Output channel 0 only of a 16 -> Cout 3x3x3 convolution without activation (the U-Net's last
-// layer at inference, whose other output channels are unused): one voxel per thread, two BF16
-// products per v_dot2 with FP32 accumulation.  The weights are uniform across the block and come
-// through the scalar path, so the LDS carries only the voxels' channels.
+// layer at inference, whose other output channels are unused), on 16 x 8 x ZT output tiles.
+// out[z][y][x] = sum over (dz, dy) of Q_{dz,dy}[z + dz][y + dy][x], where for each of the halo's
+// (ZT + 2) x 10 voxel rows Q_{dz,dy}[row][x] = sum over dx and the 16 channels of w[dz][dy][dx] x[row][x + dx].
+// The 9 (dz, dy) combinations of one row are the rows of one MFMA product (A = weights, combos
+// x channels; B = the row's 16 voxels shifted by dx, channels x voxels; three products for dx =
+// 0..2 accumulate in FP32).  The partial sums go to LDS in place of the halo (9 x rows x 16
+// floats fit in the halo's size) and each output voxel adds 9 of them: for ZT = 2, 61 + 23 + 9 KB
+// of LDS traffic per tile instead of the 221 KB that 27 taps x 32 bytes per voxel read.
+constexpr int CH0_ZT = 4;  // planes per output tile
+template <int ZT>
 __global__ __launch_bounds__(256) void k_conv3_ch0(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w, int Cout,
                                                    const float* __restrict__ bias, int D, int H, int W, float* __restrict__ out0,
                                                    const int* __restrict__ skip) {
     if (skip && *skip) return;
-    using T = Tile<2>;
+    using T = Tile<ZT>;
+    constexpr int NROW = T::HZ * T::HY;  // halo rows
+    constexpr int RPW = NROW / 4;        // rows per wave
+    static_assert(NROW % 4 == 0, "the halo rows split evenly over the 4 waves");
+    static_assert(9 * NROW * CT_X * 4 <= T::HS * 2 * 16, "the partial sums must fit in the halo's LDS");
     __shared__ uint4 tile[T::HS * 2];
     int bx, by, bz;
     xcd_tile(bx, by, bz);
-    const int x0 = bx * CT_X, y0 = by * CT_Y, z0 = bz * 2;
+    const int x0 = bx * CT_X, y0 = by * CT_Y, z0 = bz * ZT;
     {
-        Halo<2> h;
-        halo_issue<2>(h, in, 16, 0, x0, y0, z0, D, H, W, false);
+        Halo<ZT> h;
+        halo_issue<ZT>(h, in, 16, 0, x0, y0, z0, D, H, W, false);
         halo_put(tile, h);
     }
-    __syncthreads();
-    const int tx = threadIdx.x & 15, ty = (threadIdx.x >> 4) & 7, tz = threadIdx.x >> 7;
-    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
-    if (x >= W || y >= H || z >= D) return;
-    float sum = 0.f;
-    // one plane of 9 taps per iteration: 72 SGPRs of weights in flight
-#pragma unroll 1
-    for (int dz = 0; dz < 3; dz++)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, q = lane >> 4;  // MFMA column (voxel) and k group (channels 4q..4q+3)
+    // A operands: row m = (dz, dy) = (m / 3, m % 3) for m < 9, zero rows above
+    bf16x4 a[3];
 #pragma unroll
-    for (int t9 = 0; t9 < 9; t9++) {
-        const int tap = dz * 9 + t9, dy = t9 / 3, dx = t9 % 3;
-        const int hv = ((tz + dz) * T::HY + (ty + dy)) * T::HX + (tx + dx);
-        const uint4 lo = tile[hv], hi = tile[T::HS + hv];  // the voxel's channels 0-7 and 8-15
-        const uint4* wt = reinterpret_cast<const uint4*>(w + (size_t)tap * Cout * 16);  // co 0: 16 channels
-        const uint4 wl = wt[0], wh = wt[1];
-        const unsigned xv[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-        const unsigned wv[8] = {wl.x, wl.y, wl.z, wl.w, wh.x, wh.y, wh.z, wh.w};
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-            sum = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, xv[k]), __builtin_bit_cast(bf16x2, wv[k]), sum, false);
+    for (int dx = 0; dx < 3; dx++) {
+        a[dx] = bf16x4{0, 0, 0, 0};
+        if (n < 9) {
+            const int tap = (n / 3) * 9 + (n % 3) * 3 + dx;
+            a[dx] = *reinterpret_cast<const bf16x4*>(w + (size_t)tap * Cout * 16 + q * 4);
+        }
     }
-    out0[((size_t)z * H + y) * W + x] = sum + (bias ? bias[0] : 0.f);
+    __syncthreads();
+    floatx4 acc[RPW];
+    const uint2* t2 = reinterpret_cast<const uint2*>(tile);
+    const int hoff = (q >> 1) * T::HS * 2 + (q & 1);  // the lane's 8 bytes within a voxel's two halves
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+        const int row = wave + 4 * i;
+        acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) {
+            const uint2 v = t2[hoff + (row * T::HX + n + dx) * 2];
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[dx], __builtin_bit_cast(bf16x4, v), acc[i], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // every wave has read the halo: the partial sums replace it
+    float* qs = reinterpret_cast<float*>(tile);
+    // D: lane (n, q) holds combos 4q + e of voxel n
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+        const int row = wave + 4 * i;
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (4 * q + e < 9) qs[((4 * q + e) * NROW + row) * CT_X + n] = acc[i][e];
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & 15, ty = (threadIdx.x >> 4) & 7;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= W || y >= H) return;
+#pragma unroll
+    for (int tz = threadIdx.x >> 7; tz < ZT; tz += 2) {
+        const int z = z0 + tz;
+        if (z >= D) return;
+        float sum = 0.f;
+#pragma unroll
+        for (int m = 0; m < 9; m++) sum += qs[(m * NROW + (tz + m / 3) * T::HY + ty + m % 3) * CT_X + tx];
+        out0[((size_t)z * H + y) * W + x] = sum + (bias ? bias[0] : 0.f);
+    }
 }
 
 // Weight gradient partials: block (tap, co tile, chunk) sums over its voxel rows.
@@ -769,8 +807,8 @@ void conv_fwd(efs_unet_s* u, const Conv& l, const TI* in, int D, int H, int W, f
               const uint16_t* up = nullptr, int cup = 0, uint16_t* opool = nullptr) {
     if constexpr (std::is_same<TI, uint16_t>::value) {
         if (ch0) {  // channel 0 only (inference output); the layer has 16 input channels
-            const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, (D + 1) / 2);
-            hipLaunchKernelGGL(k_conv3_ch0, grid, dim3(256), 0, u->stream, in, u->wb + l.woff, l.cout, u->w + l.boff, D, H, W, of,
+            const dim3 grid((W + CT_X - 1) / CT_X, (H + CT_Y - 1) / CT_Y, (D + CH0_ZT - 1) / CH0_ZT);
+            hipLaunchKernelGGL(k_conv3_ch0<CH0_ZT>, grid, dim3(256), 0, u->stream, in, u->wb + l.woff, l.cout, u->w + l.boff, D, H, W, of,
                                u->skip);
             return;
         }

@@ -229,7 +229,10 @@ def evaluate(net, out_scale, n, seeds, opts_kw=None, log=None):
     iterations, wall time of the solve, FP64 true residual and fallback, per scene."""
     from . import HipSimulator3D, Params
     rows = []
-    for seed in seeds:
+    # the first scene is solved once more, untimed, before the timed runs: the first launch of each
+    # kernel in the process loads its code, which would otherwise be charged to that scene
+    for i, seed in enumerate([seeds[0]] + list(seeds) if seeds else []):
+        warmup = i == 0
         sc = scenes.training_scene(n, seed=seed)
         row = {"seed": seed}
         for mode in ("plain", "learned"):
@@ -249,6 +252,8 @@ def evaluate(net, out_scale, n, seeds, opts_kw=None, log=None):
             row[mode] = {"its": st.iterations, "ms": 1e3 * (time.perf_counter() - t0), "true": st.true_rel_residual,
                          "status": st.status, "fallback": st.fallback}
             sim.close()
+        if warmup:
+            continue
         rows.append(row)
         if log:
             log(f"eval seed {seed}: plain {row['plain']['its']} its {row['plain']['ms']:.1f} ms | learned "
