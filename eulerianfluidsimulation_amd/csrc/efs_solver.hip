@@ -572,6 +572,33 @@ __global__ void k_cg_fold_all(int n, double* __restrict__ x64, float* __restrict
 // FP32 pressure, the rounding of the iterate (the work of k_cg_finalize after a preconditioned
 // solve, done in the same pass over the iterate; the launch domain is the whole grid).
 constexpr int TR_NT = 256;
+// One plane's loads for the true residual: the row code, b and the x of the cell's x, y and +z
+// neighbours (the cell's own x and its -z neighbour's are the previous planes' +z values).
+struct TRPlane {
+    uint16_t rc;
+    double b, xm, xp, ym, yp, zp;
+};
+template <bool XD>
+__device__ inline void tr_load(TRPlane& p, const Dom& d, int x, int y, int z, const uint16_t* __restrict__ rcode,
+                               const float* __restrict__ b, const float* __restrict__ xf, const double* __restrict__ xdbl) {
+    const int zz = min(z, d.D - 1);  // a plane past the chunk: loaded from a valid plane, never used
+    const int c = (zz * d.H + y) * d.W + x;
+    const Nb n = nb_offsets(d, x, y, zz);
+    auto X = [&](int q) -> double {
+        if constexpr (XD) return xdbl[q];
+        else return (double)xf[q];
+    };
+    p.rc = rcode[c];
+    p.b = (double)b[c];
+    p.xm = X(c + n.xm);
+    p.xp = X(c + n.xp);
+    p.ym = X(c + n.ym);
+    p.yp = X(c + n.yp);
+    p.zp = X(c + n.zp);
+}
+// Two planes of loads stay in flight (slots s0 and s1, used alternately): one plane per wavefront
+// left the pass waiting on every plane's loads.
+template <bool XD>
 __global__ __launch_bounds__(TR_NT) void k_true_residual(Dom d, const uint16_t* __restrict__ rcode, const float* __restrict__ b,
                                                          const float* __restrict__ xf, const double* __restrict__ xdbl,
                                                          double* partials, float* __restrict__ pout) {
@@ -581,22 +608,34 @@ __global__ __launch_bounds__(TR_NT) void k_true_residual(Dom d, const uint16_t* 
         const int x = d.x0 + bx * TX + threadIdx.x, y = d.y0 + by * d.ty + threadIdx.y;
         const int zb = d.z0 + bz * d.zc, ze = min(zb + d.zc, d.z0 + d.nz);
         if (x < d.x0 + d.nx && y < d.y0 + d.ny && zb < ze) {
-            auto X = [&](int q) -> double { return xdbl ? xdbl[q] : (double)xf[q]; };
+            auto X = [&](int q) -> double {
+                if constexpr (XD) return xdbl[q];
+                else return (double)xf[q];
+            };
             // the cell's value and its -z neighbour's are the last plane's cell and +z values
             const int c0 = (zb * d.H + y) * d.W + x;
             double xc = X(c0), xzm = X(c0 + nb_offsets(d, x, y, zb).zm);
-            for (int z = zb; z < ze; z++) {
+            TRPlane s0, s1;
+            tr_load<XD>(s0, d, x, y, zb, rcode, b, xf, xdbl);
+            tr_load<XD>(s1, d, x, y, zb + 1, rcode, b, xf, xdbl);
+            auto step = [&](const TRPlane& p, int z) {
                 const int c = (z * d.H + y) * d.W + x;
-                const Nb n = nb_offsets(d, x, y, z);
-                const double xzp = X(c + n.zp);
-                const double t = apply_row_d(rcode[c], xc, X(c + n.xm), X(c + n.xp), X(c + n.ym), X(c + n.yp), xzp, xzm);
-                const double rr = (double)b[c] - t;
+                const double t = apply_row_d(p.rc, xc, p.xm, p.xp, p.ym, p.yp, p.zp, xzm);
+                const double rr = p.b - t;
                 v[0] += rr * rr;
-                v[1] += (double)b[c] * (double)b[c];
+                v[1] += p.b * p.b;
                 if (pout) pout[c] = (float)xc;
                 xzm = xc;
-                xc = xzp;
+                xc = p.zp;
+            };
+            int z = zb;
+            for (; z + 1 < ze; z += 2) {
+                step(s0, z);
+                tr_load<XD>(s0, d, x, y, z + 2, rcode, b, xf, xdbl);
+                step(s1, z + 1);
+                tr_load<XD>(s1, d, x, y, z + 3, rcode, b, xf, xdbl);
             }
+            if (z < ze) step(s0, z);
         }
     }
     __shared__ double red[2][TR_NT / 64];
@@ -816,8 +855,12 @@ int efs_true_residual(efs_sim* s, double* num, double* den, bool write_p) {
     static_assert(TX * 4 == TR_NT, "dom_simple tiles are TX x 4");
     const int nb = launch_blocks(d);
     write_p = write_p && s->p64_valid;
-    hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(TX, d.ty), 0, s->stream, d, s->rowcode, s->div, s->p,
-                       s->p64_valid ? s->cg_x64 : nullptr, s->partials, write_p ? s->p : nullptr);
+    if (s->p64_valid)
+        hipLaunchKernelGGL(k_true_residual<true>, dim3(nb), dim3(TX, d.ty), 0, s->stream, d, s->rowcode, s->div, s->p, s->cg_x64,
+                           s->partials, write_p ? s->p : nullptr);
+    else
+        hipLaunchKernelGGL(k_true_residual<false>, dim3(nb), dim3(TX, d.ty), 0, s->stream, d, s->rowcode, s->div, s->p,
+                           (const double*)nullptr, s->partials, (float*)nullptr);
     hipLaunchKernelGGL(k_true_residual_sum, dim3(1), dim3(TR_NT), 0, s->stream, nb, s->partials, s->st);
     EFS_HIP(hipGetLastError());
     EFS_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
