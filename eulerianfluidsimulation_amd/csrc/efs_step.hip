@@ -111,12 +111,16 @@ __global__ __launch_bounds__(256) void k_extrap_init(DevGrid g, uint8_t* __restr
     FULL_XYZ(g);
     const bool interior = x > 0 && y > 0 && z > 0 && x < g.W - 1 && y < g.H - 1 && z < g.D - 1;
     const float r = g.rho[c];
-    if (DECAY && interior && r < 0.5f) {
-        g.u[c] = g.u[c] * decay;
-        g.v[c] = g.v[c] * decay;
-        g.w[c] = g.w[c] * decay;
-    }
     const bool solid = (g.bf[c] & BF_EQ1) != 0;
+    if constexpr (DECAY) {
+        // loaded with rho, not after its test: most cells of a typical state are air and decay
+        const float uc = g.u[c], vc = g.v[c], wc = g.w[c];
+        if (interior && r < 0.5f) {
+            g.u[c] = uc * decay;
+            g.v[c] = vc * decay;
+            g.w[c] = wc * decay;
+        }
+    }
     stamp[c] = solid ? 254 : ((interior && r >= 0.5f) ? 0 : 255);
 }
 
