@@ -3,7 +3,7 @@
 // Layout: activations channels-last (NDHWC), channels a multiple of 16.  Convolutions are
 // implicit GEMMs on the matrix cores with BF16 operands and FP32 accumulation: the forward and
 // backward-data convolutions stage each block's input halo in LDS and run gfx950's
-// v_mfma_f32_16x16x32_bf16 (k_conv3); the weight gradients use v_mfma_f32_16x16x16_bf16.
+// v_mfma_f32_16x16x32_bf16 (k_conv3), and so do the weight gradients (k_conv_wgrad).
 //   forward        y[v][co] = b[co] + sum_tap sum_ci x[v + off(tap)][ci] * W[tap][co][ci]
 //   backward data  dx = the same kernel on dy with W flipped in space and transposed in channels
 //   backward W     dW[tap][co][ci] = sum_v dy[v][co] * x[v + off(tap)][ci]: per-block partials,
@@ -33,7 +33,6 @@ __device__ inline uint16_t f2bf(float f) {  // round to nearest even
 }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-constexpr int MAXNT = 8;  // up to 128 output channels per wavefront (weight gradients)
 
 // ---- 3x3x3 convolution (zero padding) as an implicit GEMM on v_mfma_f32_16x16x32_bf16 --------
 // A 256-thread block computes a 16 x 8 x ZT tile of output voxels (ZT = 1, 2 or 4) for NT
@@ -411,66 +410,168 @@ __global__ __launch_bounds__(256) void k_conv3_ch0(const uint16_t* __restrict__ 
     }
 }
 
-// Weight gradient partials: block (tap, co tile, chunk) sums over its voxel rows.
-// dW[tap][co][ci] = sum_v dy[v][co] * x[v + off][ci]; A = dy^T (co x voxel), B = x (voxel x ci).
+// Weight gradient partials of a 3x3x3 convolution: dW[tap][co][ci] = sum_v dy[v][co] x[v + off(tap)][ci]
+// on v_mfma_f32_16x16x32_bf16 with the voxels as the K dimension (D = co x ci).  A block takes one
+// 16-channel group of co and of ci and a strided share of the 16 x 8 x 2 voxel tiles; its 4 waves
+// take the 27 taps in turn (tap = wave + 4 j) and keep their sums in registers over all its tiles.
+// Per tile, dy and the input halo are staged in LDS channel-major (each channel a run of 16 voxels
+// per row), the halo in three copies shifted by dx = 0, 1, 2, so that every operand of every tap
+// is one aligned 16-byte LDS read: a lane's 8 K values are 8 voxels along x of one row and one
+// channel.  Each block writes its sums as partial `b` (part[b][tap][co][ci]) for the fixed-order
+// reduction; the blocks of the first ci group also sum their dy tiles per channel (the bias
+// gradient), as partial b of bpart (bpart[b][co]).
+constexpr int WG_ZT = 2;                               // planes per voxel tile
+constexpr int WG_ROWS = CT_Y * WG_ZT;                  // 16 voxel rows of 16 per tile
+constexpr int WG_HROWS = (CT_Y + 2) * (WG_ZT + 2);     // 40 halo rows
+constexpr int WG_DYS = WG_ROWS * 16 + 8;               // BF16 elements per co (16 bytes of padding:
+constexpr int WG_XS = WG_HROWS * 16 + 8;               // the 16 channels of a read fall in different banks)
+constexpr int WG_TAPS = 7;                             // taps per wave (27 = 4 waves x 7, the last 6)
+
+// dword of channel c (0..7) of two voxels' 8-channel halves: voxel a in the low half, b in the high
+__device__ inline unsigned pair_ch(const uint4& a, const uint4& b, int c) {
+    const unsigned wa = c < 2 ? a.x : (c < 4 ? a.y : (c < 6 ? a.z : a.w));
+    const unsigned wb = c < 2 ? b.x : (c < 4 ? b.y : (c < 6 ? b.z : b.w));
+    return (c & 1) ? ((wa >> 16) | (wb & 0xFFFF0000u)) : ((wa & 0xFFFFu) | (wb << 16));
+}
+
 template <typename TX>
-__global__ __launch_bounds__(64) void k_conv_wgrad(const uint16_t* __restrict__ dy, int Cout, const TX* __restrict__ x, int Cin,
-                                                   int D, int H, int W, int ks, int nchunks, float* __restrict__ part) {
-    const int lane = threadIdx.x;
-    const int ntap = ks == 3 ? 27 : 1;
-    const int tap = blockIdx.x % ntap;
-    const int cot = (blockIdx.x / ntap) % (Cout >> 4);
-    const int chunk = blockIdx.x / (ntap * (Cout >> 4));
-    const int dz = ks == 3 ? tap / 9 - 1 : 0, dyy = ks == 3 ? (tap / 3) % 3 - 1 : 0, dx = ks == 3 ? tap % 3 - 1 : 0;
-    const int row = lane & 15, kq = lane >> 4;
-    const int nci = Cin >> 4;
-    floatx4 acc[MAXNT];
+__global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__ dy, int Cout, const TX* __restrict__ x, int Cin,
+                                                    int D, int H, int W, int nper, float* __restrict__ part,
+                                                    float* __restrict__ bpart) {
+    __shared__ uint4 dyT4[16 * WG_DYS / 8];      // [co][row][16 voxels]
+    __shared__ uint4 xT4[3 * 16 * WG_XS / 8];    // [dx][ci][halo row][i]: the halo at column i + dx
+    unsigned* dyT32 = reinterpret_cast<unsigned*>(dyT4);
+    unsigned* xT32 = reinterpret_cast<unsigned*>(xT4);
+    const uint16_t* dyT = reinterpret_cast<const uint16_t*>(dyT4);
+    const uint16_t* xT = reinterpret_cast<const uint16_t*>(xT4);
+    const int ncg = Cout >> 4;
+    const int g = blockIdx.x / nper, b = blockIdx.x % nper;
+    const int cog = g % ncg, cig = g / ncg;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n16 = lane & 15, kq = lane >> 4;
+    const int tiles_x = (W + 15) / 16, tiles_y = (H + CT_Y - 1) / CT_Y, tiles_z = (D + WG_ZT - 1) / WG_ZT;
+    const int ntiles = tiles_x * tiles_y * tiles_z;
+    const __amdgpu_buffer_rsrc_t rdy = bufres(dy, (unsigned)((size_t)D * H * W * Cout * 2));
+    const __amdgpu_buffer_rsrc_t rx = bufres(x, (unsigned)((size_t)D * H * W * Cin * sizeof(TX)));
+    floatx4 acc[WG_TAPS];
 #pragma unroll
-    for (int i = 0; i < MAXNT; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int tiles_x = (W + 15) >> 4;
-    const long long ntiles = (long long)tiles_x * H * D;
-    for (long long t = chunk; t < ntiles; t += nchunks) {
-        const int tx = (int)(t % tiles_x);
-        const long long rest = t / tiles_x;
-        const int y = (int)(rest % H), z = (int)(rest / H);
-        // the 4 voxels (k = kq*4 + e) this lane supplies
-        bf16x4 a;
-        int src[4];
-        bool okx[4];
+    for (int j = 0; j < WG_TAPS; j++) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float bacc = 0.f;  // bias: thread (co = tid / 16, row = tid % 16) of the first ci group
+    // The next tile's global loads are issued into registers before the current tile's MFMAs
+    // (LDS, not registers, limits the blocks per CU).  dy: thread = (row, half of the 16 channels,
+    // voxel pair along x).  Halo: task = (halo row, half, pair j) for tasks tid, tid + 256 and
+    // tid + 512 (< 640, waves 0 and 1 only), halo columns 2j .. 2j + 3 (global x0 - 1 + ...).
+    constexpr int NTASK = WG_HROWS * 16, KT = (NTASK + 255) / 256;
+    constexpr bool F32 = std::is_same<TX, float>::value;
+    constexpr int NL = F32 ? 2 : 1;  // 16-byte loads per voxel half
+    uint4 qd[2];
+    uint4 qh[KT][4][NL];
+    auto issue = [&](int t) {
+        const int x0 = (t % tiles_x) * 16, y0 = ((t / tiles_x) % tiles_y) * CT_Y, z0 = (t / (tiles_x * tiles_y)) * WG_ZT;
+        {
+            const int j = tid & 7, h = (tid >> 3) & 1, row = tid >> 4;
+            const int gy = y0 + row % CT_Y, gz = z0 + row / CT_Y;
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int xv = tx * 16 + kq * 4 + e;
-            const bool inb = xv < W;
-            const long long v = (((long long)z * H + y) * W + xv);
-            a[e] = inb ? (short)dy[v * Cout + cot * 16 + row] : (short)0;
-            const int xs = xv + dx, ys = y + dyy, zs = z + dz;
-            okx[e] = inb && xs >= 0 && xs < W && ys >= 0 && ys < H && zs >= 0 && zs < D;
-            src[e] = okx[e] ? (int)(((long long)zs * H + ys) * W + xs) : 0;
+            for (int e = 0; e < 2; e++) {
+                const int gx = x0 + 2 * j + e;
+                const bool in = gx < W && gy < H && gz < D;
+                const unsigned off = in ? (unsigned)(((((size_t)gz * H + gy) * W + gx) * Cout + cog * 16 + h * 8) * 2) : OOB;
+                qd[e] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rdy, off, 0, 0));
+            }
         }
 #pragma unroll
-        for (int n = 0; n < MAXNT; n++) {
-            if (n >= nci) break;
-            bf16x4 b;
+        for (int k = 0; k < KT; k++) {
+            const int task = tid + 256 * k;
+            if (task >= NTASK) break;  // wave-uniform
+            const int j = task & 7, h = (task >> 3) & 1, hrow = task >> 4;
+            const int gy = y0 - 1 + hrow % (CT_Y + 2), gz = z0 - 1 + hrow / (CT_Y + 2);
+            const bool rowin = gy >= 0 && gy < H && gz >= 0 && gz < D;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                float xv = 0.f;
-                if (okx[e]) {
-                    if constexpr (sizeof(TX) == 2) xv = bf2f(x[(size_t)src[e] * Cin + n * 16 + row]);
-                    else xv = x[(size_t)src[e] * Cin + n * 16 + row];
-                }
-                b[e] = (short)f2bf(xv);
+                const int gx = x0 - 1 + 2 * j + e;
+                const bool in = rowin && gx >= 0 && gx < W;
+                const unsigned off =
+                    in ? (unsigned)(((((size_t)gz * H + gy) * W + gx) * Cin + cig * 16 + h * 8) * sizeof(TX)) : OOB;
+#pragma unroll
+                for (int l = 0; l < NL; l++)
+                    qh[k][e][l] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rx, off, 16 * l, 0));
             }
-            acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+        }
+    };
+    auto commit = [&]() {
+        {
+            const int j = tid & 7, h = (tid >> 3) & 1, row = tid >> 4;
+#pragma unroll
+            for (int c = 0; c < 8; c++) dyT32[((8 * h + c) * WG_DYS + row * 16 + 2 * j) / 2] = pair_ch(qd[0], qd[1], c);
+        }
+#pragma unroll
+        for (int k = 0; k < KT; k++) {
+            const int task = tid + 256 * k;
+            if (task >= NTASK) break;
+            const int j = task & 7, h = (task >> 3) & 1, hrow = task >> 4;
+            uint4 q[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                if constexpr (F32) {
+                    const f32x4 lo = __builtin_bit_cast(f32x4, qh[k][e][0]), hi = __builtin_bit_cast(f32x4, qh[k][e][NL - 1]);
+                    q[e] = make_uint4(pk_bf16(lo.x, lo.y), pk_bf16(lo.z, lo.w), pk_bf16(hi.x, hi.y), pk_bf16(hi.z, hi.w));
+                } else {
+                    q[e] = qh[k][e][0];
+                }
+            }
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+                    xT32[((dx * 16 + 8 * h + c) * WG_XS + hrow * 16 + 2 * j) / 2] = pair_ch(q[dx], q[dx + 1], c);
+        }
+    };
+    if (b < ntiles) issue(b);
+    for (int t = b; t < ntiles; t += nper) {
+        __syncthreads();  // every wave is done with the previous tile
+        commit();
+        __syncthreads();
+        if (t + nper < ntiles) issue(t + nper);
+        if (cig == 0) {
+            const uint16_t* rowp = dyT + (tid >> 4) * WG_DYS + (tid & 15) * 16;
+#pragma unroll
+            for (int k = 0; k < 16; k++) bacc += bf2f(rowp[k]);
+        }
+        // K step p: output rows 2p (lanes kq 0, 1) and 2p + 1 (kq 2, 3), voxels 8 (kq & 1) .. + 7
+#pragma unroll 2
+        for (int p = 0; p < WG_ROWS / 2; p++) {
+            const int r = 2 * p + (kq >> 1), xo = (kq & 1) * 8;
+            const int ty = r % CT_Y, tz = r / CT_Y;
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(dyT + n16 * WG_DYS + r * 16 + xo);
+#pragma unroll
+            for (int j = 0; j < WG_TAPS; j++) {
+                const int tap = wave + 4 * j;
+                if (tap >= 27) break;  // wave-uniform
+                const int dz = tap / 9, dyy = (tap / 3) % 3, dx = tap % 3;
+                const int hrow = (tz + dz) * (CT_Y + 2) + ty + dyy;
+                const bf16x8 bb = *reinterpret_cast<const bf16x8*>(xT + (dx * 16 + n16) * WG_XS + hrow * 16 + xo);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[j], 0, 0, 0);
+            }
         }
     }
-    // D[i = co][j = ci]: lane j + 16 * (i / 4), element i % 4
+    // D[co = 4 kq + i][ci = n16] of each tap
 #pragma unroll
-    for (int n = 0; n < MAXNT; n++) {
-        if (n >= nci) break;
+    for (int j = 0; j < WG_TAPS; j++) {
+        const int tap = wave + 4 * j;
+        if (tap >= 27) break;
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int co = cot * 16 + kq * 4 + e, ci = n * 16 + row;
-            part[(((size_t)chunk * ntap + tap) * Cout + co) * Cin + ci] = acc[n][e];
+        for (int i = 0; i < 4; i++)
+            part[(((size_t)b * 27 + tap) * Cout + cog * 16 + 4 * kq + i) * Cin + cig * 16 + n16] = acc[j][i];
+    }
+    if (cig == 0) {
+        __syncthreads();  // the last tile's dy is no longer read: its LDS takes the row sums
+        float* red = reinterpret_cast<float*>(dyT4);
+        red[tid] = bacc;
+        __syncthreads();
+        if (tid < 16) {
+            float a = 0.f;
+            for (int k = 0; k < 16; k++) a += red[tid * 16 + k];
+            bpart[(size_t)b * Cout + cog * 16 + tid] = a;
         }
     }
 }
@@ -483,27 +584,18 @@ __global__ void k_reduce_chunks(int n, int nchunks, const float* __restrict__ pa
     out[i] = s;
 }
 
-// Bias gradient g[co] = sum_v dy[v][co] in two deterministic passes: block b sums voxels
-// [b * per, (b + 1) * per) for every channel (thread t: channel t % C, every R-th voxel, R =
-// 256 / C; coalesced), the R sums of a channel added in order into part[b][co]; then
-// k_reduce_chunks adds the blocks' partials in block order.
-constexpr int BG_BLOCKS = 512;
-__global__ __launch_bounds__(256) void k_bias_part(const uint16_t* __restrict__ dy, int C, long long nvox, long long per,
-                                                   float* __restrict__ part) {
-    __shared__ float red[256];
-    const int R = 256 / C, t = threadIdx.x, c = t % C, r = t / C;
-    const long long v0 = (long long)blockIdx.x * per, v1 = min(nvox, v0 + per);
+// First pass of a two-pass fixed-order sum over chunks: block (x, seg) adds chunks
+// [seg * nchunks / nseg, (seg + 1) * nchunks / nseg) in order into tmp[seg]; k_reduce_chunks then
+// adds the nseg segment sums in order.
+__global__ void k_reduce_seg(int n, int nchunks, int nseg, const float* __restrict__ part, float* __restrict__ tmp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, seg = blockIdx.y;
+    if (i >= n) return;
+    const int c0 = (int)((long long)seg * nchunks / nseg), c1 = (int)((long long)(seg + 1) * nchunks / nseg);
     float s = 0.f;
-    if (r < R)
-        for (long long v = v0 + r; v < v1; v += R) s += bf2f(dy[v * C + c]);
-    red[t] = s;
-    __syncthreads();
-    if (t < C) {
-        float a = 0.f;
-        for (int k = 0; k < R; k++) a += red[k * C + t];
-        part[(size_t)blockIdx.x * C + t] = a;
-    }
+    for (int c = c0; c < c1; c++) s += part[(size_t)c * n + i];
+    tmp[(size_t)seg * n + i] = s;
 }
+constexpr int RED_SEG = 32;  // segments of the chunk sums
 
 // dpre = dy * (y > 0) (y is the post-ReLU activation), as BF16 MFMA operand
 __global__ void k_relu_bwd(long long n, const float* __restrict__ dy, const uint16_t* __restrict__ y, int relu,
@@ -853,28 +945,42 @@ int forward(efs_unet_s* u, bool compact = false, const float* in32 = nullptr, fl
     return 0;
 }
 
+// out[i] = sum over c of part[c * n + i] (i < n) in a fixed order, in two passes; part must have
+// room for RED_SEG * n more floats after its nchunks * n.
+void reduce_chunks(efs_unet_s* u, int n, int nchunks, float* part, float* out) {
+    const int nseg = std::min(nchunks, RED_SEG);
+    float* tmp = part + (size_t)nchunks * n;
+    hipLaunchKernelGGL(k_reduce_seg, dim3(nb(n), nseg), dim3(256), 0, u->stream, n, nchunks, nseg, part, tmp);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(nb(n)), dim3(256), 0, u->stream, n, nseg, tmp, out);
+}
+
 // Backward of one conv layer given dL/dy (FP32, post-activation gradient) in `gy`:
 // weight/bias gradients accumulate into u->g; if gx != null, dL/dx is written (not accumulated).
 template <typename TX>
 int conv_bwd(efs_unet_s* u, const Conv& l, const TX* x, const uint16_t* y_act, const float* gy, int D, int H, int W, float* gx) {
     const long long nvox = (long long)D * H * W;
     hipLaunchKernelGGL(k_relu_bwd, dim3(nb(nvox * l.cout)), dim3(256), 0, u->stream, nvox * l.cout, gy, y_act, l.relu, u->dpre);
-    const int ntap = l.ks == 3 ? 27 : 1;
-    const long long tiles = (long long)((W + 15) / 16) * H * D;
-    const int nchunks = (int)std::max<long long>(1, std::min<long long>(64, tiles / 8));
-    // the weight partials, then (after their reduction, in stream order) the bias partials
-    const size_t need = std::max((size_t)nchunks * l.nw(), (size_t)BG_BLOCKS * l.cout);
+    if (l.ks != 3) {
+        efs_set_error("U-Net weight gradients are implemented for 3x3x3 layers only");
+        return -1;
+    }
+    // about 1024 blocks over the channel groups, each with at least one voxel tile
+    const long long tiles = (long long)((W + 15) / 16) * ((H + CT_Y - 1) / CT_Y) * ((D + WG_ZT - 1) / WG_ZT);
+    const int groups = (l.cout / 16) * (l.cin / 16);
+    const int nchunks = (int)std::max<long long>(1, std::min<long long>(tiles, (1024 + groups - 1) / groups));
+    // the weight partials and the bias partials, each followed by room for the segment sums of
+    // its reduction
+    const size_t wn = (size_t)(nchunks + RED_SEG) * l.nw(), need = wn + (size_t)(nchunks + RED_SEG) * l.cout;
     if (need > u->wpart_n) {
         if (u->wpart) hipFree(u->wpart);
         UHIP(hipMalloc(&u->wpart, need * 4));
         u->wpart_n = need;
     }
-    hipLaunchKernelGGL(k_conv_wgrad<TX>, dim3(ntap * (l.cout / 16) * nchunks), dim3(64), 0, u->stream, u->dpre, l.cout, x, l.cin,
-                       D, H, W, l.ks, nchunks, u->wpart);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(nb(l.nw())), dim3(256), 0, u->stream, (int)l.nw(), nchunks, u->wpart, u->g + l.woff);
-    const long long per = (nvox + BG_BLOCKS - 1) / BG_BLOCKS;
-    hipLaunchKernelGGL(k_bias_part, dim3(BG_BLOCKS), dim3(256), 0, u->stream, u->dpre, l.cout, nvox, per, u->wpart);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(nb(l.cout)), dim3(256), 0, u->stream, l.cout, BG_BLOCKS, u->wpart, u->g + l.boff);
+    float* bpart = u->wpart + wn;
+    hipLaunchKernelGGL(k_conv_wgrad<TX>, dim3(groups * nchunks), dim3(256), 0, u->stream, u->dpre, l.cout, x, l.cin, D, H, W,
+                       nchunks, u->wpart, bpart);
+    reduce_chunks(u, (int)l.nw(), nchunks, u->wpart, u->g + l.woff);
+    reduce_chunks(u, l.cout, nchunks, bpart, u->g + l.boff);
     if (gx)  // dx = conv(dpre, W flipped / transposed), no bias, no activation
         conv3(u->stream, u->dpre, l.cout, u->wt + l.woff, nullptr, l.cin, D, H, W, 0, gx, nullptr);
     UHIP(hipGetLastError());
