@@ -322,7 +322,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=192)
     ap.add_argument("--pairs", type=int, default=2000)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=16000)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--eval", type=int, default=6, help="held-out scenes evaluated after training")
     ap.add_argument("--out", default=None, help="where to save the trained network (.npz)")
