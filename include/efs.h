@@ -268,7 +268,11 @@ typedef struct {
 } efs_learned_opts;
 void efs_default_learned_opts(efs_learned_opts* o);
 /* Attaches a network (owned by the caller) as the preconditioner used when params.precond is
- * EFS_PRECOND_LEARNED. */
+ * EFS_PRECOND_LEARNED.  When the grid's interior is a multiple of 16 per axis it also allocates
+ * the flexible CG's vectors and pinned check slots and sizes the network's buffers for the coarse
+ * grid, so the solves allocate nothing; a failed allocation returns its error code here.  The
+ * network's buffers follow the last grid it was sized for: a network shared by simulators of
+ * different sizes is resized (allocated again) whenever a solve switches it to another grid. */
 int efs_set_learned_preconditioner(efs_handle h, efs_unet net, const efs_learned_opts* opts);
 /* Half-resolution dims (interior / 2; the interior must be a multiple of 16 per axis). */
 int efs_learned_coarse_dims(efs_handle h, int32_t dims[3]);
