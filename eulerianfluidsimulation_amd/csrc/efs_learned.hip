@@ -87,14 +87,25 @@ __global__ __launch_bounds__(CG_NT) void k_restrict(Coarse cg, const uint16_t* _
     const int cx = (int)(i % cg.nx), cy = (int)((i / cg.nx) % cg.ny), cz = (int)(i / ((long long)cg.nx * cg.ny));
     float s = 0.f;
     int nf = 0, ns = 0;
+    // all 24 loads first, then the tests: a load after its row code's test would be a second
+    // round trip (v of a cell that is not a CG row is loaded and not used)
+    unsigned rcs[8], bfs[8];
+    float vs[8];
+#pragma unroll
     for (int k = 0; k < 8; k++) {
         const int x = 1 + 2 * cx + (k & 1), y = 1 + 2 * cy + ((k >> 1) & 1), z = 1 + 2 * cz + (k >> 2);
         const int c = (z * cg.H + y) * cg.W + x;
-        if (rcode[c] & RC_CG_ROW) {
-            s += v[c];
+        rcs[k] = rcode[c];
+        bfs[k] = bf[c];
+        vs[k] = v[c];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (rcs[k] & RC_CG_ROW) {
+            s += vs[k];
             nf++;
         }
-        if (bf[c] & BF_EQ1) ns++;
+        if (bfs[k] & BF_EQ1) ns++;
     }
     const float inv = (float)(1.0 / (8.0 * sigma));
     const float v0 = s * inv, v1 = nf * 0.125f, v2 = ns * 0.125f;  // channels 3-15 are 0
